@@ -1,0 +1,152 @@
+/*
+ * mappraiser_gpu.h -- C-ABI of libmappraiser_gpu.so, the MI355X (gfx950) GLS
+ * map-making hot path of MAPPRAISER/MIDAPACK for one rank's data on one GPU.
+ *
+ * The library replaces everything to the right of the PCG_GLS_true call in
+ * MLmap (mappraiser/src/mappraiser.c:210-211): the pointing products, the
+ * block-Toeplitz inverse-noise product, the block-Jacobi preconditioner with
+ * its degenerate-pixel cut, and the PCG iteration.  All entry points
+ *   - return an int status (MGPU_OK = 0) and never call exit();
+ *   - copy from / into caller buffers and never keep pointers to them;
+ *   - are stream-ordered on the handle's own HIP stream: calls on different
+ *     handles may run concurrently; calls on one handle must not overlap.
+ * Plain C types only: int / int32 indices, double data, as the reference's
+ * ctypes wrapper passes them (mappraiser_wrapper/__init__.py:23-28, 58-84).
+ *
+ * Pointing layout: m samples x nnz=3 entries, pix[3t+k] = 3p+k (global pixel
+ * p, mappraiser.py:1214-1216); flagged samples carry -3,-2,-1
+ * (utils.py:148-150).  Toeplitz blocks are contiguous and block b uses
+ * invtt[b*lambda .. b*lambda+lambda) (create_toeplitz.c:27-47).
+ */
+#ifndef MAPPRAISER_GPU_H
+#define MAPPRAISER_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MGPU_OK 0
+#define MGPU_EINVAL 1  /* invalid or unsupported input (message in last_error) */
+#define MGPU_EHIP 2    /* HIP runtime error */
+#define MGPU_ENOMEM 3  /* device allocation failed */
+#define MGPU_ESTATE 4  /* call out of order (e.g. precond_apply before precond_build) */
+
+/* ---- precision modes ---- */
+#define MGPU_FP64 0    /* every kernel in FP64 (parity mode) */
+#define MGPU_MIXED 1   /* FP32 Toeplitz FFT + P/Pt, FP64 dots, 3x3 blocks and periodic FP64 residual */
+
+typedef struct mgpu_problem mgpu_problem;
+
+typedef struct mgpu_options {
+    int precision;      /* MGPU_FP64 or MGPU_MIXED */
+    int refresh_every;  /* MGPU_MIXED: recompute the FP64 residual every k iterations (default 10) */
+    int tile_log2;      /* real FFT tile length 2^k of the Toeplitz product; 0 = autotuned choice */
+    int poll_every;     /* PCG: iterations launched between host convergence polls (default 8) */
+    int reserved[4];
+} mgpu_options;
+
+typedef struct mgpu_report {
+    int    iterations;  /* value of the loop counter k when PCG stopped (maxiter if not converged) */
+    int    converged;   /* 1 if ||g_k||^2 <= tol^2 ||g_0||^2 (pcg_true.c:200) */
+    double res0;        /* ||g_0||^2 */
+    double res;         /* final ||g_k||^2 */
+    int    n_map;       /* map size after the pixel cut (3 x kept pixels) */
+    int    lcount;      /* local index count incl. trash entries (Mat.lcount after the rebuild) */
+    int    trash_pix;   /* Mat.trash_pix after the rebuild */
+    int    n_cut;       /* pixels removed by the rcond <= 0.1 cut (precond.c:560) */
+    double t_setup_ms;  /* device time: pointing build + preconditioner */
+    double t_solve_ms;  /* device time: PCG iterations */
+    double true_res_rel;/* ||b - A x||/||b - A x0|| recomputed in FP64 at exit (MGPU_MIXED), else -1 */
+    int    refreshes;   /* FP64 residual recomputations performed (MGPU_MIXED) */
+    int    pad;
+} mgpu_report;
+
+/* ---- library ---- */
+/* Select the HIP device for subsequent handles.  No reference counterpart
+ * (MPI_Init plays this role for the CPU path). */
+int mappraiser_gpu_init(int device);
+int mappraiser_gpu_finalize(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *mappraiser_gpu_last_error(void);
+void mappraiser_gpu_options_default(mgpu_options *opt);
+
+/* ---- problem (replaces MatInit + defineBlocks_avg/defineTpltz_avg,
+ *      mappraiser.c:93-195, mapmat.c:54-73, create_toeplitz.c:10-47) ---- */
+/* Host inputs: pix int32[m*nnz] global indices, pixweights f64[m*nnz],
+ * local_blocks_sizes int32[nb], invtt f64[nb*lambda].  opt may be NULL. */
+int mappraiser_gpu_problem_create(mgpu_problem **out, int m, int nnz, const int *pix,
+                                  const double *pixweights, int nb, const int *local_blocks_sizes,
+                                  int lambda, const double *invtt, const mgpu_options *opt);
+/* Same inputs already resident in device memory (d_pix, d_pixweights,
+ * d_invtt are device pointers; local_blocks_sizes is a host array). */
+int mappraiser_gpu_problem_create_device(mgpu_problem **out, int m, int nnz, const int *d_pix,
+                                         const double *d_pixweights, int nb,
+                                         const int *local_blocks_sizes, int lambda,
+                                         const double *d_invtt, const mgpu_options *opt);
+int mappraiser_gpu_problem_destroy(mgpu_problem *pb);
+
+/* MatLocalShape results (mapmat.c:496-521): lcount, trash flag, sorted unique
+ * indices (lindices, cap lcount) and the local indices (cap m*nnz).  Either
+ * array may be NULL.  After precond_build these describe the rebuilt matrix
+ * (precond.c:659-670). */
+int mappraiser_gpu_local_shape(mgpu_problem *pb, int *lcount, int *trash_pix, int *lindices,
+                               int *indices);
+
+/* ---- operators on host buffers (test and integration entry points) ---- */
+/* y[m] = P x (MatVecProd, mapmat.c:704-729); x has lcount - 3*trash_pix entries. */
+int mappraiser_gpu_matvec(mgpu_problem *pb, const double *x, double *y);
+/* x = P^T y (TrMatVecProd, mapmat.c:819-859; the MPI reduction is the identity). */
+int mappraiser_gpu_trmatvec(mgpu_problem *pb, const double *y, double *x);
+/* tod <- N^-1 tod in place (apply_weights, pcg_true.c:578-600 -> stbmmProd,
+ * toeplitz_wizard.c:80-111). */
+int mappraiser_gpu_apply_weights(mgpu_problem *pb, double *tod);
+
+/* Block-Jacobi preconditioner with the degenerate-pixel cut
+ * (precondblockjacobilike, precond.c:456-702; build_precond precond=0,
+ * precond.c:1310-1340).  signal[m] is modified in place like the reference
+ * (cut samples set to 0).  cond/lhits receive n_map/3 entries (cap: pixels
+ * before the cut); n_map receives the map size after the cut.  Any output
+ * pointer may be NULL. */
+int mappraiser_gpu_precond_build(mgpu_problem *pb, double *signal, double *cond, int *lhits,
+                                 int *n_map);
+/* Cg = BJ_inv g (apply_precond, BJ branch, precond.c:1403-1409). */
+int mappraiser_gpu_precond_apply(mgpu_problem *pb, const double *g, double *Cg);
+
+/* PCG_GLS_true iteration (pcg_true.c:20-241) after precond_build: x0 = 0,
+ * rhs = P^T N^-1 (signal + noise), Polak-Ribiere PCG.  Writes x[n_map] and,
+ * when res_hist is not NULL, sqrt(res_k)/sqrt(res_0) for k = 0.. (cap maxiter),
+ * like the pcg_residuals_<ref>.dat file (pcg_true.c:116-125,195).
+ * stop_after > 0 stops after that many iterations regardless of tol. */
+int mappraiser_gpu_pcg(mgpu_problem *pb, const double *signal, const double *noise, double tol,
+                       int maxiter, int stop_after, double *x, double *res_hist, mgpu_report *rep);
+
+/* Device-resident variant of precond_build + pcg for callers that keep TOD in
+ * HBM (signal and noise are device pointers of m doubles; d_signal is
+ * modified in place on cut samples).  x/res_hist/cond/lhits are host. */
+int mappraiser_gpu_solve_device(mgpu_problem *pb, double *d_signal, const double *d_noise,
+                                double tol, int maxiter, double *x, double *res_hist,
+                                double *cond, int *lhits, mgpu_report *rep);
+
+/* Drop-in for PCG_GLS_true(outpath, ref, &A, &Nm1, x, signal, noise, cond,
+ * lhits, tol, maxiter, precond=0, Z_2lvl) at mappraiser.c:211 for one rank,
+ * taking MLmap's raw arguments.  Side effects mirror the reference:
+ *   pix        -> local indices of the rebuilt matrix (A.indices, precond.c:659-670)
+ *   pixweights -> zeroed on cut samples (precond.c:601,613)
+ *   signal     -> zeroed on cut samples (precond.c:605,610)
+ *   x, cond, lhits -> first n_map, n_map/3, n_map/3 entries (cap: pixels before the cut)
+ *   lindices   -> rebuilt A.lindices (cap m*3), lcount/trash_pix in rep.
+ * Writes <outpath>/pcg_residuals_<ref>.dat like pcg_true.c:116-125 when outpath
+ * and ref are not NULL. */
+int mappraiser_gpu_pcg_gls(const char *outpath, const char *ref, int m, int nnz, int *pix,
+                           double *pixweights, double *signal, const double *noise, int nb,
+                           const int *local_blocks_sizes, int lambda, const double *invtt,
+                           double tol, int maxiter, double *x, double *cond, int *lhits,
+                           int *lindices, const mgpu_options *opt, mgpu_report *rep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAPPRAISER_GPU_H */

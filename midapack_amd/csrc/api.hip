@@ -1,0 +1,316 @@
+// api.hip -- the extern "C" boundary of libmappraiser_gpu.so.
+//
+// Each entry point validates its inputs, runs on the handle's own HIP stream
+// and returns an int status; the message of the last failure on the calling
+// thread is kept for mappraiser_gpu_last_error().  Nothing here calls exit()
+// (the reference exits on allocation or LAPACK failures, mappraiser.c:120-123,
+// precond.c:485-488, 577).
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
+#include "mgpu_internal.h"
+
+namespace mgpu {
+
+static thread_local char g_err[1024] = "";
+static int g_device = 0;
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+Problem::~Problem() {
+    if (h_pinned) (void) hipHostFree(h_pinned);
+    if (ev0) (void) hipEventDestroy(ev0);
+    if (ev1) (void) hipEventDestroy(ev1);
+    if (stream) {
+        (void) hipStreamSynchronize(stream);
+        (void) hipStreamDestroy(stream);
+    }
+}
+
+template <typename T>
+static int upload(Problem &pb, T **d, const T *h, size_t n) {
+    MGPU_RET(pb.arena.alloc(d, n));
+    MGPU_HIP(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, pb.stream));
+    return MGPU_OK;
+}
+
+static int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt) {
+    pb.reset(new Problem());
+    pb->device = g_device;
+    MGPU_HIP(hipSetDevice(g_device));
+    MGPU_HIP(hipStreamCreateWithFlags(&pb->stream, hipStreamNonBlocking));
+    mappraiser_gpu_options_default(&pb->opt);
+    if (opt) pb->opt = *opt;
+    MGPU_ARG(pb->opt.precision == MGPU_FP64 || pb->opt.precision == MGPU_MIXED, "unknown precision %d",
+             pb->opt.precision);
+    return MGPU_OK;
+}
+
+static int check_common(int m, int nnz, int nb, const int *bs, int lambda) {
+    MGPU_ARG(m > 0, "m=%d must be positive", m);
+    MGPU_ARG(nnz == 3, "nnz=%d: only the I,Q,U pointing (nnz=3) is supported", nnz);
+    MGPU_ARG(nb > 0 && bs, "nb_blocks=%d / local_blocks_sizes missing", nb);
+    MGPU_ARG(lambda >= 1, "lambda=%d must be >= 1", lambda);
+    return MGPU_OK;
+}
+
+}  // namespace mgpu
+
+using namespace mgpu;
+
+#define PB(h) (reinterpret_cast<Problem *>(h))
+
+extern "C" {
+
+int mappraiser_gpu_init(int device) {
+    int n = 0;
+    MGPU_HIP(hipGetDeviceCount(&n));
+    MGPU_ARG(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
+    MGPU_HIP(hipSetDevice(device));
+    g_device = device;
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_finalize(void) { return MGPU_OK; }
+
+const char *mappraiser_gpu_last_error(void) { return g_err; }
+
+void mappraiser_gpu_options_default(mgpu_options *opt) {
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->precision = MGPU_FP64;
+    opt->refresh_every = 10;
+    opt->tile_log2 = 0;
+    opt->poll_every = 8;
+}
+
+int mappraiser_gpu_problem_create(mgpu_problem **out, int m, int nnz, const int *pix, const double *pixweights,
+                                  int nb, const int *local_blocks_sizes, int lambda, const double *invtt,
+                                  const mgpu_options *opt) {
+    g_err[0] = 0;
+    MGPU_ARG(out && pix && pixweights && invtt, "NULL argument");
+    MGPU_RET(check_common(m, nnz, nb, local_blocks_sizes, lambda));
+    std::unique_ptr<Problem> pb;
+    MGPU_RET(new_problem(pb, opt));
+    int *d_pix;
+    double *d_w;
+    MGPU_RET(upload(*pb, &d_pix, pix, (size_t) m * nnz));
+    MGPU_RET(upload(*pb, &d_w, pixweights, (size_t) m * nnz));
+    MGPU_RET(pointing_build(*pb, d_pix, d_w, m, nnz));
+    pb->arena.release(d_pix);
+    pb->arena.release(d_w);
+    MGPU_RET(toeplitz_plan(*pb, local_blocks_sizes, nb, lambda, nullptr, invtt));
+    *out = reinterpret_cast<mgpu_problem *>(pb.release());
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_problem_create_device(mgpu_problem **out, int m, int nnz, const int *d_pix,
+                                         const double *d_pixweights, int nb, const int *local_blocks_sizes,
+                                         int lambda, const double *d_invtt, const mgpu_options *opt) {
+    g_err[0] = 0;
+    MGPU_ARG(out && d_pix && d_pixweights && d_invtt, "NULL argument");
+    MGPU_RET(check_common(m, nnz, nb, local_blocks_sizes, lambda));
+    std::unique_ptr<Problem> pb;
+    MGPU_RET(new_problem(pb, opt));
+    MGPU_RET(pointing_build(*pb, d_pix, d_pixweights, m, nnz));
+    MGPU_RET(toeplitz_plan(*pb, local_blocks_sizes, nb, lambda, d_invtt, nullptr));
+    *out = reinterpret_cast<mgpu_problem *>(pb.release());
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_problem_destroy(mgpu_problem *h) {
+    if (h) {
+        (void) hipSetDevice(PB(h)->device);
+        delete PB(h);
+    }
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_local_shape(mgpu_problem *h, int *lcount, int *trash_pix, int *lindices, int *indices) {
+    MGPU_ARG(h, "NULL handle");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    const int lc = 3 * pb.pt.npix + 3 * pb.pt.trash;
+    if (lcount) *lcount = lc;
+    if (trash_pix) *trash_pix = pb.pt.trash;
+    int *d;
+    if (lindices) {
+        MGPU_RET(pb.arena.alloc(&d, (size_t) lc));
+        MGPU_RET(pointing_lindices(pb, d));
+        MGPU_HIP(hipMemcpyAsync(lindices, d, sizeof(int) * lc, hipMemcpyDeviceToHost, pb.stream));
+        MGPU_HIP(hipStreamSynchronize(pb.stream));
+        pb.arena.release(d);
+    }
+    if (indices) {
+        MGPU_RET(pb.arena.alloc(&d, (size_t) pb.pt.m * 3));
+        MGPU_RET(pointing_local_indices(pb, d));
+        MGPU_HIP(hipMemcpyAsync(indices, d, sizeof(int) * pb.pt.m * 3, hipMemcpyDeviceToHost, pb.stream));
+        MGPU_HIP(hipStreamSynchronize(pb.stream));
+        pb.arena.release(d);
+    }
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_matvec(mgpu_problem *h, const double *x, double *y) {
+    MGPU_ARG(h && x && y, "NULL argument");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    MGPU_RET(ensure_work(pb));
+    const int n = 3 * pb.pt.npix;
+    MGPU_HIP(hipMemcpyAsync(pb.d_map[0], x, sizeof(double) * n, hipMemcpyHostToDevice, pb.stream));
+    MGPU_RET(launch_matvec(pb, pb.d_map[0], pb.d_tod, false));
+    MGPU_HIP(hipMemcpyAsync(y, pb.d_tod, sizeof(double) * pb.pt.m, hipMemcpyDeviceToHost, pb.stream));
+    MGPU_HIP(hipStreamSynchronize(pb.stream));
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_trmatvec(mgpu_problem *h, const double *y, double *x) {
+    MGPU_ARG(h && x && y, "NULL argument");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    MGPU_RET(ensure_work(pb));
+    const int n = 3 * pb.pt.npix;
+    MGPU_HIP(hipMemcpyAsync(pb.d_tod, y, sizeof(double) * pb.pt.m, hipMemcpyHostToDevice, pb.stream));
+    MGPU_RET(launch_trmatvec(pb, pb.d_tod, pb.d_map[0], nullptr, nullptr, nullptr));
+    MGPU_HIP(hipMemcpyAsync(x, pb.d_map[0], sizeof(double) * n, hipMemcpyDeviceToHost, pb.stream));
+    MGPU_HIP(hipStreamSynchronize(pb.stream));
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_apply_weights(mgpu_problem *h, double *tod) {
+    MGPU_ARG(h && tod, "NULL argument");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    MGPU_RET(ensure_work(pb));
+    MGPU_HIP(hipMemcpyAsync(pb.d_tod, tod, sizeof(double) * pb.pt.m, hipMemcpyHostToDevice, pb.stream));
+    MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, true));
+    MGPU_HIP(hipMemcpyAsync(tod, pb.d_tod2, sizeof(double) * pb.pt.m, hipMemcpyDeviceToHost, pb.stream));
+    MGPU_HIP(hipStreamSynchronize(pb.stream));
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_precond_build(mgpu_problem *h, double *signal, double *cond, int *lhits, int *n_map) {
+    MGPU_ARG(h, "NULL handle");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    double *d_sig = nullptr;
+    if (signal) {
+        if (!pb.d_sig) MGPU_RET(pb.arena.alloc(&pb.d_sig, (size_t) pb.pt.m));
+        d_sig = pb.d_sig;
+        MGPU_HIP(hipMemcpyAsync(d_sig, signal, sizeof(double) * pb.pt.m, hipMemcpyHostToDevice, pb.stream));
+    }
+    MGPU_RET(precond_build(pb, d_sig));
+    const int np = pb.pt.npix;
+    if (signal)
+        MGPU_HIP(hipMemcpyAsync(signal, d_sig, sizeof(double) * pb.pt.m, hipMemcpyDeviceToHost, pb.stream));
+    if (cond) MGPU_HIP(hipMemcpyAsync(cond, pb.pc.d_cond, sizeof(double) * np, hipMemcpyDeviceToHost, pb.stream));
+    if (lhits) MGPU_HIP(hipMemcpyAsync(lhits, pb.pc.d_hits, sizeof(int) * np, hipMemcpyDeviceToHost, pb.stream));
+    MGPU_HIP(hipStreamSynchronize(pb.stream));
+    if (n_map) *n_map = 3 * np;
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_precond_apply(mgpu_problem *h, const double *g, double *Cg) {
+    MGPU_ARG(h && g && Cg, "NULL argument");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    MGPU_ARG(pb.pc.built, "precond_apply before precond_build");
+    MGPU_RET(ensure_work(pb));
+    const int n = 3 * pb.pt.npix;
+    MGPU_HIP(hipMemcpyAsync(pb.d_map[0], g, sizeof(double) * n, hipMemcpyHostToDevice, pb.stream));
+    MGPU_RET(launch_precond_apply(pb, pb.d_map[0], pb.d_map[1]));
+    MGPU_HIP(hipMemcpyAsync(Cg, pb.d_map[1], sizeof(double) * n, hipMemcpyDeviceToHost, pb.stream));
+    MGPU_HIP(hipStreamSynchronize(pb.stream));
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_pcg(mgpu_problem *h, const double *signal, const double *noise, double tol, int maxiter,
+                       int stop_after, double *x, double *res_hist, mgpu_report *rep) {
+    MGPU_ARG(h && signal && noise, "NULL argument");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    MGPU_ARG(pb.pc.built, "pcg before precond_build");
+    if (!pb.d_sig) MGPU_RET(pb.arena.alloc(&pb.d_sig, (size_t) pb.pt.m));
+    if (!pb.d_noise) MGPU_RET(pb.arena.alloc(&pb.d_noise, (size_t) pb.pt.m));
+    MGPU_HIP(hipMemcpyAsync(pb.d_sig, signal, sizeof(double) * pb.pt.m, hipMemcpyHostToDevice, pb.stream));
+    MGPU_HIP(hipMemcpyAsync(pb.d_noise, noise, sizeof(double) * pb.pt.m, hipMemcpyHostToDevice, pb.stream));
+    mgpu_report r{};
+    MGPU_RET(pcg_run(pb, pb.d_sig, pb.d_noise, tol, maxiter, stop_after, x, res_hist, &r));
+    if (rep) *rep = r;
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_solve_device(mgpu_problem *h, double *d_signal, const double *d_noise, double tol, int maxiter,
+                                double *x, double *res_hist, double *cond, int *lhits, mgpu_report *rep) {
+    MGPU_ARG(h && d_signal && d_noise, "NULL argument");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    MGPU_RET(ensure_work(pb));
+    MGPU_HIP(hipEventRecord(pb.ev0, pb.stream));
+    MGPU_RET(precond_build(pb, d_signal));
+    MGPU_HIP(hipEventRecord(pb.ev1, pb.stream));
+    MGPU_HIP(hipEventSynchronize(pb.ev1));
+    float setup_ms = 0.f;
+    MGPU_HIP(hipEventElapsedTime(&setup_ms, pb.ev0, pb.ev1));
+    mgpu_report r{};
+    MGPU_RET(pcg_run(pb, d_signal, d_noise, tol, maxiter, 0, x, res_hist, &r));
+    r.t_setup_ms = setup_ms;
+    const int np = pb.pt.npix;
+    if (cond) MGPU_HIP(hipMemcpyAsync(cond, pb.pc.d_cond, sizeof(double) * np, hipMemcpyDeviceToHost, pb.stream));
+    if (lhits) MGPU_HIP(hipMemcpyAsync(lhits, pb.pc.d_hits, sizeof(int) * np, hipMemcpyDeviceToHost, pb.stream));
+    MGPU_HIP(hipStreamSynchronize(pb.stream));
+    if (rep) *rep = r;
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_pcg_gls(const char *outpath, const char *ref, int m, int nnz, int *pix, double *pixweights,
+                           double *signal, const double *noise, int nb, const int *local_blocks_sizes, int lambda,
+                           const double *invtt, double tol, int maxiter, double *x, double *cond, int *lhits,
+                           int *lindices, const mgpu_options *opt, mgpu_report *rep) {
+    mgpu_problem *h = nullptr;
+    MGPU_ARG(pix && pixweights && signal && noise && x, "NULL argument");
+    MGPU_RET(mappraiser_gpu_problem_create(&h, m, nnz, pix, pixweights, nb, local_blocks_sizes, lambda, invtt, opt));
+    std::unique_ptr<Problem> guard(PB(h));
+    Problem &pb = *guard;
+    int nmap = 0;
+    MGPU_RET(mappraiser_gpu_precond_build(h, signal, cond, lhits, &nmap));
+    std::vector<double> hist(maxiter > 0 ? maxiter : 1);
+    mgpu_report r{};
+    MGPU_RET(mappraiser_gpu_pcg(h, signal, noise, tol, maxiter, 0, x, hist.data(), &r));
+    // side effects of the reference on the caller's pointing: local indices of
+    // the rebuilt matrix; weights of cut samples zeroed
+    const int64_t mn = (int64_t) m * 3;
+    std::vector<int> loc((size_t) mn);
+    int lcount = 0, trash = 0;
+    MGPU_RET(mappraiser_gpu_local_shape(h, &lcount, &trash, lindices, loc.data()));
+    for (int64_t t = 0; t < m; t++) {
+        const bool was_valid = pix[3 * t] >= 0;
+        const bool now_trash = trash && loc[3 * t] == 0;
+        if (was_valid && now_trash)
+            for (int k = 0; k < 3; k++) pixweights[3 * t + k] = 0.0;
+    }
+    std::memcpy(pix, loc.data(), sizeof(int) * mn);
+    r.lcount = lcount;
+    r.trash_pix = trash;
+    if (outpath && ref) {
+        char fn[4096];
+        std::snprintf(fn, sizeof(fn), "%s/pcg_residuals_%s.dat", outpath, ref);
+        FILE *fp = std::fopen(fn, "wb");
+        if (fp) {
+            const int nh = std::min(maxiter, r.iterations + 1);
+            std::fwrite(hist.data(), sizeof(double), (size_t) nh, fp);
+            std::fclose(fp);
+        }
+    }
+    if (rep) *rep = r;
+    (void) pb;
+    return MGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,261 @@
+// mgpu_internal.h -- shared declarations of libmappraiser_gpu.so (gfx950).
+//
+// Device-resident state of one rank's GLS problem.  Layout in HBM (m samples,
+// npix valid pixels, nh = hits on valid pixels, nb Toeplitz blocks):
+//   pointing, sample-major  q[m] int32 (valid pixel number, -1 = trash row),
+//                           w1[m], w2[m] (and w0[m] when the I weight is not 1)
+//   pointing, pixel-major   off[npix+1] int32, ct[nh] int32 (sample index, t
+//                           increasing within a pixel), cw1[nh], cw2[nh] (cw0)
+//   map vectors             3*npix doubles, x[3q+k] (local map order of the
+//                           reference, mapmat.c:496-521)
+//   TOD vectors             m elements (double, or float in MGPU_MIXED)
+//   Toeplitz                per-block start offsets, tile descriptors, spectral
+//                           coefficients per distinct filter, twiddle table
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mappraiser_gpu.h"
+
+namespace mgpu {
+
+void set_error(const char *fmt, ...);
+
+#define MGPU_HIP(expr)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            ::mgpu::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,               \
+                              hipGetErrorString(e_));                                          \
+            return MGPU_EHIP;                                                                  \
+        }                                                                                      \
+    } while (0)
+
+#define MGPU_ARG(cond, ...)                                                                    \
+    do {                                                                                       \
+        if (!(cond)) {                                                                         \
+            ::mgpu::set_error(__VA_ARGS__);                                                    \
+            return MGPU_EINVAL;                                                                \
+        }                                                                                      \
+    } while (0)
+
+#define MGPU_RET(expr)                                                                         \
+    do {                                                                                       \
+        int r_ = (expr);                                                                       \
+        if (r_) return r_;                                                                     \
+    } while (0)
+
+// Launch check: kernel launches report configuration errors through
+// hipGetLastError.
+#define MGPU_LAUNCHED() MGPU_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// device buffers (owned by a problem; freed in its destructor)
+// ---------------------------------------------------------------------------
+struct Arena {
+    std::vector<void *> ptrs;
+    template <typename T>
+    int alloc(T **p, size_t n) {
+        *p = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc((void **) p, n * sizeof(T));
+        if (e != hipSuccess) {
+            (void) hipGetLastError();
+            set_error("hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+            return MGPU_ENOMEM;
+        }
+        ptrs.push_back((void *) *p);
+        return MGPU_OK;
+    }
+    void release(void *p) {
+        for (auto &q : ptrs)
+            if (q == p) {
+                (void) hipFree(q);
+                q = nullptr;
+            }
+    }
+    ~Arena() {
+        for (void *p : ptrs)
+            if (p) (void) hipFree(p);
+    }
+};
+
+// Scratch that grows on demand (hipcub temporaries, host staging).
+struct Scratch {
+    void *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= n) return MGPU_OK;
+        if (p) (void) hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void) hipGetLastError();
+            set_error("hipMalloc scratch of %zu bytes failed", bytes);
+            return MGPU_ENOMEM;
+        }
+        n = bytes;
+        return MGPU_OK;
+    }
+    ~Scratch() {
+        if (p) (void) hipFree(p);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Toeplitz plan
+// ---------------------------------------------------------------------------
+struct TileDesc {
+    int64_t blk_lo;   // first sample of the block
+    int64_t blk_hi;   // one past the last sample of the block
+    int64_t out_lo;   // first output sample of this tile (blk_lo + k*L)
+    int32_t block;    // block index (mean slot)
+    int32_t filter;   // spectral coefficient set
+};
+
+constexpr int kChunk = 1024;  // samples per partial-sum chunk (P epilogue / partials kernel)
+
+struct ToeplitzPlan {
+    int lambda = 0;
+    int nb = 0;
+    int64_t m = 0;
+    int64_t covered = 0;             // sum of block sizes (samples past it are left unchanged)
+    std::vector<int64_t> h_start;    // nb+1 block starts
+    int64_t *d_start = nullptr;      // nb+1
+    double *d_T0 = nullptr;          // nb, T_b[0]
+    // FFT tiles
+    int log2n = 0;                   // real tile length N = 2^log2n
+    int L = 0;                       // useful outputs per tile, N - 2(lambda-1)
+    int ntiles = 0;
+    TileDesc *d_tiles = nullptr;
+    int nfilters = 0;
+    std::vector<int> h_block_filter; // nb
+    double *d_coef64 = nullptr;      // nfilters * NC * 2  (a_k, b_k) interleaved, includes 1/NC
+    float *d_coef32 = nullptr;
+    double *d_tw64 = nullptr;        // NC complex: exp(-2 pi i m / NC)
+    float *d_tw32 = nullptr;
+    // blocks shorter than lambda-1 are only mean-subtracted (stmm_main:912-913)
+    std::vector<int> h_short_blocks;
+    int *d_short_blocks = nullptr;
+    // bands too wide for an LDS tile: direct product over these blocks
+    std::vector<int> h_direct_blocks;
+    int *d_direct_blocks = nullptr;
+    double *d_direct_invtt = nullptr;
+    // partial sums for the per-block mean (toeplitz_wizard.c:86-95)
+    int64_t nchunks = 0;
+    double *d_partials = nullptr;    // nchunks + nb
+    double *d_mean = nullptr;        // nb
+};
+
+// ---------------------------------------------------------------------------
+// Pointing
+// ---------------------------------------------------------------------------
+struct Pointing {
+    int64_t m = 0;
+    int nnz = 3;
+    int npix = 0;          // valid pixels (current)
+    int trash = 0;         // Mat.trash_pix (current)
+    int npix0 = 0;         // valid pixels of the first MatInit (before the cut)
+    int trash0 = 0;
+    bool unit_w0 = true;   // I weight identically 1
+    int64_t nh = 0;        // sliced-ELL length (hits + padding)
+    int *d_q = nullptr;    // [m]
+    double *d_w0 = nullptr, *d_w1 = nullptr, *d_w2 = nullptr;   // [m]
+    float *d_w0f = nullptr, *d_w1f = nullptr, *d_w2f = nullptr; // [m] (mixed)
+    int64_t nvalid = 0;    // samples on valid pixels (CSR length)
+    int *d_off = nullptr;  // [npix+1] CSR offsets (hits of pixel q: off[q]..off[q+1])
+    int *d_cst = nullptr;  // [nvalid] CSR sample indices, increasing within a pixel
+    int64_t *d_ell_base = nullptr; // [nslice+1] sliced-ELL slice offsets (elements)
+    int *d_ct = nullptr;   // [nh] sliced-ELL sample index (-1 = padding)
+    double *d_cw0 = nullptr, *d_cw1 = nullptr, *d_cw2 = nullptr;
+    float *d_cw0f = nullptr, *d_cw1f = nullptr, *d_cw2f = nullptr;
+    int *d_pixg = nullptr; // [npix0] global pixel number p (valid pixels, increasing)
+};
+
+// ---------------------------------------------------------------------------
+// Preconditioner
+// ---------------------------------------------------------------------------
+struct Precond {
+    bool built = false;
+    int n_cut = 0;
+    double *d_inv = nullptr;   // [npix * 9] row-major inverse blocks
+    double *d_cond = nullptr;  // [npix0]
+    int *d_hits = nullptr;     // [npix0]
+};
+
+// ---------------------------------------------------------------------------
+// Problem
+// ---------------------------------------------------------------------------
+struct Problem {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    mgpu_options opt{};
+    Arena arena;
+    Scratch cub_tmp;
+    Pointing pt;
+    ToeplitzPlan tp;
+    Precond pc;
+    // working TOD / map vectors (allocated lazily)
+    double *d_tod = nullptr, *d_tod2 = nullptr;   // [m] FP64
+    float *d_todf = nullptr, *d_tod2f = nullptr;  // [m] FP32 (mixed)
+    double *d_sig = nullptr;                      // [m] signal (device copy)
+    double *d_noise = nullptr;                    // [m]
+    double *d_map[8] = {nullptr};                 // map work vectors (3*npix0 each)
+    float *d_mapf = nullptr;                      // FP32 copy of a map vector (mixed P)
+    double *d_red = nullptr;                      // reduction partials
+    double *d_scal = nullptr;                     // device scalars of PCG
+    double *h_pinned = nullptr;                   // pinned host scalars
+    int *d_flag = nullptr;                        // PCG stop flag
+    int64_t nred = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~Problem();
+};
+
+// ---------------------------------------------------------------------------
+// entry points between translation units
+// ---------------------------------------------------------------------------
+int pointing_build(Problem &pb, const int *d_pix, const double *d_w, int64_t m, int nnz);
+int pointing_local_indices(Problem &pb, int *d_out);   // MatLocalShape indices [m*3]
+int pointing_lindices(Problem &pb, int *d_out);        // lindices [lcount]
+int pointing_make_fp32(Problem &pb);
+// remove pixels with kept[q]==0: renumber q, compact CSR, rebuild the ELL
+int pointing_compact(Problem &pb, const unsigned *d_kept);
+
+// y = P x (FP64); optional fused per-block partials when tp has blocks
+int launch_matvec(Problem &pb, const double *d_x, double *d_y, bool partials);
+int launch_matvec_f32(Problem &pb, const float *d_x, float *d_y, bool partials);
+// x = P^T y; optional fused partial dots of <x, hvec> into d_dot (one per workgroup)
+int launch_trmatvec(Problem &pb, const double *d_y, double *d_x, const double *d_hvec,
+                    double *d_dotpart, int *nparts);
+int launch_trmatvec_f32(Problem &pb, const float *d_y, double *d_x, const double *d_hvec,
+                        double *d_dotpart, int *nparts);
+int trmatvec_nparts(const Problem &pb);
+
+int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const double *d_invtt,
+                  const double *h_invtt);
+// N^-1 on d_in (its per-block partials must already be in tp.d_partials unless
+// compute_partials) -> d_out (may not alias d_in)
+int toeplitz_apply(Problem &pb, const double *d_in, double *d_out, bool compute_partials);
+int toeplitz_apply_f32(Problem &pb, const float *d_in, float *d_out, bool compute_partials);
+
+int precond_build(Problem &pb, double *d_signal);
+int launch_precond_apply(Problem &pb, const double *d_g, double *d_Cg);
+
+int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double tol, int maxiter,
+            int stop_after, double *h_x, double *h_hist, mgpu_report *rep);
+
+int ensure_work(Problem &pb);
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+inline unsigned grid_for(int64_t n, int block) { return (unsigned) ((n + block - 1) / block); }
+
+}  // namespace mgpu

@@ -1,0 +1,696 @@
+// pointing.hip -- local index tables and the pointing products P, P^T (gfx950).
+//
+// Reference semantics (one rank):
+//   MatLocalShape  mapmat.c:496-521 with counting_sort (csort.c:111-135) and
+//                  dichotomy (cindex.c:102-115): lindices = sorted unique pix,
+//                  indices = position in lindices, trash flag if lindices[0]<0.
+//   MatVecProd     mapmat.c:704-729: y[t] = ((0 + w0 x0) + w1 x1) + w2 x2.
+//   TrMatVecProd   mapmat.c:819-851: x[i] += w y[t] in increasing t.
+//
+// Layout choices for HBM:
+//   * P runs sample-major (one lane per sample, 4 samples per lane, 1024 per
+//     workgroup): q/w loads and the y store are fully coalesced; the map gather
+//     x[3q+k] is L2-resident (map << TOD).  The per-block sums needed by the
+//     mean removal of N^-1 are fused into the epilogue (one partial per
+//     (chunk, block) pair, fixed-order tree, deterministic).
+//   * P^T runs pixel-major over a sliced-ELL layout with slice height 64 (one
+//     wavefront): slice s stores hit i of lane l at base_s + 64 i + l, hits of a
+//     pixel in increasing t.  Each lane accumulates its pixel sequentially, so
+//     the summation order equals the reference's, the result is bit-exact and
+//     deterministic, and no atomics are used.  All loads of the ELL arrays are
+//     coalesced; the gather of y[t] is coalesced for scans whose neighbouring
+//     pixels are neighbours in time (rasters).
+//
+// Every kernel is compiled with -ffp-contract=off: products and sums are
+// separately rounded like the reference's C build.
+#include <hipcub/hipcub.hpp>
+
+#include "mgpu_internal.h"
+
+namespace mgpu {
+
+constexpr int kSlice = 64;   // ELL slice height (one wavefront)
+constexpr int kPBlock = 256; // P workgroup size (kChunk / 4 samples per lane)
+
+// ---------------------------------------------------------------------------
+// validation + pixel extraction
+// ---------------------------------------------------------------------------
+// status bits: 1 = malformed row, 2 = trash row present, 4 = I weight != 1
+__global__ void k_classify(const int *__restrict__ pix, const double *__restrict__ w, int64_t m,
+                           int *__restrict__ p_of_t, int *__restrict__ status,
+                           int *__restrict__ pmin, int *__restrict__ pmax) {
+    int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    int lo = INT32_MAX, hi = INT32_MIN, st = 0;
+    if (t < m) {
+        int a = pix[3 * t], b = pix[3 * t + 1], c = pix[3 * t + 2];
+        int p = -1;
+        if (a >= 0) {
+            if (a % 3 != 0 || b != a + 1 || c != a + 2) st |= 1;
+            else p = a / 3;
+        } else {
+            if (a != -3 || b != -2 || c != -1) st |= 1;
+            else st |= 2;
+        }
+        if (w[3 * t] != 1.0) st |= 4;
+        p_of_t[t] = p;
+        if (p >= 0) {
+            lo = p;
+            hi = p;
+        }
+    }
+    // wavefront reduce, then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+        st |= __shfl_xor(st, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (lo != INT32_MAX) {
+            atomicMin(pmin, lo);
+            atomicMax(pmax, hi);
+        }
+        if (st) atomicOr(status, st);
+    }
+}
+
+__global__ void k_mark(const int *__restrict__ p_of_t, int64_t m, int pmin,
+                       unsigned *__restrict__ flags) {
+    int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        int p = p_of_t[t];
+        if (p >= 0) flags[p - pmin] = 1u;
+    }
+}
+
+__global__ void k_assign(const int *__restrict__ p_of_t, int64_t m, int pmin,
+                         const unsigned *__restrict__ rank, const double *__restrict__ w,
+                         int *__restrict__ q, double *__restrict__ w0, double *__restrict__ w1,
+                         double *__restrict__ w2, unsigned *__restrict__ key, int *__restrict__ tval,
+                         unsigned trash_key) {
+    int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        int p = p_of_t[t];
+        int qq = p >= 0 ? (int) rank[p - pmin] : -1;
+        q[t] = qq;
+        if (w0) w0[t] = w[3 * t];
+        w1[t] = w[3 * t + 1];
+        w2[t] = w[3 * t + 2];
+        key[t] = qq >= 0 ? (unsigned) qq : trash_key;
+        tval[t] = (int) t;
+    }
+}
+
+__global__ void k_pixg(const unsigned *__restrict__ flags, const unsigned *__restrict__ rank,
+                       int64_t span, int pmin, int *__restrict__ pixg) {
+    int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < span && flags[i]) pixg[rank[i]] = pmin + (int) i;
+}
+
+// CSR offsets from the sorted keys: off[k] = first position of key k
+__global__ void k_offsets(const unsigned *__restrict__ skey, int64_t nh, int npix,
+                          int *__restrict__ off) {
+    int64_t h = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < nh) {
+        unsigned k = skey[h];
+        if (h == 0 || skey[h - 1] != k) off[k] = (int) h;
+    }
+    if (h == 0) off[npix] = (int) nh;
+}
+
+// slice widths: width[s] = max hits over the 64 pixels of slice s
+__global__ void k_slice_width(const int *__restrict__ off, int npix, int nslice,
+                              int64_t *__restrict__ width) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nslice) {
+        int wmax = 0;
+        for (int l = 0; l < kSlice; l++) {
+            int qq = s * kSlice + l;
+            if (qq < npix) wmax = max(wmax, off[qq + 1] - off[qq]);
+        }
+        width[s] = (int64_t) wmax;
+    }
+}
+
+// Scatter the CSR into sliced-ELL (pixel q = 64 s + lane; hit i at base_s + 64 i + lane)
+template <bool W0>
+__global__ void k_fill_ell(const int *__restrict__ off, const int *__restrict__ st,
+                           const int64_t *__restrict__ base, int npix, int nslice,
+                           const double *__restrict__ w0, const double *__restrict__ w1,
+                           const double *__restrict__ w2, int *__restrict__ ct, double *__restrict__ cw0,
+                           double *__restrict__ cw1, double *__restrict__ cw2) {
+    int qq = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = qq / kSlice, lane = qq % kSlice;
+    if (s >= nslice) return;
+    int64_t b0 = base[s], width = (base[s + 1] - base[s]) / kSlice;
+    int lo = 0, hi = 0;
+    if (qq < npix) {
+        lo = off[qq];
+        hi = off[qq + 1];
+    }
+    for (int64_t i = 0; i < width; i++) {
+        int64_t dst = b0 + i * kSlice + lane;
+        int h = lo + (int) i;
+        if (h < hi) {
+            int t = st[h];
+            ct[dst] = t;
+            if (W0) cw0[dst] = w0[t];
+            cw1[dst] = w1[t];
+            cw2[dst] = w2[t];
+        } else {
+            ct[dst] = -1;
+            if (W0) cw0[dst] = 0.0;
+            cw1[dst] = 0.0;
+            cw2[dst] = 0.0;
+        }
+    }
+}
+
+__global__ void k_scale64(int64_t *a, int64_t n, int64_t f) {
+    int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] *= f;
+}
+
+// number of valid samples: keys < npix are sorted before the trash keys
+__global__ void k_count_valid(const unsigned *__restrict__ skey, int64_t m, unsigned npix,
+                              int64_t *__restrict__ cnt) {
+    int64_t h = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < m && skey[h] < npix && (h + 1 == m || skey[h + 1] >= npix)) *cnt = h + 1;
+}
+
+__global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+    int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float) a[i];
+}
+
+// compaction after the pixel cut
+__global__ void k_kept_counts(const unsigned *__restrict__ kept, const int *__restrict__ off, int npix,
+                              unsigned *__restrict__ keep_u, int *__restrict__ cnt) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < npix) {
+        keep_u[q] = kept[q] ? 1u : 0u;
+        cnt[q] = kept[q] ? off[q + 1] - off[q] : 0;
+    } else if (q == npix) {
+        keep_u[q] = 0u;
+        cnt[q] = 0;
+    }
+}
+
+__global__ void k_compact_pixels(const unsigned *__restrict__ kept, const unsigned *__restrict__ newidx,
+                                 const int *__restrict__ hscan, const int *__restrict__ pixg_old, int npix,
+                                 int *__restrict__ pixg_new, int *__restrict__ off_new, int npix_new) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < npix && kept[q]) {
+        pixg_new[newidx[q]] = pixg_old[q];
+        off_new[newidx[q]] = hscan[q];
+    }
+    if (q == npix) off_new[npix_new] = hscan[npix];
+}
+
+__global__ void k_compact_hits(const unsigned *__restrict__ kept, const unsigned *__restrict__ newidx,
+                               const int *__restrict__ off_old, const int *__restrict__ cst_old,
+                               const int *__restrict__ off_new, int npix, int *__restrict__ cst_new) {
+    int q = blockIdx.x;
+    if (q >= npix || !kept[q]) return;
+    const int lo = off_old[q], hi = off_old[q + 1], dst = off_new[newidx[q]];
+    for (int h = lo + threadIdx.x; h < hi; h += blockDim.x) cst_new[dst + (h - lo)] = cst_old[h];
+}
+
+__global__ void k_remap_q(int *__restrict__ q, int64_t m, const unsigned *__restrict__ kept,
+                          const unsigned *__restrict__ newidx) {
+    int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        int qq = q[t];
+        if (qq >= 0) q[t] = kept[qq] ? (int) newidx[qq] : -1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// build
+// ---------------------------------------------------------------------------
+static int build_ell(Problem &pb) {
+    Pointing &P = pb.pt;
+    hipStream_t s = pb.stream;
+    const int npix = P.npix;
+    const int nslice = (npix + kSlice - 1) / kSlice;
+    if (P.d_ell_base) pb.arena.release(P.d_ell_base);
+    if (P.d_ct) pb.arena.release(P.d_ct);
+    if (P.d_cw0) pb.arena.release(P.d_cw0);
+    if (P.d_cw1) pb.arena.release(P.d_cw1);
+    if (P.d_cw2) pb.arena.release(P.d_cw2);
+    P.d_ct = nullptr;
+    P.d_cw0 = P.d_cw1 = P.d_cw2 = nullptr;
+    int64_t *d_w, *d_base;
+    MGPU_RET(pb.arena.alloc(&d_w, (size_t) nslice + 1));
+    MGPU_RET(pb.arena.alloc(&d_base, (size_t) nslice + 1));
+    MGPU_HIP(hipMemsetAsync(d_w, 0, sizeof(int64_t) * (nslice + 1), s));
+    if (nslice > 0) {
+        k_slice_width<<<grid_for(nslice, 256), 256, 0, s>>>(P.d_off, npix, nslice, d_w);
+        MGPU_LAUNCHED();
+    }
+    size_t tb = 0;
+    MGPU_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_w, d_base, nslice + 1, s));
+    MGPU_RET(pb.cub_tmp.ensure(tb));
+    MGPU_HIP(hipcub::DeviceScan::ExclusiveSum(pb.cub_tmp.p, tb, d_w, d_base, nslice + 1, s));
+    k_scale64<<<grid_for(nslice + 1, 256), 256, 0, s>>>(d_base, nslice + 1, kSlice);
+    MGPU_LAUNCHED();
+    int64_t nell = 0;
+    MGPU_HIP(hipMemcpyAsync(&nell, d_base + nslice, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipStreamSynchronize(s));
+    pb.arena.release(d_w);
+    MGPU_RET(pb.arena.alloc(&P.d_ct, (size_t) nell));
+    if (!P.unit_w0) MGPU_RET(pb.arena.alloc(&P.d_cw0, (size_t) nell));
+    MGPU_RET(pb.arena.alloc(&P.d_cw1, (size_t) nell));
+    MGPU_RET(pb.arena.alloc(&P.d_cw2, (size_t) nell));
+    P.nh = nell;
+    P.d_ell_base = d_base;
+    const int nthr = nslice * kSlice;
+    if (nthr > 0) {
+        if (P.unit_w0)
+            k_fill_ell<false><<<grid_for(nthr, 256), 256, 0, s>>>(P.d_off, P.d_cst, d_base, npix, nslice, nullptr,
+                                                                  P.d_w1, P.d_w2, P.d_ct, nullptr, P.d_cw1,
+                                                                  P.d_cw2);
+        else
+            k_fill_ell<true><<<grid_for(nthr, 256), 256, 0, s>>>(P.d_off, P.d_cst, d_base, npix, nslice, P.d_w0, P.d_w1,
+                                                                 P.d_w2, P.d_ct, P.d_cw0, P.d_cw1, P.d_cw2);
+        MGPU_LAUNCHED();
+    }
+    // FP32 copies are stale now
+    if (P.d_cw1f) {
+        pb.arena.release(P.d_cw1f);
+        pb.arena.release(P.d_cw2f);
+        if (P.d_cw0f) pb.arena.release(P.d_cw0f);
+        pb.arena.release(P.d_w1f);
+        pb.arena.release(P.d_w2f);
+        if (P.d_w0f) pb.arena.release(P.d_w0f);
+        P.d_cw0f = P.d_cw1f = P.d_cw2f = nullptr;
+        P.d_w0f = P.d_w1f = P.d_w2f = nullptr;
+    }
+    return MGPU_OK;
+}
+
+int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m, int nnz) {
+    Pointing &P = pb.pt;
+    MGPU_ARG(nnz == 3, "nnz=%d: the GPU path supports the I,Q,U pointing (nnz=3) only", nnz);
+    MGPU_ARG(m > 0 && m < (int64_t) INT32_MAX, "m=%lld out of range", (long long) m);
+    P.m = m;
+    P.nnz = nnz;
+    hipStream_t s = pb.stream;
+
+    int *d_p, *d_stat;
+    MGPU_RET(pb.arena.alloc(&d_p, (size_t) m));
+    MGPU_RET(pb.arena.alloc(&d_stat, 3));
+    int init[3] = {0, INT32_MAX, INT32_MIN};
+    MGPU_HIP(hipMemcpyAsync(d_stat, init, sizeof(init), hipMemcpyHostToDevice, s));
+    k_classify<<<grid_for(m, 256), 256, 0, s>>>(d_pix, d_wts, m, d_p, d_stat, d_stat + 1, d_stat + 2);
+    MGPU_LAUNCHED();
+    int hst[3];
+    MGPU_HIP(hipMemcpyAsync(hst, d_stat, sizeof(hst), hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipStreamSynchronize(s));
+    pb.arena.release(d_stat);
+    MGPU_ARG(!(hst[0] & 1), "pointing is not encoded 3p+k (mappraiser.py:1214-1216) with flagged "
+                            "samples as -3,-2,-1 (utils.py:148-150)");
+    MGPU_ARG(hst[1] != INT32_MAX, "no valid sample: every sample is flagged");
+    const int pmin = hst[1], pmax = hst[2];
+    P.trash0 = P.trash = (hst[0] & 2) ? 1 : 0;
+    P.unit_w0 = !(hst[0] & 4);
+    const int64_t span = (int64_t) pmax - pmin + 1;
+
+    unsigned *d_flags, *d_rank;
+    MGPU_RET(pb.arena.alloc(&d_flags, (size_t) span + 1));
+    MGPU_RET(pb.arena.alloc(&d_rank, (size_t) span + 1));
+    MGPU_HIP(hipMemsetAsync(d_flags, 0, sizeof(unsigned) * (span + 1), s));
+    k_mark<<<grid_for(m, 256), 256, 0, s>>>(d_p, m, pmin, d_flags);
+    MGPU_LAUNCHED();
+    size_t tb = 0;
+    MGPU_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_flags, d_rank, span + 1, s));
+    MGPU_RET(pb.cub_tmp.ensure(tb));
+    MGPU_HIP(hipcub::DeviceScan::ExclusiveSum(pb.cub_tmp.p, tb, d_flags, d_rank, span + 1, s));
+    unsigned npix_u = 0;
+    MGPU_HIP(hipMemcpyAsync(&npix_u, d_rank + span, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipStreamSynchronize(s));
+    P.npix0 = P.npix = (int) npix_u;
+
+    MGPU_RET(pb.arena.alloc(&P.d_q, (size_t) m));
+    if (!P.unit_w0) MGPU_RET(pb.arena.alloc(&P.d_w0, (size_t) m));
+    MGPU_RET(pb.arena.alloc(&P.d_w1, (size_t) m));
+    MGPU_RET(pb.arena.alloc(&P.d_w2, (size_t) m));
+    MGPU_RET(pb.arena.alloc(&P.d_pixg, (size_t) P.npix0));
+    unsigned *d_key, *d_skey;
+    int *d_t;
+    MGPU_RET(pb.arena.alloc(&d_key, (size_t) m));
+    MGPU_RET(pb.arena.alloc(&d_skey, (size_t) m));
+    MGPU_RET(pb.arena.alloc(&d_t, (size_t) m));
+    MGPU_RET(pb.arena.alloc(&P.d_cst, (size_t) m));
+    k_assign<<<grid_for(m, 256), 256, 0, s>>>(d_p, m, pmin, d_rank, d_wts, P.d_q, P.d_w0, P.d_w1, P.d_w2,
+                                              d_key, d_t, (unsigned) P.npix0);
+    MGPU_LAUNCHED();
+    k_pixg<<<grid_for(span, 256), 256, 0, s>>>(d_flags, d_rank, span, pmin, P.d_pixg);
+    MGPU_LAUNCHED();
+
+    // stable radix sort of (pixel, t): t stays increasing within each pixel
+    int end_bit = 1;
+    while (end_bit < 32 && ((unsigned) P.npix0 >> end_bit) != 0) end_bit++;
+    tb = 0;
+    MGPU_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_key, d_skey, d_t, P.d_cst, (int) m, 0, end_bit,
+                                                s));
+    MGPU_RET(pb.cub_tmp.ensure(tb));
+    MGPU_HIP(hipcub::DeviceRadixSort::SortPairs(pb.cub_tmp.p, tb, d_key, d_skey, d_t, P.d_cst, (int) m, 0,
+                                                end_bit, s));
+    int64_t *d_cnt;
+    MGPU_RET(pb.arena.alloc(&d_cnt, 1));
+    MGPU_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int64_t), s));
+    k_count_valid<<<grid_for(m, 256), 256, 0, s>>>(d_skey, m, (unsigned) P.npix0, d_cnt);
+    MGPU_LAUNCHED();
+    int64_t nvalid = 0;
+    MGPU_HIP(hipMemcpyAsync(&nvalid, d_cnt, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipStreamSynchronize(s));
+    P.nvalid = nvalid;
+    MGPU_RET(pb.arena.alloc(&P.d_off, (size_t) P.npix0 + 1));
+    k_offsets<<<grid_for(nvalid > 0 ? nvalid : 1, 256), 256, 0, s>>>(d_skey, nvalid, P.npix0, P.d_off);
+    MGPU_LAUNCHED();
+    MGPU_RET(build_ell(pb));
+    MGPU_HIP(hipStreamSynchronize(s));
+    pb.arena.release(d_cnt);
+    pb.arena.release(d_key);
+    pb.arena.release(d_skey);
+    pb.arena.release(d_t);
+    pb.arena.release(d_flags);
+    pb.arena.release(d_rank);
+    pb.arena.release(d_p);
+    return MGPU_OK;
+}
+
+int pointing_compact(Problem &pb, const unsigned *d_kept) {
+    Pointing &P = pb.pt;
+    hipStream_t s = pb.stream;
+    const int npix = P.npix;
+    unsigned *d_keep, *d_new;
+    int *d_cnt, *d_hscan;
+    MGPU_RET(pb.arena.alloc(&d_keep, (size_t) npix + 1));
+    MGPU_RET(pb.arena.alloc(&d_new, (size_t) npix + 1));
+    MGPU_RET(pb.arena.alloc(&d_cnt, (size_t) npix + 1));
+    MGPU_RET(pb.arena.alloc(&d_hscan, (size_t) npix + 1));
+    k_kept_counts<<<grid_for(npix + 1, 256), 256, 0, s>>>(d_kept, P.d_off, npix, d_keep, d_cnt);
+    MGPU_LAUNCHED();
+    size_t tb = 0, tb2 = 0;
+    MGPU_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_keep, d_new, npix + 1, s));
+    MGPU_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, d_cnt, d_hscan, npix + 1, s));
+    MGPU_RET(pb.cub_tmp.ensure(tb > tb2 ? tb : tb2));
+    MGPU_HIP(hipcub::DeviceScan::ExclusiveSum(pb.cub_tmp.p, tb, d_keep, d_new, npix + 1, s));
+    MGPU_HIP(hipcub::DeviceScan::ExclusiveSum(pb.cub_tmp.p, tb2, d_cnt, d_hscan, npix + 1, s));
+    unsigned nnew = 0;
+    int nvalid_new = 0;
+    MGPU_HIP(hipMemcpyAsync(&nnew, d_new + npix, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipMemcpyAsync(&nvalid_new, d_hscan + npix, sizeof(int), hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipStreamSynchronize(s));
+    int *d_pixg2, *d_off2, *d_cst2;
+    MGPU_RET(pb.arena.alloc(&d_pixg2, (size_t) nnew));
+    MGPU_RET(pb.arena.alloc(&d_off2, (size_t) nnew + 1));
+    MGPU_RET(pb.arena.alloc(&d_cst2, (size_t) nvalid_new));
+    k_compact_pixels<<<grid_for(npix + 1, 256), 256, 0, s>>>(d_kept, d_new, d_hscan, P.d_pixg, npix, d_pixg2,
+                                                             d_off2, (int) nnew);
+    MGPU_LAUNCHED();
+    if (npix > 0) {
+        k_compact_hits<<<npix, 64, 0, s>>>(d_kept, d_new, P.d_off, P.d_cst, d_off2, npix, d_cst2);
+        MGPU_LAUNCHED();
+    }
+    k_remap_q<<<grid_for(P.m, 256), 256, 0, s>>>(P.d_q, P.m, d_kept, d_new);
+    MGPU_LAUNCHED();
+    MGPU_HIP(hipStreamSynchronize(s));
+    pb.arena.release(P.d_pixg);
+    pb.arena.release(P.d_off);
+    pb.arena.release(P.d_cst);
+    P.d_pixg = d_pixg2;
+    P.d_off = d_off2;
+    P.d_cst = d_cst2;
+    P.nvalid = nvalid_new;
+    P.npix = (int) nnew;
+    MGPU_RET(build_ell(pb));
+    MGPU_HIP(hipStreamSynchronize(s));
+    pb.arena.release(d_keep);
+    pb.arena.release(d_new);
+    pb.arena.release(d_cnt);
+    pb.arena.release(d_hscan);
+    return MGPU_OK;
+}
+
+int pointing_make_fp32(Problem &pb) {
+    Pointing &P = pb.pt;
+    if (P.d_w1f) return MGPU_OK;
+    MGPU_RET(pb.arena.alloc(&P.d_w1f, (size_t) P.m));
+    MGPU_RET(pb.arena.alloc(&P.d_w2f, (size_t) P.m));
+    MGPU_RET(pb.arena.alloc(&P.d_cw1f, (size_t) P.nh));
+    MGPU_RET(pb.arena.alloc(&P.d_cw2f, (size_t) P.nh));
+    k_to_f32<<<grid_for(P.m, 256), 256, 0, pb.stream>>>(P.d_w1, P.d_w1f, P.m);
+    k_to_f32<<<grid_for(P.m, 256), 256, 0, pb.stream>>>(P.d_w2, P.d_w2f, P.m);
+    k_to_f32<<<grid_for(P.nh, 256), 256, 0, pb.stream>>>(P.d_cw1, P.d_cw1f, P.nh);
+    k_to_f32<<<grid_for(P.nh, 256), 256, 0, pb.stream>>>(P.d_cw2, P.d_cw2f, P.nh);
+    if (!P.unit_w0) {
+        MGPU_RET(pb.arena.alloc(&P.d_w0f, (size_t) P.m));
+        MGPU_RET(pb.arena.alloc(&P.d_cw0f, (size_t) P.nh));
+        k_to_f32<<<grid_for(P.m, 256), 256, 0, pb.stream>>>(P.d_w0, P.d_w0f, P.m);
+        k_to_f32<<<grid_for(P.nh, 256), 256, 0, pb.stream>>>(P.d_cw0, P.d_cw0f, P.nh);
+    }
+    MGPU_LAUNCHED();
+    return MGPU_OK;
+}
+
+// local indices of MatLocalShape for every (t,k): valid rows 3(q+trash)+k,
+// trash rows k (lindices starts with -3,-2,-1)
+__global__ void k_local_indices(const int *__restrict__ q, int64_t m, int trash, int *__restrict__ out) {
+    int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        int qq = q[t];
+        for (int k = 0; k < 3; k++) out[3 * t + k] = qq >= 0 ? 3 * (qq + trash) + k : k;
+    }
+}
+
+__global__ void k_lindices(const int *__restrict__ pixg, int npix, int trash, int *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * trash) out[i] = i - 3;
+    if (i < npix)
+        for (int k = 0; k < 3; k++) out[3 * trash + 3 * i + k] = 3 * pixg[i] + k;
+}
+
+int pointing_local_indices(Problem &pb, int *d_out) {
+    k_local_indices<<<grid_for(pb.pt.m, 256), 256, 0, pb.stream>>>(pb.pt.d_q, pb.pt.m, pb.pt.trash, d_out);
+    MGPU_LAUNCHED();
+    return MGPU_OK;
+}
+
+int pointing_lindices(Problem &pb, int *d_out) {
+    int n = pb.pt.npix > 3 ? pb.pt.npix : 3;
+    k_lindices<<<grid_for(n, 256), 256, 0, pb.stream>>>(pb.pt.d_pixg, pb.pt.npix, pb.pt.trash, d_out);
+    MGPU_LAUNCHED();
+    return MGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// P: y = P x, sample-major, with fused per-(chunk,block) partial sums
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wg_sum_256(double v, double *red) {
+    // deterministic: fixed xor-butterfly inside each wave, then waves in order
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        for (int i = 0; i < nw; i++) s += red[i];
+    }
+    return s;  // valid in thread 0
+}
+
+// first block whose range contains t (start[nb] = covered); returns nb if t >= covered
+__device__ __forceinline__ int find_block(const int64_t *__restrict__ start, int nb, int64_t t) {
+    int lo = 0, hi = nb;  // invariant: start[lo] <= t < start[hi] (if t < covered)
+    if (t >= start[nb]) return nb;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (start[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <typename TX, typename TW, bool W0, bool PARTIALS>
+__global__ __launch_bounds__(kPBlock) void k_matvec(
+        const int *__restrict__ q, const TW *__restrict__ w0, const TW *__restrict__ w1,
+        const TW *__restrict__ w2, const TX *__restrict__ x, TX *__restrict__ y, int64_t m,
+        const int64_t *__restrict__ start, int nb, double *__restrict__ partials) {
+    __shared__ double red[kPBlock / 64];
+    const int64_t c = blockIdx.x;
+    const int64_t c0 = c * kChunk;
+    TX yv[kChunk / kPBlock];
+#pragma unroll
+    for (int i = 0; i < kChunk / kPBlock; i++) {
+        const int64_t t = c0 + threadIdx.x + (int64_t) i * kPBlock;
+        TX v = (TX) 0;
+        if (t < m) {
+            const int qq = q[t];
+            if (qq >= 0) {
+                const TX x0 = x[3 * qq], x1 = x[3 * qq + 1], x2 = x[3 * qq + 2];
+                TX acc = (TX) 0;
+                acc += (W0 ? (TX) w0[t] * x0 : x0);
+                acc += (TX) w1[t] * x1;
+                acc += (TX) w2[t] * x2;
+                v = acc;
+            }
+            y[t] = v;
+        }
+        yv[i] = v;
+    }
+    if (PARTIALS) {
+        const int64_t c1 = min(c0 + (int64_t) kChunk, m);
+        int b = find_block(start, nb, c0);
+        for (; b < nb && start[b] < c1; b++) {
+            const int64_t lo = start[b], hi = start[b + 1];
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < kChunk / kPBlock; i++) {
+                const int64_t t = c0 + threadIdx.x + (int64_t) i * kPBlock;
+                if (t >= lo && t < hi) s += (double) yv[i];
+            }
+            s = wg_sum_256(s, red);
+            if (threadIdx.x == 0) partials[c + b] = s;
+        }
+    }
+}
+
+int launch_matvec(Problem &pb, const double *d_x, double *d_y, bool partials) {
+    const Pointing &P = pb.pt;
+    const ToeplitzPlan &T = pb.tp;
+    const unsigned g = grid_for(P.m, kChunk);
+    partials = partials && T.nb > 0 && T.d_partials;
+    if (P.unit_w0) {
+        if (partials)
+            k_matvec<double, double, false, true><<<g, kPBlock, 0, pb.stream>>>(
+                    P.d_q, nullptr, P.d_w1, P.d_w2, d_x, d_y, P.m, T.d_start, T.nb, T.d_partials);
+        else
+            k_matvec<double, double, false, false><<<g, kPBlock, 0, pb.stream>>>(
+                    P.d_q, nullptr, P.d_w1, P.d_w2, d_x, d_y, P.m, T.d_start, T.nb, nullptr);
+    } else {
+        if (partials)
+            k_matvec<double, double, true, true><<<g, kPBlock, 0, pb.stream>>>(
+                    P.d_q, P.d_w0, P.d_w1, P.d_w2, d_x, d_y, P.m, T.d_start, T.nb, T.d_partials);
+        else
+            k_matvec<double, double, true, false><<<g, kPBlock, 0, pb.stream>>>(
+                    P.d_q, P.d_w0, P.d_w1, P.d_w2, d_x, d_y, P.m, T.d_start, T.nb, nullptr);
+    }
+    MGPU_LAUNCHED();
+    return MGPU_OK;
+}
+
+int launch_matvec_f32(Problem &pb, const float *d_x, float *d_y, bool partials) {
+    const Pointing &P = pb.pt;
+    const ToeplitzPlan &T = pb.tp;
+    const unsigned g = grid_for(P.m, kChunk);
+    partials = partials && T.nb > 0 && T.d_partials;
+    if (P.unit_w0) {
+        if (partials)
+            k_matvec<float, float, false, true><<<g, kPBlock, 0, pb.stream>>>(
+                    P.d_q, nullptr, P.d_w1f, P.d_w2f, d_x, d_y, P.m, T.d_start, T.nb, T.d_partials);
+        else
+            k_matvec<float, float, false, false><<<g, kPBlock, 0, pb.stream>>>(
+                    P.d_q, nullptr, P.d_w1f, P.d_w2f, d_x, d_y, P.m, T.d_start, T.nb, nullptr);
+    } else {
+        if (partials)
+            k_matvec<float, float, true, true><<<g, kPBlock, 0, pb.stream>>>(
+                    P.d_q, P.d_w0f, P.d_w1f, P.d_w2f, d_x, d_y, P.m, T.d_start, T.nb, T.d_partials);
+        else
+            k_matvec<float, float, true, false><<<g, kPBlock, 0, pb.stream>>>(
+                    P.d_q, P.d_w0f, P.d_w1f, P.d_w2f, d_x, d_y, P.m, T.d_start, T.nb, nullptr);
+    }
+    MGPU_LAUNCHED();
+    return MGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// P^T: x = P^T y over sliced ELL; optional fused dot <x, h>
+// ---------------------------------------------------------------------------
+constexpr int kPtBlock = 256;
+
+template <typename TY, typename TW, bool W0, bool DOT>
+__global__ __launch_bounds__(kPtBlock) void k_trmatvec(
+        const int64_t *__restrict__ base, const int *__restrict__ ct, const TW *__restrict__ cw0,
+        const TW *__restrict__ cw1, const TW *__restrict__ cw2, const TY *__restrict__ y,
+        double *__restrict__ x, int npix, const double *__restrict__ hvec,
+        double *__restrict__ dotpart) {
+    __shared__ double red[kPtBlock / 64];
+    const int qq = blockIdx.x * kPtBlock + threadIdx.x;
+    const int s = qq / kSlice, lane = qq % kSlice;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (qq < npix) {
+        const int64_t b0 = base[s];
+        const int64_t width = (base[s + 1] - b0) / kSlice;
+        for (int64_t i = 0; i < width; i++) {
+            const int64_t e = b0 + i * kSlice + lane;
+            const int t = ct[e];
+            if (t < 0) break;  // padding: hits of this lane are exhausted
+            const double yt = (double) y[t];
+            a0 += (W0 ? (double) cw0[e] * yt : yt);
+            a1 += (double) cw1[e] * yt;
+            a2 += (double) cw2[e] * yt;
+        }
+        x[3 * qq] = a0;
+        x[3 * qq + 1] = a1;
+        x[3 * qq + 2] = a2;
+    }
+    if (DOT) {
+        double d = 0.0;
+        if (qq < npix) {
+            d += hvec[3 * qq] * a0;
+            d += hvec[3 * qq + 1] * a1;
+            d += hvec[3 * qq + 2] * a2;
+        }
+        d = wg_sum_256(d, red);
+        if (threadIdx.x == 0) dotpart[blockIdx.x] = d;
+    }
+}
+
+int trmatvec_nparts(const Problem &pb) { return (int) grid_for(pb.pt.npix, kPtBlock); }
+
+template <typename TY, typename TW>
+static int launch_trmatvec_t(Problem &pb, const TY *d_y, double *d_x, const double *d_h, double *d_dot,
+                             const TW *cw0, const TW *cw1, const TW *cw2, int *nparts) {
+    const Pointing &P = pb.pt;
+    const unsigned g = grid_for(P.npix, kPtBlock);
+    if (nparts) *nparts = (int) g;
+    if (g == 0) return MGPU_OK;
+    const bool dot = d_h && d_dot;
+    if (P.unit_w0) {
+        if (dot)
+            k_trmatvec<TY, TW, false, true><<<g, kPtBlock, 0, pb.stream>>>(P.d_ell_base, P.d_ct, cw0, cw1, cw2,
+                                                                          d_y, d_x, P.npix, d_h, d_dot);
+        else
+            k_trmatvec<TY, TW, false, false><<<g, kPtBlock, 0, pb.stream>>>(P.d_ell_base, P.d_ct, cw0, cw1, cw2,
+                                                                           d_y, d_x, P.npix, nullptr, nullptr);
+    } else {
+        if (dot)
+            k_trmatvec<TY, TW, true, true><<<g, kPtBlock, 0, pb.stream>>>(P.d_ell_base, P.d_ct, cw0, cw1, cw2,
+                                                                         d_y, d_x, P.npix, d_h, d_dot);
+        else
+            k_trmatvec<TY, TW, true, false><<<g, kPtBlock, 0, pb.stream>>>(P.d_ell_base, P.d_ct, cw0, cw1, cw2,
+                                                                          d_y, d_x, P.npix, nullptr, nullptr);
+    }
+    MGPU_LAUNCHED();
+    return MGPU_OK;
+}
+
+int launch_trmatvec(Problem &pb, const double *d_y, double *d_x, const double *d_h, double *d_dot,
+                    int *nparts) {
+    const Pointing &P = pb.pt;
+    return launch_trmatvec_t<double, double>(pb, d_y, d_x, d_h, d_dot, P.d_cw0, P.d_cw1, P.d_cw2, nparts);
+}
+
+int launch_trmatvec_f32(Problem &pb, const float *d_y, double *d_x, const double *d_h, double *d_dot,
+                        int *nparts) {
+    const Pointing &P = pb.pt;
+    return launch_trmatvec_t<float, float>(pb, d_y, d_x, d_h, d_dot, P.d_cw0f, P.d_cw1f, P.d_cw2f, nparts);
+}
+
+}  // namespace mgpu

@@ -1,0 +1,894 @@
+// toeplitz.hip -- N^-1 = block-diagonal banded Toeplitz product with per-block
+// mean removal (gfx950).
+//
+// Reference semantics (one rank):
+//   apply_weights  pcg_true.c:578-600: lambda == 1 -> tod *= T_b[0], else stbmmProd.
+//   stbmmProd      toeplitz_wizard.c:80-111: V_b -= mean(V_b) for every block,
+//                  then mpi_stbmm (toeplitz_block.c:115-636), which on one rank
+//                  multiplies every block [idv, idv+n) by its banded symmetric
+//                  Toeplitz matrix T_b (band T_b[0..lambda-1]) with zero
+//                  boundaries: (T V)[i] = sum_{|i-j|<lambda, j in block} T[|i-j|] V[j].
+//                  The FFT shift-and-overlap of stmm_core (toeplitz.c:731-865)
+//                  computes exactly this product; blocks with n < lambda-1 are
+//                  left as they are (stmm_main, toeplitz.c:912-913).
+//
+// MI355X design.  Each tile is one workgroup.  A real tile of N = 2*NC samples
+// (NC complex points) is packed z[n] = x[2n] + i x[2n+1] and held in registers,
+// V complex values per lane (NT = NC/V lanes).  A mixed-radix Stockham FFT
+// (radix 4..32 butterflies in registers) exchanges data through LDS between
+// passes; when the complex tile does not fit the 160 KB LDS twice (FP64 at
+// NC = 16384) the real and imaginary planes are exchanged one after the other.
+// The last forward pass and the first inverse pass give each lane the pair of
+// butterflies {j, NC/R - j}, so the real-FFT untangling, the multiplication by
+// the real spectrum H of the circulant embedding and the re-tangling collapse
+// to one in-register step
+//     Z'[k] = a_k Z[k] + i b_k conj(Z[NC-k]),
+//     a_k = (Hs - Hd sin(2 pi k/N))/NC,  b_k = Hd cos(2 pi k/N)/NC,
+//     Hs = (H[k] + H[k+NC])/2,  Hd = (H[k] - H[k+NC])/2,
+// and the inverse transform is conj(FFT(conj(Z'))), reusing the forward passes.
+// Tiles read y - mean (zero outside the block) straight from HBM in the first
+// pass and write only their central L = N - 2(lambda-1) outputs.  Only the
+// filter spectrum differs from the reference's complex multiply: H is real and
+// even (T_circ is real and symmetric), so the imaginary rounding noise that
+// FFTW leaves in C_fft (and that toeplitz.c:587-588 multiplies in) is dropped.
+#include "mgpu_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+namespace mgpu {
+
+// ---------------------------------------------------------------------------
+// compile-time schedule
+// ---------------------------------------------------------------------------
+// Forward schedule: the last pass has radix V/2 (two paired butterflies per
+// lane); the other LOGNC - (LOGV-1) bits are spread as evenly as possible over
+// passes of at most LOGR bits.  The inverse schedule is the reverse.
+constexpr int n_passes(int LOGNC, int LOGV, int LOGR) {
+    return 1 + (LOGNC - (LOGV - 1) + LOGR - 1) / LOGR;
+}
+
+constexpr int fwd_bits(int LOGNC, int LOGV, int LOGR, int p) {
+    return (p == n_passes(LOGNC, LOGV, LOGR) - 1)
+                   ? LOGV - 1
+                   : (LOGNC - (LOGV - 1)) / (n_passes(LOGNC, LOGV, LOGR) - 1) +
+                             ((p < (LOGNC - (LOGV - 1)) % (n_passes(LOGNC, LOGV, LOGR) - 1)) ? 1 : 0);
+}
+
+// bits of pass p of the forward (INV=false) or inverse (reversed) schedule
+constexpr int sched_bits(int LOGNC, int LOGV, int LOGR, bool INV, int p) {
+    return INV ? fwd_bits(LOGNC, LOGV, LOGR, n_passes(LOGNC, LOGV, LOGR) - 1 - p)
+               : fwd_bits(LOGNC, LOGV, LOGR, p);
+}
+
+constexpr int sched_logns(int LOGNC, int LOGV, int LOGR, bool INV, int p) {
+    return p == 0 ? 0
+                  : sched_logns(LOGNC, LOGV, LOGR, INV, p - 1) + sched_bits(LOGNC, LOGV, LOGR, INV, p - 1);
+}
+
+#ifndef TILE14_F64_LOGV
+#define TILE14_F64_LOGV 5
+#endif
+#ifndef TILE14_F64_LOGR
+#define TILE14_F64_LOGR 4
+#endif
+#ifndef TILE14_F32_LOGV
+#define TILE14_F32_LOGV 5
+#endif
+#ifndef TILE14_F32_LOGR
+#define TILE14_F32_LOGR 4
+#endif
+
+constexpr int kPadShift = 4;  // one padding element every 16 in LDS (bank spread)
+__device__ __forceinline__ int lpad(int i) { return i + (i >> kPadShift); }
+constexpr int lds_elems(int NC) { return NC + (NC >> kPadShift); }
+
+// ---------------------------------------------------------------------------
+// in-register DFT of size 2^LR (forward), natural order in and out
+// ---------------------------------------------------------------------------
+// cos/sin(2 pi j / 32), j = 0..31
+__device__ constexpr double kCos32[32] = {
+        1.0,
+        0.98078528040323044913,
+        0.92387953251128675613,
+        0.83146961230254523708,
+        0.70710678118654752440,
+        0.55557023301960222474,
+        0.38268343236508977173,
+        0.19509032201612826785,
+        0.0,
+        -0.19509032201612826785,
+        -0.38268343236508977173,
+        -0.55557023301960222474,
+        -0.70710678118654752440,
+        -0.83146961230254523708,
+        -0.92387953251128675613,
+        -0.98078528040323044913,
+        -1.0,
+        -0.98078528040323044913,
+        -0.92387953251128675613,
+        -0.83146961230254523708,
+        -0.70710678118654752440,
+        -0.55557023301960222474,
+        -0.38268343236508977173,
+        -0.19509032201612826785,
+        0.0,
+        0.19509032201612826785,
+        0.38268343236508977173,
+        0.55557023301960222474,
+        0.70710678118654752440,
+        0.83146961230254523708,
+        0.92387953251128675613,
+        0.98078528040323044913,
+};
+
+template <int N>
+__device__ constexpr int bitrev(int k) {
+    int r = 0;
+    for (int b = 1; b < N; b <<= 1) {
+        r = (r << 1) | (k & 1);
+        k >>= 1;
+    }
+    return r;
+}
+
+template <typename R, int LEN, int J>
+__device__ __forceinline__ void twiddle_dif(R &dr, R &di) {
+    // (dr + i di) * exp(-2 pi i J / LEN)
+    constexpr int e = (32 / LEN) * J;  // index into the 32-point table
+    if constexpr (J == 0) {
+        return;
+    } else if constexpr (4 * J == LEN) {  // -i
+        R t = dr;
+        dr = di;
+        di = -t;
+    } else if constexpr (8 * J == LEN) {  // (1 - i)/sqrt2
+        const R c = (R) kCos32[4];
+        R a = dr, b = di;
+        dr = c * (a + b);
+        di = c * (b - a);
+    } else if constexpr (8 * J == 3 * LEN) {  // (-1 - i)/sqrt2
+        const R c = (R) kCos32[4];
+        R a = dr, b = di;
+        dr = c * (b - a);
+        di = -c * (a + b);
+    } else {
+        const R c = (R) kCos32[e], s = (R) kCos32[(e + 24) & 31];  // sin(x) = cos(x - pi/2)
+        R a = dr, b = di;
+        dr = fma(a, c, b * s);
+        di = fma(b, c, -(a * s));
+    }
+}
+
+template <typename R, int RAD, int LEN, int START, int J>
+struct DifStage {
+    static __device__ __forceinline__ void run(R *re, R *im) {
+        if constexpr (START < RAD) {
+            if constexpr (J < LEN / 2) {
+                constexpr int i0 = START + J, i1 = START + J + LEN / 2;
+                R ar = re[i0], ai = im[i0], br = re[i1], bi = im[i1];
+                re[i0] = ar + br;
+                im[i0] = ai + bi;
+                R dr = ar - br, di = ai - bi;
+                twiddle_dif<R, LEN, J>(dr, di);
+                re[i1] = dr;
+                im[i1] = di;
+                DifStage<R, RAD, LEN, START, J + 1>::run(re, im);
+            } else {
+                DifStage<R, RAD, LEN, START + LEN, 0>::run(re, im);
+            }
+        }
+    }
+};
+
+template <typename R, int RAD, int LEN>
+struct DifAll {
+    static __device__ __forceinline__ void run(R *re, R *im) {
+        if constexpr (LEN >= 2) {
+            DifStage<R, RAD, LEN, 0, 0>::run(re, im);
+            DifAll<R, RAD, LEN / 2>::run(re, im);
+        }
+    }
+};
+
+// forward DFT of RAD points at re/im[0..RAD), result in natural order
+template <typename R, int RAD>
+__device__ __forceinline__ void dft(R *re, R *im) {
+    if constexpr (RAD > 1) {
+        DifAll<R, RAD, RAD>::run(re, im);
+        R tr[RAD], ti[RAD];
+#pragma unroll
+        for (int k = 0; k < RAD; k++) {
+            tr[k] = re[bitrev<RAD>(k)];
+            ti[k] = im[bitrev<RAD>(k)];
+        }
+#pragma unroll
+        for (int k = 0; k < RAD; k++) {
+            re[k] = tr[k];
+            im[k] = ti[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stockham pass pieces
+// ---------------------------------------------------------------------------
+template <int NT, bool PAIRED>
+__device__ __forceinline__ int slot_j(int t, int b) {
+    if constexpr (!PAIRED) {
+        return t + b * NT;
+    } else {
+        return b == 0 ? t : (t == 0 ? NT : 2 * NT - t);
+    }
+}
+
+template <typename R>
+struct Cx {
+    R re, im;
+};
+
+// twiddle + butterfly of every slot of a pass (radix 2^LR, sub-FFT size 2^LOGNS)
+template <typename R, int LOGNC, int LOGV, int LR, int LOGNS, bool PAIRED>
+__device__ __forceinline__ void pass_compute(R *re, R *im, const Cx<R> *__restrict__ tw, int t) {
+    constexpr int V = 1 << LOGV, RAD = 1 << LR, B = V / RAD, NT = (1 << LOGNC) / V;
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+        R *vr = re + b * RAD;
+        R *vi = im + b * RAD;
+        if constexpr (LOGNS > 0) {
+            const int j = slot_j<NT, PAIRED>(t, b);
+            const int k = j & ((1 << LOGNS) - 1);
+            const Cx<R> w1 = tw[k << (LOGNC - LOGNS - LR)];
+            R wr = w1.re, wi = w1.im;
+#pragma unroll
+            for (int r = 1; r < RAD; r++) {
+                const R xr = vr[r], xi = vi[r];
+                vr[r] = fma(xr, wr, -(xi * wi));
+                vi[r] = fma(xr, wi, xi * wr);
+                if (r + 1 < RAD) {
+                    const R nr = fma(wr, w1.re, -(wi * w1.im));
+                    const R ni = fma(wr, w1.im, wi * w1.re);
+                    wr = nr;
+                    wi = ni;
+                }
+            }
+        }
+        dft<R, RAD>(vr, vi);
+    }
+}
+
+// write a pass's outputs (one plane) to LDS.  With sub-FFT size Ns >= 16 the
+// padded address is linear in r (lpad(base + r Ns) = lpad(base) + r Ns 17/16),
+// so one address per slot plus instruction offsets serve all r.
+template <typename R, int LOGNC, int LOGV, int LR, int LOGNS, bool PAIRED>
+__device__ __forceinline__ void pass_store(R *__restrict__ plane, const R *v, int t) {
+    constexpr int V = 1 << LOGV, RAD = 1 << LR, B = V / RAD, NT = (1 << LOGNC) / V;
+    constexpr int NS = 1 << LOGNS;
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+        const int j = slot_j<NT, PAIRED>(t, b);
+        const int k = j & (NS - 1);
+        const int base = ((j >> LOGNS) << (LOGNS + LR)) + k;
+        if constexpr (NS >= (1 << kPadShift) || (LOGNS == 0 && LR <= kPadShift)) {
+            // LOGNS == 0, RAD <= 16: base = j RAD, so base % 16 + r < 16 never crosses a pad boundary
+            R *p = plane + lpad(base);
+#pragma unroll
+            for (int r = 0; r < RAD; r++) p[r * (NS + (NS >> kPadShift))] = v[b * RAD + r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < RAD; r++) plane[lpad(base + (r << LOGNS))] = v[b * RAD + r];
+        }
+    }
+}
+
+// read the inputs of a pass (one plane) from LDS; the stride NC/RAD is a
+// multiple of 16, so the padded address is linear in r
+template <typename R, int LOGNC, int LOGV, int LR, bool PAIRED>
+__device__ __forceinline__ void pass_load(const R *__restrict__ plane, R *v, int t) {
+    constexpr int V = 1 << LOGV, RAD = 1 << LR, B = V / RAD, NT = (1 << LOGNC) / V;
+    constexpr int S = (1 << LOGNC) >> LR;
+    static_assert(S % (1 << kPadShift) == 0, "pass stride must be a multiple of the pad period");
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+        const int j = slot_j<NT, PAIRED>(t, b);
+        const R *p = plane + lpad(j);
+#pragma unroll
+        for (int r = 0; r < RAD; r++) v[b * RAD + r] = p[r * (S + (S >> kPadShift))];
+    }
+}
+
+template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO, bool INV, int P>
+struct Passes {
+    static __device__ __forceinline__ void run(R *re, R *im, R *lds, const Cx<R> *__restrict__ tw, int t) {
+        constexpr int NP = n_passes(LOGNC, LOGV, LOGR);
+        constexpr int LR = sched_bits(LOGNC, LOGV, LOGR, INV, P);
+        constexpr int LOGNS = sched_logns(LOGNC, LOGV, LOGR, INV, P);
+        constexpr bool PAIRED = INV ? (P == 0) : (P == NP - 1);
+        pass_compute<R, LOGNC, LOGV, LR, LOGNS, PAIRED>(re, im, tw, t);
+        if constexpr (P + 1 < NP) {
+            constexpr int LRN = sched_bits(LOGNC, LOGV, LOGR, INV, P + 1);
+            constexpr bool PAIREDN = INV ? false : (P + 1 == NP - 1);
+            constexpr int NCP = lds_elems(1 << LOGNC);
+            if constexpr (TWO) {
+                pass_store<R, LOGNC, LOGV, LR, LOGNS, PAIRED>(lds, re, t);
+                pass_store<R, LOGNC, LOGV, LR, LOGNS, PAIRED>(lds + NCP, im, t);
+                __syncthreads();
+                pass_load<R, LOGNC, LOGV, LRN, PAIREDN>(lds, re, t);
+                pass_load<R, LOGNC, LOGV, LRN, PAIREDN>(lds + NCP, im, t);
+                __syncthreads();
+            } else {
+                pass_store<R, LOGNC, LOGV, LR, LOGNS, PAIRED>(lds, re, t);
+                __syncthreads();
+                pass_load<R, LOGNC, LOGV, LRN, PAIREDN>(lds, re, t);
+                __syncthreads();
+                pass_store<R, LOGNC, LOGV, LR, LOGNS, PAIRED>(lds, im, t);
+                __syncthreads();
+                pass_load<R, LOGNC, LOGV, LRN, PAIREDN>(lds, im, t);
+                __syncthreads();
+            }
+            Passes<R, LOGNC, LOGV, LOGR, TWO, INV, P + 1>::run(re, im, lds, tw, t);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// the tile kernel
+// ---------------------------------------------------------------------------
+struct TileArgs {
+    const void *y;           // TOD in (R)
+    void *z;                 // TOD out (R)
+    const TileDesc *tiles;
+    const void *coef;        // Cx<R> [nfilters * NC]
+    const void *tw;          // Cx<R> [NC]
+    const double *mean;      // [nb]
+    int lambda;
+    int L;
+};
+
+template <typename R>
+__device__ __forceinline__ void spectral_pair(R &pr, R &pi, R &qr, R &qi, Cx<R> cp, Cx<R> cq) {
+    // Z'[k] = a_k P + i b_k conj(Q); Z'[k'] = a_k' Q + i b_k' conj(P); then conjugate
+    const R npr = fma(cp.re, pr, cp.im * qi);
+    const R npi = -fma(cp.re, pi, cp.im * qr);
+    const R nqr = fma(cq.re, qr, cq.im * pi);
+    const R nqi = -fma(cq.re, qi, cq.im * pr);
+    pr = npr;
+    pi = npi;
+    qr = nqr;
+    qi = nqi;
+}
+
+template <typename R>
+__device__ __forceinline__ void spectral_self(R &pr, R &pi, Cx<R> c) {
+    const R npr = fma(c.re, pr, c.im * pi);
+    const R npi = -fma(c.re, pi, c.im * pr);
+    pr = npr;
+    pi = npi;
+}
+
+template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO>
+__global__ __launch_bounds__((1 << LOGNC) >> LOGV) void k_tiles(TileArgs a) {
+    constexpr int NC = 1 << LOGNC, V = 1 << LOGV, NT = NC / V;
+    constexpr int NP = n_passes(LOGNC, LOGV, LOGR);
+    constexpr int LR0 = fwd_bits(LOGNC, LOGV, LOGR, 0);
+    static_assert(LOGV - 1 <= LOGR, "paired pass radix exceeds the maximum radix");
+    // FP64 tiles with 16+ values per lane run close to the register budget:
+    // keep the scheduler from hoisting every load of a phase at once
+    constexpr bool SCHED_FENCE = sizeof(R) == 8 && LOGV >= 4;
+    constexpr int RAD0 = 1 << LR0, B0 = V / RAD0;
+    constexpr int RL = V / 2;  // radix of the paired passes
+    extern __shared__ unsigned char smem_raw[];
+    R *lds = reinterpret_cast<R *>(smem_raw);
+
+    const int t = threadIdx.x;
+    const TileDesc d = a.tiles[blockIdx.x];
+    const R *__restrict__ y = reinterpret_cast<const R *>(a.y);
+    R *__restrict__ z = reinterpret_cast<R *>(a.z);
+    const Cx<R> *__restrict__ tw = reinterpret_cast<const Cx<R> *>(a.tw);
+    const Cx<R> *__restrict__ cf = reinterpret_cast<const Cx<R> *>(a.coef) + (int64_t) d.filter * NC;
+    const R mean = (R) a.mean[d.block];
+    const int64_t in0 = d.out_lo - (a.lambda - 1);
+
+    // block-relative 32-bit offsets from a uniform base: loads are clamped into
+    // the block and masked, so the first pass has no divergent branches
+    const R *__restrict__ yb = y + d.blk_lo;
+    R *__restrict__ zb = z + d.blk_lo;
+    const int nblk = (int) (d.blk_hi - d.blk_lo);
+    const int rel0 = (int) (in0 - d.blk_lo);
+
+    R re[V], im[V];
+    // first forward pass inputs: z[n] = x[2n] + i x[2n+1], n = j + r NC/RAD0
+#pragma unroll
+    for (int b = 0; b < B0; b++) {
+        const int j = t + b * NT;
+#pragma unroll
+        for (int r = 0; r < RAD0; r++) {
+            const int n = j + r * (NC / RAD0);
+            const int i0 = rel0 + 2 * n, i1 = i0 + 1;
+            const R v0 = yb[(unsigned) min(max(i0, 0), nblk - 1)];
+            const R v1 = yb[(unsigned) min(max(i1, 0), nblk - 1)];
+            re[b * RAD0 + r] = ((unsigned) i0 < (unsigned) nblk) ? v0 - mean : (R) 0;
+            im[b * RAD0 + r] = ((unsigned) i1 < (unsigned) nblk) ? v1 - mean : (R) 0;
+        }
+        if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+    }
+    Passes<R, LOGNC, LOGV, LOGR, TWO, false, 0>::run(re, im, lds, tw, t);
+
+    // spectral step on the paired slots: slot 0 holds k = t + r 2NT, slot 1 holds
+    // k = j1 + r 2NT with j1 = 2NT - t (t > 0) or NT (t == 0)
+    if (t != 0) {
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const int k = t + r * 2 * NT;
+            spectral_pair<R>(re[r], im[r], re[RL + RL - 1 - r], im[RL + RL - 1 - r], cf[k], cf[NC - k]);
+            if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+        spectral_self<R>(re[0], im[0], cf[0]);
+        spectral_self<R>(re[RL / 2], im[RL / 2], cf[(RL / 2) * 2 * NT]);
+#pragma unroll
+        for (int r = 1; r < RL / 2; r++) {
+            const int k = r * 2 * NT;
+            spectral_pair<R>(re[r], im[r], re[RL - r], im[RL - r], cf[k], cf[NC - k]);
+        }
+#pragma unroll
+        for (int r = 0; r < RL / 2; r++) {
+            const int k = NT + r * 2 * NT;
+            spectral_pair<R>(re[RL + r], im[RL + r], re[RL + RL - 1 - r], im[RL + RL - 1 - r], cf[k], cf[NC - k]);
+        }
+    }
+
+    Passes<R, LOGNC, LOGV, LOGR, TWO, true, 0>::run(re, im, lds, tw, t);
+
+    // last inverse pass (radix RAD0, not paired): position n = j + r NC/RAD0 holds
+    // conj(y[2n] + i y[2n+1]); tile sample i -> block offset rel0 + i
+    const int lo = a.lambda - 1, hi = a.lambda - 1 + a.L;
+#pragma unroll
+    for (int b = 0; b < B0; b++) {
+        const int j = t + b * NT;
+#pragma unroll
+        for (int r = 0; r < RAD0; r++) {
+            const int n = j + r * (NC / RAD0);
+            const int i0 = 2 * n;
+            const int o0 = rel0 + i0;
+            if (i0 >= lo && i0 < hi && o0 < nblk) zb[(unsigned) o0] = re[b * RAD0 + r];
+            if (i0 + 1 >= lo && i0 + 1 < hi && o0 + 1 < nblk) zb[(unsigned) (o0 + 1)] = -im[b * RAD0 + r];
+        }
+    }
+    (void) NP;
+}
+
+// ---------------------------------------------------------------------------
+// mean, short blocks, tail, lambda == 1, direct fallback
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_block_t(const int64_t *__restrict__ start, int nb, int64_t t) {
+    if (t >= start[nb]) return nb;
+    int lo = 0, hi = nb;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (start[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_partials(const R *__restrict__ y, int64_t m,
+                                                  const int64_t *__restrict__ start, int nb,
+                                                  double *__restrict__ partials) {
+    __shared__ double red[4];
+    const int64_t c = blockIdx.x, c0 = c * kChunk;
+    const int64_t c1 = min(c0 + (int64_t) kChunk, m);
+    int b = find_block_t(start, nb, c0);
+    for (; b < nb && start[b] < c1; b++) {
+        const int64_t lo = start[b], hi = start[b + 1];
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < kChunk / 256; i++) {
+            const int64_t t = c0 + threadIdx.x + (int64_t) i * 256;
+            if (t < m && t >= lo && t < hi) s += (double) y[t];
+        }
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) partials[c + b] = ((red[0] + red[1]) + red[2]) + red[3];
+    }
+}
+
+// mean[b] = (sum of the block's partials, fixed order) / n_b
+__global__ __launch_bounds__(256) void k_block_mean(const double *__restrict__ partials,
+                                                    const int64_t *__restrict__ start, int nb,
+                                                    double *__restrict__ mean) {
+    __shared__ double red[256];
+    const int b = blockIdx.x;
+    const int64_t lo = start[b], hi = start[b + 1];
+    const int64_t first = lo / kChunk + b, last = (hi - 1) / kChunk + b;
+    double s = 0.0;
+    for (int64_t i = first + threadIdx.x; i <= last; i += 256) s += partials[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mean[b] = red[0] / (double) (hi - lo);
+}
+
+// z = y - mean over the listed (short) blocks
+template <typename R>
+__global__ void k_mean_only(const R *__restrict__ y, R *__restrict__ z, const int64_t *__restrict__ start,
+                            const int *__restrict__ blocks, const double *__restrict__ mean) {
+    const int b = blocks[blockIdx.y];
+    const int64_t lo = start[b], hi = start[b + 1];
+    const R mu = (R) mean[b];
+    for (int64_t t = lo + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < hi;
+         t += (int64_t) gridDim.x * blockDim.x)
+        z[t] = y[t] - mu;
+}
+
+template <typename R>
+__global__ void k_copy_tail(const R *__restrict__ y, R *__restrict__ z, int64_t lo, int64_t hi) {
+    int64_t t = lo + (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < hi) z[t] = y[t];
+}
+
+// lambda == 1: z = T_b[0] * y inside blocks (no mean removal), copy after them
+template <typename R>
+__global__ void k_diag(const R *__restrict__ y, R *__restrict__ z, int64_t m, const int64_t *__restrict__ start,
+                       int nb, const double *__restrict__ T0) {
+    int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int b = find_block_t(start, nb, t);
+    z[t] = b < nb ? (R) T0[b] * y[t] : y[t];
+}
+
+// direct banded product for bands too wide for an LDS tile (slow path)
+template <typename R>
+__global__ void k_direct(const R *__restrict__ y, R *__restrict__ z, const int64_t *__restrict__ start,
+                         const int *__restrict__ blocks, const double *__restrict__ mean,
+                         const double *__restrict__ invtt, int lambda) {
+    const int b = blocks[blockIdx.y];
+    const int64_t lo = start[b], hi = start[b + 1];
+    const double mu = mean[b];
+    const double *T = invtt + (int64_t) b * lambda;
+    for (int64_t i = lo + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < hi;
+         i += (int64_t) gridDim.x * blockDim.x) {
+        const int64_t j0 = max(lo, i - (lambda - 1)), j1 = min(hi, i + lambda);
+        double acc = 0.0;
+        for (int64_t j = j0; j < j1; j++) acc += T[j > i ? j - i : i - j] * ((double) y[j] - mu);
+        z[i] = (R) acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// filter spectra: H = FFT_N(T_circ) (toeplitz.c:441-470), then (a_k, b_k)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_circ_fft(const double *__restrict__ invtt, const int *__restrict__ src_block,
+                                                    int lambda, int log2n, double2 *__restrict__ buf) {
+    const int f = blockIdx.x;
+    const int N = 1 << log2n;
+    double2 *a = buf + (int64_t) f * N;
+    const double *T = invtt + (int64_t) src_block[f] * lambda;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        int r = (int) (__builtin_bitreverse32((unsigned) i) >> (32 - log2n));
+        double v = 0.0;
+        if (r == 0)
+            v = T[0];
+        else if (r < lambda)
+            v = T[r];
+        else if (N - r < lambda)
+            v = T[N - r];
+        a[i] = make_double2(v, 0.0);
+    }
+    __syncthreads();
+    for (int s = 1; s <= log2n; s++) {
+        const int half = 1 << (s - 1);
+        for (int idx = threadIdx.x; idx < N / 2; idx += blockDim.x) {
+            const int grp = idx >> (s - 1), j = idx & (half - 1);
+            const int i0 = (grp << s) + j, i1 = i0 + half;
+            double sn, cs;
+            sincospi(-(double) j / (double) half, &sn, &cs);
+            const double2 u = a[i0], v = a[i1];
+            const double vr = v.x * cs - v.y * sn, vi = v.x * sn + v.y * cs;
+            a[i0] = make_double2(u.x + vr, u.y + vi);
+            a[i1] = make_double2(u.x - vr, u.y - vi);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_coef(const double2 *__restrict__ buf, int log2n, int nf, double *__restrict__ c64,
+                       float *__restrict__ c32) {
+    const int NC = 1 << (log2n - 1), N = 1 << log2n;
+    const int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t) nf * NC) return;
+    const int f = (int) (idx / NC), k = (int) (idx % NC);
+    const double2 *H = buf + (int64_t) f * N;
+    const double hs = 0.5 * (H[k].x + H[k + NC].x), hd = 0.5 * (H[k].x - H[k + NC].x);
+    double sn, cs;
+    sincospi(2.0 * (double) k / (double) N, &sn, &cs);
+    const double inv = 1.0 / (double) NC;
+    const double ak = (hs - hd * sn) * inv, bk = (hd * cs) * inv;
+    c64[2 * idx] = ak;
+    c64[2 * idx + 1] = bk;
+    c32[2 * idx] = (float) ak;
+    c32[2 * idx + 1] = (float) bk;
+}
+
+__global__ void k_twiddles(int NC, double *__restrict__ t64, float *__restrict__ t32) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= NC) return;
+    double sn, cs;
+    sincospi(-2.0 * (double) m / (double) NC, &sn, &cs);
+    t64[2 * m] = cs;
+    t64[2 * m + 1] = sn;
+    t32[2 * m] = (float) cs;
+    t32[2 * m + 1] = (float) sn;
+}
+
+// ---------------------------------------------------------------------------
+// host side: plan + dispatch
+// ---------------------------------------------------------------------------
+static int max_log2n(bool fp32) { return fp32 ? 16 : 15; }
+static int min_log2n() { return 10; }
+
+// tile length choice: minimise N (log2 N + c) / L over the sizes that fit LDS
+static int choose_log2n(int lambda, int64_t nmax, bool fp32, int forced) {
+    if (forced > 0) return forced;
+    int best = -1;
+    double best_cost = 1e300;
+    for (int l2 = min_log2n(); l2 <= max_log2n(fp32); l2++) {
+        const double N = (double) (1 << l2);
+        const double L = N - 2.0 * (lambda - 1);
+        if (L < 1) continue;
+        // no benefit in tiles much longer than the longest block
+        const double useful = std::min(L, (double) nmax);
+        const double cost = N * (l2 + 8.0) / useful;
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = l2;
+        }
+    }
+    return best;
+}
+
+int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const double *d_invtt,
+                  const double *h_invtt) {
+    ToeplitzPlan &T = pb.tp;
+    hipStream_t s = pb.stream;
+    MGPU_ARG(nb > 0, "nb_blocks=%d must be positive", nb);
+    MGPU_ARG(lambda >= 1, "lambda=%d must be >= 1", lambda);
+    T.lambda = lambda;
+    T.nb = nb;
+    T.m = pb.pt.m;
+    T.h_start.assign(nb + 1, 0);
+    int64_t nmax = 0;
+    for (int b = 0; b < nb; b++) {
+        MGPU_ARG(block_sizes[b] > 0, "local_blocks_sizes[%d]=%d must be positive", b, block_sizes[b]);
+        T.h_start[b + 1] = T.h_start[b] + block_sizes[b];
+        nmax = std::max<int64_t>(nmax, block_sizes[b]);
+    }
+    T.covered = T.h_start[nb];
+    MGPU_ARG(T.covered <= T.m, "sum of local_blocks_sizes (%lld) exceeds m (%lld)", (long long) T.covered,
+             (long long) T.m);
+    MGPU_RET(pb.arena.alloc(&T.d_start, (size_t) nb + 1));
+    MGPU_HIP(hipMemcpyAsync(T.d_start, T.h_start.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, s));
+    // host copy of invtt (dedupe filters, T0)
+    std::vector<double> hinv;
+    const double *inv = h_invtt;
+    if (!inv) {
+        hinv.resize((size_t) nb * lambda);
+        MGPU_HIP(hipMemcpyAsync(hinv.data(), d_invtt, sizeof(double) * hinv.size(), hipMemcpyDeviceToHost, s));
+        MGPU_HIP(hipStreamSynchronize(s));
+        inv = hinv.data();
+    }
+    std::vector<double> T0(nb);
+    for (int b = 0; b < nb; b++) T0[b] = inv[(int64_t) b * lambda];
+    MGPU_RET(pb.arena.alloc(&T.d_T0, (size_t) nb));
+    MGPU_HIP(hipMemcpyAsync(T.d_T0, T0.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
+
+    T.nchunks = (T.m + kChunk - 1) / kChunk;
+    MGPU_RET(pb.arena.alloc(&T.d_partials, (size_t) (T.nchunks + nb)));
+    MGPU_RET(pb.arena.alloc(&T.d_mean, (size_t) nb));
+    if (lambda == 1) {
+        MGPU_HIP(hipStreamSynchronize(s));
+        return MGPU_OK;
+    }
+
+    const bool fp32 = pb.opt.precision == MGPU_MIXED;
+    // FP64 tiles use the FP64 limit even in mixed mode (the FP64 residual refresh needs them)
+    int l2 = choose_log2n(lambda, nmax, false, pb.opt.tile_log2);
+    (void) fp32;
+    T.h_short_blocks.clear();
+    std::vector<int> direct_blocks;
+    if (l2 < 0 || l2 < min_log2n() || l2 > max_log2n(false)) {
+        l2 = -1;
+    }
+    T.log2n = l2;
+    // dedupe filters by content
+    std::vector<int> filter_src;
+    T.h_block_filter.assign(nb, -1);
+    {
+        std::unordered_map<uint64_t, std::vector<int>> by_hash;
+        for (int b = 0; b < nb; b++) {
+            const double *Tb = inv + (int64_t) b * lambda;
+            uint64_t h = 1469598103934665603ull;
+            const unsigned char *bytes = reinterpret_cast<const unsigned char *>(Tb);
+            for (size_t i = 0; i < sizeof(double) * (size_t) lambda; i++) h = (h ^ bytes[i]) * 1099511628211ull;
+            int fid = -1;
+            auto &cand = by_hash[h];
+            for (int f : cand)
+                if (std::memcmp(inv + (int64_t) filter_src[f] * lambda, Tb, sizeof(double) * lambda) == 0) {
+                    fid = f;
+                    break;
+                }
+            if (fid < 0) {
+                fid = (int) filter_src.size();
+                filter_src.push_back(b);
+                cand.push_back(fid);
+            }
+            T.h_block_filter[b] = fid;
+        }
+    }
+    std::vector<TileDesc> tiles;
+    if (l2 > 0) {
+        const int N = 1 << l2;
+        T.L = N - 2 * (lambda - 1);
+        for (int b = 0; b < nb; b++) {
+            const int64_t n = block_sizes[b];
+            if (n < lambda - 1) {
+                T.h_short_blocks.push_back(b);
+                continue;
+            }
+            const int64_t nt = (n + T.L - 1) / T.L;
+            for (int64_t k = 0; k < nt; k++)
+                tiles.push_back(TileDesc{T.h_start[b], T.h_start[b + 1], T.h_start[b] + k * T.L, b,
+                                         T.h_block_filter[b]});
+        }
+    } else {
+        for (int b = 0; b < nb; b++) {
+            if (block_sizes[b] < lambda - 1)
+                T.h_short_blocks.push_back(b);
+            else
+                direct_blocks.push_back(b);
+        }
+    }
+    T.ntiles = (int) tiles.size();
+    if (T.ntiles) {
+        MGPU_RET(pb.arena.alloc(&T.d_tiles, tiles.size()));
+        MGPU_HIP(hipMemcpyAsync(T.d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice, s));
+        T.nfilters = (int) filter_src.size();
+        const int N = 1 << l2, NC = N / 2;
+        int *d_src;
+        double2 *d_buf;
+        MGPU_RET(pb.arena.alloc(&d_src, filter_src.size()));
+        MGPU_HIP(hipMemcpyAsync(d_src, filter_src.data(), sizeof(int) * filter_src.size(), hipMemcpyHostToDevice, s));
+        MGPU_RET(pb.arena.alloc(&d_buf, (size_t) T.nfilters * N));
+        // the spectra need invtt on the device
+        const double *d_inv = d_invtt;
+        double *d_inv_tmp = nullptr;
+        if (!d_inv) {
+            MGPU_RET(pb.arena.alloc(&d_inv_tmp, (size_t) nb * lambda));
+            MGPU_HIP(hipMemcpyAsync(d_inv_tmp, inv, sizeof(double) * (size_t) nb * lambda, hipMemcpyHostToDevice, s));
+            d_inv = d_inv_tmp;
+        }
+        k_circ_fft<<<T.nfilters, 1024, 0, s>>>(d_inv, d_src, lambda, l2, d_buf);
+        MGPU_LAUNCHED();
+        MGPU_RET(pb.arena.alloc(&T.d_coef64, (size_t) T.nfilters * NC * 2));
+        MGPU_RET(pb.arena.alloc(&T.d_coef32, (size_t) T.nfilters * NC * 2));
+        k_coef<<<grid_for((int64_t) T.nfilters * NC, 256), 256, 0, s>>>(d_buf, l2, T.nfilters, T.d_coef64,
+                                                                          T.d_coef32);
+        MGPU_LAUNCHED();
+        MGPU_RET(pb.arena.alloc(&T.d_tw64, (size_t) NC * 2));
+        MGPU_RET(pb.arena.alloc(&T.d_tw32, (size_t) NC * 2));
+        k_twiddles<<<grid_for(NC, 256), 256, 0, s>>>(NC, T.d_tw64, T.d_tw32);
+        MGPU_LAUNCHED();
+        MGPU_HIP(hipStreamSynchronize(s));
+        pb.arena.release(d_buf);
+        pb.arena.release(d_src);
+        if (d_inv_tmp) pb.arena.release(d_inv_tmp);
+    }
+    if (!direct_blocks.empty()) {
+        // keep invtt on device for the direct product
+        double *d_inv_keep;
+        MGPU_RET(pb.arena.alloc(&d_inv_keep, (size_t) nb * lambda));
+        MGPU_HIP(hipMemcpyAsync(d_inv_keep, inv, sizeof(double) * (size_t) nb * lambda, hipMemcpyHostToDevice, s));
+        T.d_direct_invtt = d_inv_keep;
+        T.h_direct_blocks = direct_blocks;
+        MGPU_RET(pb.arena.alloc(&T.d_direct_blocks, direct_blocks.size()));
+        MGPU_HIP(hipMemcpyAsync(T.d_direct_blocks, direct_blocks.data(), sizeof(int) * direct_blocks.size(),
+                                hipMemcpyHostToDevice, s));
+    }
+    if (!T.h_short_blocks.empty()) {
+        MGPU_RET(pb.arena.alloc(&T.d_short_blocks, T.h_short_blocks.size()));
+        MGPU_HIP(hipMemcpyAsync(T.d_short_blocks, T.h_short_blocks.data(), sizeof(int) * T.h_short_blocks.size(),
+                                hipMemcpyHostToDevice, s));
+    }
+    MGPU_HIP(hipStreamSynchronize(s));
+    return MGPU_OK;
+}
+
+template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO>
+static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *coef, const void *tw) {
+    const ToeplitzPlan &T = pb.tp;
+    constexpr int NC = 1 << LOGNC, NT = NC / (1 << LOGV);
+    constexpr size_t lds = sizeof(R) * (size_t) lds_elems(NC) * (TWO ? 2 : 1);
+    static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
+    auto kern = k_tiles<R, LOGNC, LOGV, LOGR, TWO>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+        attr_set = true;
+    }
+    TileArgs a{d_in, d_out, T.d_tiles, coef, tw, T.d_mean, T.lambda, T.L};
+    hipLaunchKernelGGL(kern, dim3(T.ntiles), dim3(NT), lds, pb.stream, a);
+    MGPU_LAUNCHED();
+    return MGPU_OK;
+}
+
+template <typename R>
+static int launch_tiles(Problem &pb, const R *d_in, R *d_out) {
+    const ToeplitzPlan &T = pb.tp;
+    constexpr bool F32 = sizeof(R) == 4;
+    const void *coef = F32 ? (const void *) T.d_coef32 : (const void *) T.d_coef64;
+    const void *tw = F32 ? (const void *) T.d_tw32 : (const void *) T.d_tw64;
+    switch (T.log2n - 1) {  // LOGNC
+        case 9: return launch_tiles_cfg<R, 9, 3, 3, true>(pb, d_in, d_out, coef, tw);
+        case 10: return launch_tiles_cfg<R, 10, 4, 4, true>(pb, d_in, d_out, coef, tw);
+        case 11: return launch_tiles_cfg<R, 11, 4, 4, true>(pb, d_in, d_out, coef, tw);
+        case 12: return launch_tiles_cfg<R, 12, 4, 4, true>(pb, d_in, d_out, coef, tw);
+        case 13: return launch_tiles_cfg<R, 13, 4, 4, true>(pb, d_in, d_out, coef, tw);
+        case 14:
+            if constexpr (F32) return launch_tiles_cfg<R, 14, TILE14_F32_LOGV, TILE14_F32_LOGR, true>(pb, d_in, d_out, coef, tw);
+            else return launch_tiles_cfg<R, 14, TILE14_F64_LOGV, TILE14_F64_LOGR, false>(pb, d_in, d_out, coef, tw);
+        default: set_error("unsupported tile length 2^%d", T.log2n); return MGPU_EINVAL;
+    }
+}
+
+template <typename R>
+static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_partials) {
+    ToeplitzPlan &T = pb.tp;
+    hipStream_t s = pb.stream;
+    if (T.lambda == 1) {
+        k_diag<R><<<grid_for(T.m, 256), 256, 0, s>>>(d_in, d_out, T.m, T.d_start, T.nb, T.d_T0);
+        MGPU_LAUNCHED();
+        return MGPU_OK;
+    }
+    if (compute_partials) {
+        k_partials<R><<<(unsigned) T.nchunks, 256, 0, s>>>(d_in, T.m, T.d_start, T.nb, T.d_partials);
+        MGPU_LAUNCHED();
+    }
+    k_block_mean<<<T.nb, 256, 0, s>>>(T.d_partials, T.d_start, T.nb, T.d_mean);
+    MGPU_LAUNCHED();
+    if (T.ntiles) MGPU_RET(launch_tiles<R>(pb, d_in, d_out));
+    if (!T.h_direct_blocks.empty()) {
+        k_direct<R><<<dim3(64, (unsigned) T.h_direct_blocks.size()), 256, 0, s>>>(
+                d_in, d_out, T.d_start, T.d_direct_blocks, T.d_mean, T.d_direct_invtt, T.lambda);
+        MGPU_LAUNCHED();
+    }
+    if (!T.h_short_blocks.empty()) {
+        k_mean_only<R><<<dim3(16, (unsigned) T.h_short_blocks.size()), 256, 0, s>>>(d_in, d_out, T.d_start,
+                                                                                  T.d_short_blocks, T.d_mean);
+        MGPU_LAUNCHED();
+    }
+    if (T.covered < T.m) {
+        k_copy_tail<R><<<grid_for(T.m - T.covered, 256), 256, 0, s>>>(d_in, d_out, T.covered, T.m);
+        MGPU_LAUNCHED();
+    }
+    return MGPU_OK;
+}
+
+int toeplitz_apply(Problem &pb, const double *d_in, double *d_out, bool compute_partials) {
+    return toeplitz_apply_t<double>(pb, d_in, d_out, compute_partials);
+}
+
+int toeplitz_apply_f32(Problem &pb, const float *d_in, float *d_out, bool compute_partials) {
+    return toeplitz_apply_t<float>(pb, d_in, d_out, compute_partials);
+}
+
+}  // namespace mgpu

@@ -1,0 +1,267 @@
+"""Python host mirror of the libmappraiser_gpu.so C-ABI (include/mappraiser_gpu.h).
+
+The library is the product; this module only loads it and marshals numpy
+arrays with the dtype contract of the reference's ctypes wrapper
+(mappraiser_wrapper/__init__.py:23-28: int32 pixels, float64 everything else).
+There is no CPU fallback: if the shared library is missing or fails to load,
+every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmappraiser_gpu.so")
+
+MGPU_FP64 = 0
+MGPU_MIXED = 1
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = ct.c_void_p
+
+# every symbol declared in include/mappraiser_gpu.h
+EXPORTS = [
+    "mappraiser_gpu_init", "mappraiser_gpu_finalize", "mappraiser_gpu_last_error",
+    "mappraiser_gpu_options_default", "mappraiser_gpu_problem_create",
+    "mappraiser_gpu_problem_create_device", "mappraiser_gpu_problem_destroy",
+    "mappraiser_gpu_local_shape", "mappraiser_gpu_matvec", "mappraiser_gpu_trmatvec",
+    "mappraiser_gpu_apply_weights", "mappraiser_gpu_precond_build", "mappraiser_gpu_precond_apply",
+    "mappraiser_gpu_pcg", "mappraiser_gpu_solve_device", "mappraiser_gpu_pcg_gls",
+]
+
+
+class Options(ct.Structure):
+    _fields_ = [("precision", ct.c_int), ("refresh_every", ct.c_int), ("tile_log2", ct.c_int),
+                ("poll_every", ct.c_int), ("reserved", ct.c_int * 4)]
+
+
+class Report(ct.Structure):
+    _fields_ = [("iterations", ct.c_int), ("converged", ct.c_int), ("res0", ct.c_double),
+                ("res", ct.c_double), ("n_map", ct.c_int), ("lcount", ct.c_int), ("trash_pix", ct.c_int),
+                ("n_cut", ct.c_int), ("t_setup_ms", ct.c_double), ("t_solve_ms", ct.c_double),
+                ("true_res_rel", ct.c_double), ("refreshes", ct.c_int), ("pad", ct.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class GPUError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load():
+    """Load libmappraiser_gpu.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise GPUError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = ct.CDLL(LIB_PATH)
+    L.mappraiser_gpu_init.argtypes = [ct.c_int]
+    L.mappraiser_gpu_last_error.restype = ct.c_char_p
+    L.mappraiser_gpu_options_default.argtypes = [ct.POINTER(Options)]
+    L.mappraiser_gpu_options_default.restype = None
+    L.mappraiser_gpu_problem_create.argtypes = [ct.POINTER(_vp), ct.c_int, ct.c_int, _i32p, _f64p, ct.c_int, _i32p,
+                                                ct.c_int, _f64p, ct.POINTER(Options)]
+    L.mappraiser_gpu_problem_create_device.argtypes = [ct.POINTER(_vp), ct.c_int, ct.c_int, _vp, _vp, ct.c_int,
+                                                       _i32p, ct.c_int, _vp, ct.POINTER(Options)]
+    L.mappraiser_gpu_problem_destroy.argtypes = [_vp]
+    L.mappraiser_gpu_local_shape.argtypes = [_vp, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), _vp, _vp]
+    L.mappraiser_gpu_matvec.argtypes = [_vp, _f64p, _f64p]
+    L.mappraiser_gpu_trmatvec.argtypes = [_vp, _f64p, _f64p]
+    L.mappraiser_gpu_apply_weights.argtypes = [_vp, _f64p]
+    L.mappraiser_gpu_precond_build.argtypes = [_vp, _vp, _vp, _vp, ct.POINTER(ct.c_int)]
+    L.mappraiser_gpu_precond_apply.argtypes = [_vp, _f64p, _f64p]
+    L.mappraiser_gpu_pcg.argtypes = [_vp, _f64p, _f64p, ct.c_double, ct.c_int, ct.c_int, _f64p, _vp,
+                                     ct.POINTER(Report)]
+    L.mappraiser_gpu_solve_device.argtypes = [_vp, _vp, _vp, ct.c_double, ct.c_int, _f64p, _vp, _vp, _vp,
+                                              ct.POINTER(Report)]
+    L.mappraiser_gpu_pcg_gls.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p,
+                                         ct.c_int, _i32p, ct.c_int, _f64p, ct.c_double, ct.c_int, _f64p, _f64p,
+                                         _i32p, _i32p, ct.POINTER(Options), ct.POINTER(Report)]
+    _LIB = L
+    return L
+
+
+def last_error() -> str:
+    return load().mappraiser_gpu_last_error().decode(errors="replace")
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise GPUError(f"{what} failed (status {rc}): {last_error()}")
+
+
+def options(precision=MGPU_FP64, refresh_every=10, tile_log2=0, poll_every=8) -> Options:
+    o = Options()
+    load().mappraiser_gpu_options_default(ct.byref(o))
+    o.precision, o.refresh_every, o.tile_log2, o.poll_every = precision, refresh_every, tile_log2, poll_every
+    return o
+
+
+def init(device: int = 0):
+    _check(load().mappraiser_gpu_init(int(device)), "mappraiser_gpu_init")
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Problem:
+    """One rank's GLS problem resident on the GPU (MatInit + Toeplitz blocks)."""
+
+    def __init__(self, pix, weights, block_sizes, lam, invtt, nnz=3, opts: Options | None = None):
+        L = load()
+        self._h = _vp()
+        pix = _c(pix, np.int32)
+        w = _c(weights, np.float64)
+        bs = _c(block_sizes, np.int32)
+        self.m = pix.size // nnz
+        self.opts = opts or options()
+        _check(L.mappraiser_gpu_problem_create(ct.byref(self._h), self.m, nnz, pix, w, bs.size, bs, int(lam),
+                                               _c(invtt, np.float64), ct.byref(self.opts)),
+               "problem_create")
+        self.n_map = None
+
+    @classmethod
+    def from_device(cls, m, d_pix, d_w, block_sizes, lam, d_invtt, nnz=3, opts: Options | None = None):
+        self = cls.__new__(cls)
+        L = load()
+        self._h = _vp()
+        bs = _c(block_sizes, np.int32)
+        self.m = int(m)
+        self.opts = opts or options()
+        _check(L.mappraiser_gpu_problem_create_device(ct.byref(self._h), self.m, nnz, _vp(d_pix), _vp(d_w), bs.size,
+                                                      bs, int(lam), _vp(d_invtt), ct.byref(self.opts)),
+               "problem_create_device")
+        self.n_map = None
+        return self
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().mappraiser_gpu_problem_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- MatLocalShape ------------------------------------------------------
+    def local_shape(self, want_indices=True):
+        L = load()
+        lc, tr = ct.c_int(0), ct.c_int(0)
+        _check(L.mappraiser_gpu_local_shape(self._h, ct.byref(lc), ct.byref(tr), None, None), "local_shape")
+        lind = np.empty(max(lc.value, 1), np.int32)
+        idx = np.empty(self.m * 3, np.int32) if want_indices else None
+        _check(L.mappraiser_gpu_local_shape(self._h, ct.byref(lc), ct.byref(tr), lind.ctypes.data,
+                                            idx.ctypes.data if idx is not None else None), "local_shape")
+        return lind[: lc.value], idx, int(tr.value)
+
+    def map_size(self):
+        lind, _, tr = self.local_shape(want_indices=False)
+        return lind.size - 3 * tr
+
+    # -- operators ------------------------------------------------------------
+    def matvec(self, x):
+        y = np.empty(self.m, np.float64)
+        _check(load().mappraiser_gpu_matvec(self._h, _c(x, np.float64), y), "matvec")
+        return y
+
+    def trmatvec(self, y):
+        x = np.empty(self.map_size(), np.float64)
+        _check(load().mappraiser_gpu_trmatvec(self._h, _c(y, np.float64), x), "trmatvec")
+        return x
+
+    def apply_weights(self, tod):
+        v = _c(tod, np.float64).copy()
+        _check(load().mappraiser_gpu_apply_weights(self._h, v), "apply_weights")
+        return v
+
+    def precond_build(self, signal=None):
+        """Returns (signal after the cut, cond, lhits)."""
+        npix0 = self.map_size() // 3
+        cond = np.empty(max(npix0, 1), np.float64)
+        hits = np.empty(max(npix0, 1), np.int32)
+        n = ct.c_int(0)
+        sig = _c(signal, np.float64).copy() if signal is not None else None
+        _check(load().mappraiser_gpu_precond_build(self._h, sig.ctypes.data if sig is not None else None,
+                                                   cond.ctypes.data, hits.ctypes.data, ct.byref(n)),
+               "precond_build")
+        self.n_map = n.value
+        return sig, cond[: n.value // 3].copy(), hits[: n.value // 3].copy()
+
+    def precond_apply(self, g):
+        out = np.empty(self.map_size(), np.float64)
+        _check(load().mappraiser_gpu_precond_apply(self._h, _c(g, np.float64), out), "precond_apply")
+        return out
+
+    def pcg(self, signal, noise, tol=1e-8, maxiter=500, stop_after=0, history=True):
+        n = self.map_size()
+        x = np.empty(max(n, 1), np.float64)
+        hist = np.zeros(max(maxiter, 1), np.float64) if history else None
+        rep = Report()
+        _check(load().mappraiser_gpu_pcg(self._h, _c(signal, np.float64), _c(noise, np.float64), float(tol),
+                                         int(maxiter), int(stop_after), x,
+                                         hist.ctypes.data if hist is not None else None, ct.byref(rep)), "pcg")
+        nh = min(rep.iterations + 1, maxiter)
+        return x[:n], (hist[:nh] if hist is not None else None), rep
+
+
+@dataclass
+class GLSResult:
+    x: np.ndarray
+    lindices: np.ndarray
+    cond: np.ndarray
+    hits: np.ndarray
+    res_hist: np.ndarray
+    report: dict
+    pix_local: np.ndarray
+    weights: np.ndarray
+    signal: np.ndarray
+
+
+def pcg_gls(pix, weights, signal, noise, block_sizes, lam, invtt, tol=1e-8, maxiter=500, nnz=3,
+            opts: Options | None = None, outpath: str | None = None, ref: str | None = None) -> GLSResult:
+    """PCG_GLS_true drop-in on host arrays (copies; the reference's in-place side
+    effects are applied to the copies returned in the result)."""
+    L = load()
+    p = _c(pix, np.int32).copy()
+    w = _c(weights, np.float64).copy()
+    s = _c(signal, np.float64).copy()
+    nz = _c(noise, np.float64)
+    m = p.size // nnz
+    bs = _c(block_sizes, np.int32)
+    x = np.zeros(max(p.size, 1), np.float64)
+    cond = np.zeros(max(p.size // nnz, 1), np.float64)
+    hits = np.zeros(max(p.size // nnz, 1), np.int32)
+    lind = np.zeros(max(p.size, 1), np.int32)
+    rep = Report()
+    o = opts or options()
+    _check(L.mappraiser_gpu_pcg_gls(outpath.encode() if outpath else None, ref.encode() if ref else None, m, nnz, p,
+                                    w, s, nz, bs.size, bs, int(lam), _c(invtt, np.float64), float(tol), int(maxiter),
+                                    x, cond, hits, lind, ct.byref(o), ct.byref(rep)), "pcg_gls")
+    n = rep.n_map
+    hist = None
+    if outpath and ref:
+        fn = os.path.join(outpath, f"pcg_residuals_{ref}.dat")
+        if os.path.exists(fn):
+            hist = np.fromfile(fn, dtype=np.float64)
+    return GLSResult(x=x[:n].copy(), lindices=lind[: rep.lcount].copy(), cond=cond[: n // 3].copy(),
+                     hits=hits[: n // 3].copy(), res_hist=hist, report=rep.as_dict(), pix_local=p, weights=w,
+                     signal=s)

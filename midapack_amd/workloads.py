@@ -1,0 +1,165 @@
+"""Synthetic GLS map-making workloads (seeded, reproducible on any host).
+
+Each workload is one rank's share of time-ordered data (TOD) in the exact
+layout MAPPRAISER's ``MLmap`` receives from its TOAST operator:
+
+* ``pix``      int32[m*3]   pixel index encoded ``3p+k`` (mappraiser.py:1214-1216),
+                            flagged samples ``-3,-2,-1`` (utils.py:148-150)
+* ``weights``  float64[m*3] (1, cos 2psi, sin 2psi)
+* ``signal``, ``noise``  float64[m]
+* ``block_sizes`` int32[nb], ``invtt`` float64[nb*lambda] (utils.py:416-481)
+
+Scan: boustrophedon raster over an Nx x Ny flat grid with per-detector seeded
+offsets; psi ~ U[0, pi).  Noise: seeded 1/f timestreams with PSD
+sigma^2 (1 + (fknee/f)^alpha).  invtt: the analytic form of ``noise2invtt``
+(utils.py:500-601): irfft of the inverse PSD over the block length, apodised
+by ifftshift(chebwin(2 lambda, 150 dB))[:lambda].
+
+numpy's PCG64 generator is seeded per (workload, detector, stream), so the
+same inputs are regenerated bit-for-bit on every host with this image.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+FSAMP = 10.0  # Hz
+
+
+@dataclass
+class Workload:
+    name: str
+    pix: np.ndarray
+    weights: np.ndarray
+    signal: np.ndarray
+    noise: np.ndarray
+    block_sizes: np.ndarray
+    lam: int
+    invtt: np.ndarray
+    nx: int
+    ny: int
+    sky: np.ndarray = field(repr=False)   # true I,Q,U map, float64[npix_grid*3]
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def m(self) -> int:
+        return int(self.signal.size)
+
+
+def _rng(seed, *keys):
+    return np.random.default_rng([int(seed)] + [int(k) for k in keys])
+
+
+def raster_pixels(nsamp, nx, ny, ox, oy):
+    """Boustrophedon raster: row advances every nx samples, odd passes reversed."""
+    t = np.arange(nsamp, dtype=np.int64)
+    rpass = t // nx
+    c0 = t % nx
+    c = np.where(rpass % 2 == 0, c0, nx - 1 - c0)
+    row = (rpass + oy) % ny
+    col = (c + ox) % nx
+    return (row * nx + col).astype(np.int64)
+
+
+def one_over_f_noise(rng, nsamp, sigma, fknee, alpha, fsamp=FSAMP):
+    w = rng.standard_normal(nsamp)
+    f = np.fft.rfftfreq(nsamp, 1.0 / fsamp)
+    amp = np.zeros_like(f)
+    amp[1:] = np.sqrt(1.0 + (fknee / f[1:]) ** alpha)
+    return sigma * np.fft.irfft(np.fft.rfft(w) * amp, n=nsamp)
+
+
+def invtt_block(nb_samp, lam, sigma, fknee, alpha, fsamp=FSAMP):
+    """Analytic noise2invtt (utils.py:550-580): first row of the inverse-noise band."""
+    from scipy.signal import get_window
+
+    if lam > nb_samp // 2:
+        raise ValueError("Bandwidth cannot be larger than timestream (utils.py:561).")
+    f = np.fft.rfftfreq(nb_samp, 1.0 / fsamp)
+    psdm1 = np.zeros_like(f)
+    psdm1[1:] = 1.0 / (sigma**2 * (1.0 + (fknee / f[1:]) ** alpha))
+    inv_tt = np.fft.irfft(psdm1, n=nb_samp)
+    if lam == 1:
+        return inv_tt[:1].copy()
+    window = np.fft.ifftshift(get_window(("chebwin", 150), 2 * lam))[:lam]
+    return np.multiply(window, inv_tt[:lam], dtype=np.float64)
+
+
+def make_workload(name, ndet, nsamp, nx, ny, lam, seed, block_len=None, fknee=0.1, alpha=1.0,
+                  sigma=1.0, fknee_range=None, alpha_range=None, det_lengths=None, flag_frac=0.0):
+    """Build one rank's data.  ``det_lengths`` overrides ``nsamp`` per detector;
+    ``block_len`` splits every detector into Toeplitz blocks of that length
+    (the last block takes the remainder); all blocks of a detector share its T."""
+    lens = [int(nsamp)] * ndet if det_lengths is None else [int(v) for v in det_lengths]
+    m = int(sum(lens))
+    npix_grid = nx * ny
+    rs = _rng(seed, 0)
+    sky = np.empty(npix_grid * 3)
+    sky[0::3] = rs.standard_normal(npix_grid)
+    sky[1::3] = 0.1 * rs.standard_normal(npix_grid)
+    sky[2::3] = 0.1 * rs.standard_normal(npix_grid)
+
+    pix = np.empty(m * 3, np.int32)
+    weights = np.empty(m * 3, np.float64)
+    signal = np.empty(m, np.float64)
+    noise = np.empty(m, np.float64)
+    bsizes, invtt = [], []
+    t0 = 0
+    for d, n_d in enumerate(lens):
+        rd = _rng(seed, 1, d)
+        ox, oy = int(rd.integers(0, nx)), int(rd.integers(0, ny))
+        fk = fknee if fknee_range is None else float(rd.uniform(*fknee_range))
+        al = alpha if alpha_range is None else float(rd.uniform(*alpha_range))
+        p = raster_pixels(n_d, nx, ny, ox, oy)
+        psi = _rng(seed, 2, d).random(n_d) * np.pi
+        c2, s2 = np.cos(2 * psi), np.sin(2 * psi)
+        sl = slice(t0, t0 + n_d)
+        base = 3 * p
+        pix[3 * t0: 3 * (t0 + n_d): 3] = base
+        pix[3 * t0 + 1: 3 * (t0 + n_d): 3] = base + 1
+        pix[3 * t0 + 2: 3 * (t0 + n_d): 3] = base + 2
+        weights[3 * t0: 3 * (t0 + n_d): 3] = 1.0
+        weights[3 * t0 + 1: 3 * (t0 + n_d): 3] = c2
+        weights[3 * t0 + 2: 3 * (t0 + n_d): 3] = s2
+        signal[sl] = sky[base] + c2 * sky[base + 1] + s2 * sky[base + 2]
+        noise[sl] = one_over_f_noise(_rng(seed, 3, d), n_d, sigma, fk, al)
+        if flag_frac > 0:
+            fl = _rng(seed, 4, d).random(n_d) < flag_frac
+            idx = np.nonzero(fl)[0] + t0
+            for k in range(3):
+                pix[3 * idx + k] = k - 3
+        L = n_d if block_len is None else int(block_len)
+        nblk = max(1, (n_d + L - 1) // L)
+        sizes = [L] * (nblk - 1) + [n_d - L * (nblk - 1)]
+        T = invtt_block(L if nblk > 1 else n_d, lam, sigma, fk, al)
+        for s in sizes:
+            bsizes.append(s)
+            invtt.append(T)
+        t0 += n_d
+    return Workload(name=name, pix=pix, weights=weights, signal=signal, noise=noise,
+                    block_sizes=np.asarray(bsizes, np.int32), lam=int(lam),
+                    invtt=np.concatenate(invtt).astype(np.float64), nx=nx, ny=ny, sky=sky,
+                    meta=dict(ndet=ndet, m=m, seed=seed))
+
+
+# ----------------------------------------------------------------------------
+# Named configurations (one rank's share each)
+# ----------------------------------------------------------------------------
+def config_c1(seed=1):
+    """4 det x 65,536 samples, 128x128, lambda=1024, one block per detector."""
+    return make_workload("C1: 4x65536 on 128^2, lambda=1024, FP64", 4, 65536, 128, 128, 1024, seed)
+
+
+def config_c2(seed=2, ndet=64, nsamp=1 << 20):
+    """64 det x 2^20 samples, 1024x1024, lambda=8192, 2^17-sample blocks,
+    fknee ~ U[0.05,0.5] Hz, alpha ~ U[1,2] per detector."""
+    return make_workload(f"C2: {ndet}x{nsamp} on 1024^2, lambda=8192, 2^17 blocks", ndet, nsamp, 1024, 1024,
+                         8192, seed, block_len=1 << 17, fknee_range=(0.05, 0.5), alpha_range=(1.0, 2.0))
+
+
+def config_small(seed=7, ndet=2, nsamp=4096, nx=16, ny=16, lam=64, block_len=None, flag_frac=0.0,
+                 det_lengths=None):
+    """Small case for parity tests (oracle runs in well under a second)."""
+    return make_workload(f"small {ndet}x{nsamp} {nx}x{ny} lambda={lam}", ndet, nsamp, nx, ny, lam, seed,
+                         block_len=block_len, flag_frac=flag_frac, det_lengths=det_lengths)

@@ -1,0 +1,115 @@
+/*
+ * oracle.h -- CPU restatement of the MAPPRAISER/MIDAPACK GLS map-making path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product path (libmappraiser_gpu.so) never
+ * links, loads or calls it.
+ *
+ * Each function restates one reference routine on ONE rank (every MPI
+ * reduction on the path is an identity there).  Citations are to files of the
+ * reference tree (B3Dcmb/midapack).  Arithmetic is plain IEEE double with
+ * floating-point contraction disabled (-ffp-contract=off), matching a build of
+ * the reference without FMA.
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   orc_local_shape, orc_matvec, orc_trmatvec  -- bit-exact against the reference
+ *       mapmat.c/csort.c/cindex.c compiled from source (oracle/_ref) and against
+ *       numpy unique/searchsorted brute force.
+ *   orc_stbmm / orc_apply_weights -- FFT shift-and-overlap path equals the direct
+ *       banded product (closed form, mean removal + zero-boundary band) to 1e-13;
+ *       special cases lambda=1, T=delta, constant input.
+ *   orc_fft -- against numpy.fft.
+ *   orc_dgetrf / orc_dgecon1 / orc_dgetri / orc_dlange1 -- against scipy's LAPACK
+ *       (dgetrf/dgecon/dgetri/dlange) on random and degenerate 3x3 blocks, and the
+ *       inverse against the adjugate closed form.
+ *   orc_getlocalW -- closed form sum_t w w^T T0.
+ *   orc_gls_solve -- tiny systems against a dense LU solve of the GLS normal
+ *       equations (with mean-removal N^-1), and x_k against a dense-operator PCG.
+ */
+#ifndef MAPPRAISER_ORACLE_H
+#define MAPPRAISER_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_OK 0
+#define ORC_EINVAL (-1)
+#define ORC_ENOMEM (-2)
+
+/* ---- index tables (mapmat.c:496-521, csort.c:111-135, cindex.c:34-41,102-115) ---- */
+int orc_counting_sort(int *indices, int64_t count);
+int orc_dichotomy(int nT, const int *T, int e);
+int orc_local_shape(int64_t count, int *indices, int *lindices, int *lcount, int *trash_pix);
+
+/* ---- pointing products (mapmat.c:704-729, 819-851) ---- */
+void orc_matvec(int m, int nnz, const int *indices, const double *values, int trash_pix,
+                const double *x, double *y);
+void orc_trmatvec(int m, int nnz, const int *indices, const double *values, int trash_pix,
+                  int lcount, const double *y, double *x);
+
+/* ---- pixel -> samples linked list (mappraiser.c:110-145) ---- */
+void orc_pixel_lists(int m, int nnz, const int *indices, int trash_pix, int npix_valid,
+                     int *id_last_pix, int *ll);
+
+/* ---- FFT used by the Toeplitz restatement (stands for FFTW's r2c/c2r) ---- */
+/* in/out interleaved complex (re,im), length n, sign -1 forward / +1 backward,
+ * unnormalised. Any n >= 1. */
+int orc_fft(int n, const double *in, double *out, int sign);
+
+/* ---- Toeplitz (toeplitz.c:168-253, 441-470, 569-700, 731-865; toeplitz_block.c:115-636;
+ *      toeplitz_wizard.c:80-111; pcg_true.c:578-600) ---- */
+int orc_define_blocksize(int n, int lambda);
+/* one block, in place: banded symmetric Toeplitz product with zero boundaries.
+ * use_fft=1: shift-and-overlap with the reference's tile rule; 0: direct O(n lambda). */
+int orc_stmm_block(double *V, int n, const double *T, int lambda, int use_fft);
+/* N^-1 on a local TOD vector: per-block mean removal then the Toeplitz product. */
+int orc_stbmm(int nb, const int *block_sizes, int lambda, const double *invtt, int64_t m,
+              double *V, int use_fft);
+/* apply_weights: lambda==1 -> diagonal scale by T0 (no mean removal), else orc_stbmm. */
+int orc_apply_weights(int nb, const int *block_sizes, int lambda, const double *invtt,
+                      int64_t m, double *V, int use_fft);
+
+/* ---- LAPACK restatement (reference LAPACK algorithms, row-major n x n, small n) ---- */
+double orc_dlange1(int n, const double *a);
+int orc_dgetrf(int n, double *a, int *ipiv);          /* returns LAPACK info */
+double orc_dgecon1(int n, const double *lu, double anorm);
+int orc_dgetri(int n, double *a, const int *ipiv);    /* returns LAPACK info */
+
+/* ---- preconditioner (precond.c:42-161, 456-702) ---- */
+void orc_getlocalW(int m, int nnz, const int *indices, const double *values, int trash_pix,
+                   int lcount, int nb, const int *block_sizes, int lambda, const double *invtt,
+                   double *vpixBlock, int *lhits);
+
+typedef struct {
+    int    iterations;   /* value of k when the loop ended (K if unconverged) */
+    int    converged;
+    double res0;         /* ||g0||^2 */
+    double res;          /* final ||g||^2 */
+    int    n_map;        /* map size after the pixel cut */
+    int    lcount;       /* final local index count (including trash entries) */
+    int    trash_pix;    /* final trash flag */
+    int    n_cut;        /* number of pixels removed by the rcond cut */
+} orc_report;
+
+/* Whole one-rank drop-in: MLmap setup (mappraiser.c:64-195) + PCG_GLS_true
+ * (pcg_true.c:20-241) with the block-Jacobi preconditioner (precond=0).
+ * pix (global on input, local on output) and pixweights/signal are modified in
+ * place exactly as the reference modifies them.  Output buffers:
+ *   x_out[m*nnz cap], lindices_out[m*nnz cap], cond_out/lhits_out[m cap],
+ *   res_hist[maxiter cap] (rel. residual sqrt(res)/sqrt(res0) per k, k=0..).
+ * stop_after>0 stops after that many loop iterations regardless of tol (for
+ * iterate-by-iterate parity tests). */
+int orc_gls_solve(int m, int nnz, int *pix, double *pixweights, double *signal,
+                  const double *noise, int nb, const int *block_sizes, int lambda,
+                  const double *invtt, double tol, int maxiter, int stop_after,
+                  double *x_out, int *lindices_out, double *cond_out, int *lhits_out,
+                  double *res_hist, orc_report *rep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

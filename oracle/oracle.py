@@ -1,0 +1,279 @@
+"""ctypes bindings for the CPU oracle (liboracle.so) and the reference mapmat
+build (oracle/_ref/libref_mapmat.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` -- as the checker or the timed CPU
+baseline, never by the product path.  Every function restates one reference
+routine on one rank; see oracle.h for the citations and what pins each one.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class OracleReport(ct.Structure):
+    _fields_ = [
+        ("iterations", ct.c_int),
+        ("converged", ct.c_int),
+        ("res0", ct.c_double),
+        ("res", ct.c_double),
+        ("n_map", ct.c_int),
+        ("lcount", ct.c_int),
+        ("trash_pix", ct.c_int),
+        ("n_cut", ct.c_int),
+    ]
+
+
+def lib():
+    """Load liboracle.so (built by ``make -C oracle``)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run __graft_entry__.build())")
+        L = ct.CDLL(path)
+        L.orc_local_shape.argtypes = [ct.c_int64, _i32p, _i32p, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]
+        L.orc_matvec.argtypes = [ct.c_int, ct.c_int, _i32p, _f64p, ct.c_int, _f64p, _f64p]
+        L.orc_matvec.restype = None
+        L.orc_trmatvec.argtypes = [ct.c_int, ct.c_int, _i32p, _f64p, ct.c_int, ct.c_int, _f64p, _f64p]
+        L.orc_trmatvec.restype = None
+        L.orc_pixel_lists.argtypes = [ct.c_int, ct.c_int, _i32p, ct.c_int, ct.c_int, _i32p, _i32p]
+        L.orc_pixel_lists.restype = None
+        L.orc_fft.argtypes = [ct.c_int, _f64p, _f64p, ct.c_int]
+        L.orc_define_blocksize.argtypes = [ct.c_int, ct.c_int]
+        L.orc_stmm_block.argtypes = [_f64p, ct.c_int, _f64p, ct.c_int, ct.c_int]
+        L.orc_stbmm.argtypes = [ct.c_int, _i32p, ct.c_int, _f64p, ct.c_int64, _f64p, ct.c_int]
+        L.orc_apply_weights.argtypes = [ct.c_int, _i32p, ct.c_int, _f64p, ct.c_int64, _f64p, ct.c_int]
+        L.orc_dlange1.argtypes = [ct.c_int, _f64p]
+        L.orc_dlange1.restype = ct.c_double
+        L.orc_dgetrf.argtypes = [ct.c_int, _f64p, _i32p]
+        L.orc_dgecon1.argtypes = [ct.c_int, _f64p, ct.c_double]
+        L.orc_dgecon1.restype = ct.c_double
+        L.orc_dgetri.argtypes = [ct.c_int, _f64p, _i32p]
+        L.orc_getlocalW.argtypes = [ct.c_int, ct.c_int, _i32p, _f64p, ct.c_int, ct.c_int, ct.c_int, _i32p,
+                                    ct.c_int, _f64p, _f64p, _i32p]
+        L.orc_getlocalW.restype = None
+        L.orc_gls_solve.argtypes = [ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p, ct.c_int, _i32p, ct.c_int,
+                                    _f64p, ct.c_double, ct.c_int, ct.c_int, _f64p, _i32p, _f64p, _i32p, _f64p,
+                                    ct.POINTER(OracleReport)]
+        _LIB = L
+    return _LIB
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libref_mapmat.so"))
+
+
+def ref():
+    """Load the reference mapmat build (only where /root/reference was present at build time)."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libref_mapmat.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference mapmat build missing: {path}")
+        R = ct.CDLL(path)
+        R.ref_local_shape.argtypes = [ct.c_int, ct.c_int, _i32p, _f64p, _i32p, ct.POINTER(ct.c_int),
+                                      ct.POINTER(ct.c_int)]
+        _vp = ct.c_void_p
+        R.ref_products.argtypes = [ct.c_int, ct.c_int, _i32p, _f64p, _vp, _vp, _vp, _vp]
+        _REF = R
+    return _REF
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ----------------------------------------------------------------------------
+# index tables and pointing
+# ----------------------------------------------------------------------------
+def local_shape(pix):
+    """MatLocalShape: returns (lindices, local indices, trash_pix)."""
+    idx = _c(pix, np.int32).copy()
+    lind = np.empty(max(idx.size, 1), np.int32)
+    lc, tr = ct.c_int(0), ct.c_int(0)
+    rc = lib().orc_local_shape(idx.size, idx, lind, ct.byref(lc), ct.byref(tr))
+    if rc:
+        raise RuntimeError(f"orc_local_shape rc={rc}")
+    return lind[: lc.value].copy(), idx, int(tr.value)
+
+
+def matvec(indices, values, trash, nnz, x):
+    m = indices.size // nnz
+    y = np.empty(m, np.float64)
+    lib().orc_matvec(m, nnz, _c(indices, np.int32), _c(values, np.float64), int(trash), _c(x, np.float64), y)
+    return y
+
+
+def trmatvec(indices, values, trash, nnz, lcount, y):
+    m = indices.size // nnz
+    x = np.empty(max(lcount, 1), np.float64)
+    lib().orc_trmatvec(m, nnz, _c(indices, np.int32), _c(values, np.float64), int(trash), int(lcount),
+                       _c(y, np.float64), x)
+    n = lcount - (nnz if trash else 0)
+    return x[:n].copy()
+
+
+def ref_local_shape(pix, weights, nnz):
+    idx = _c(pix, np.int32).copy()
+    w = _c(weights, np.float64).copy()
+    lind = np.empty(max(idx.size, 1), np.int32)
+    lc, tr = ct.c_int(0), ct.c_int(0)
+    ref().ref_local_shape(idx.size // nnz, nnz, idx, w, lind, ct.byref(lc), ct.byref(tr))
+    return lind[: lc.value].copy(), idx, int(tr.value)
+
+
+def ref_products(pix, weights, nnz, x=None, y=None):
+    """Reference MatVecProd(x) and/or TrMatVecProd(y) on global indices."""
+    idx = _c(pix, np.int32)
+    w = _c(weights, np.float64).copy()
+    m = idx.size // nnz
+    lind, _, trash = ref_local_shape(pix, weights, nnz)
+    yo = np.empty(m, np.float64) if x is not None else None
+    xo = np.empty(max(lind.size, 1), np.float64) if y is not None else None
+    xin = _c(x, np.float64) if x is not None else None
+    yin = _c(y, np.float64) if y is not None else None
+    ref().ref_products(m, nnz, idx, w,
+                       xin.ctypes.data if xin is not None else None,
+                       yo.ctypes.data if yo is not None else None,
+                       yin.ctypes.data if yin is not None else None,
+                       xo.ctypes.data if xo is not None else None)
+    if xo is not None:
+        xo = xo[: lind.size - (nnz if trash else 0)]
+    return yo, xo
+
+
+def pixel_lists(indices, trash, nnz, npix):
+    m = indices.size // nnz
+    idl = np.empty(max(npix, 1), np.int32)
+    ll = np.empty(max(m, 1), np.int32)
+    lib().orc_pixel_lists(m, nnz, _c(indices, np.int32), int(trash), int(npix), idl, ll)
+    return idl[:npix], ll[:m]
+
+
+# ----------------------------------------------------------------------------
+# FFT / Toeplitz
+# ----------------------------------------------------------------------------
+def fft(z, sign=-1):
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    a = z.view(np.float64).copy()
+    out = np.empty_like(a)
+    rc = lib().orc_fft(z.size, a, out, int(sign))
+    if rc:
+        raise RuntimeError(rc)
+    return out.view(np.complex128)
+
+
+def define_blocksize(n, lam):
+    return int(lib().orc_define_blocksize(int(n), int(lam)))
+
+
+def stmm_block(v, T, lam, use_fft=True):
+    V = _c(v, np.float64).copy()
+    rc = lib().orc_stmm_block(V, V.size, _c(T, np.float64), int(lam), int(bool(use_fft)))
+    if rc < 0:
+        raise RuntimeError(rc)
+    return V
+
+
+def apply_weights(tod, block_sizes, lam, invtt, use_fft=True):
+    """apply_weights (pcg_true.c:578-600): N^-1 applied to a TOD vector (copy)."""
+    V = _c(tod, np.float64).copy()
+    bs = _c(block_sizes, np.int32)
+    rc = lib().orc_apply_weights(bs.size, bs, int(lam), _c(invtt, np.float64), V.size, V, int(bool(use_fft)))
+    if rc < 0:
+        raise RuntimeError(f"orc_apply_weights rc={rc}")
+    return V
+
+
+# ----------------------------------------------------------------------------
+# LAPACK restatement / preconditioner
+# ----------------------------------------------------------------------------
+def dlange1(a):
+    a = _c(a, np.float64)
+    return lib().orc_dlange1(a.shape[0], a)
+
+
+def dgetrf(a):
+    lu = _c(a, np.float64).copy()
+    ipiv = np.zeros(lu.shape[0], np.int32)
+    info = lib().orc_dgetrf(lu.shape[0], lu, ipiv)
+    return lu, ipiv, info
+
+
+def dgecon1(lu, anorm):
+    return lib().orc_dgecon1(lu.shape[0], _c(lu, np.float64), float(anorm))
+
+
+def dgetri(lu, ipiv):
+    a = _c(lu, np.float64).copy()
+    info = lib().orc_dgetri(a.shape[0], a, _c(ipiv, np.int32))
+    return a, info
+
+
+def getlocalW(indices, values, trash, nnz, lcount, block_sizes, lam, invtt):
+    m = indices.size // nnz
+    n_valid = lcount - (nnz if trash else 0)
+    W = np.empty(max(n_valid * nnz, 1), np.float64)
+    hits = np.zeros(max(n_valid // nnz, 1), np.int32)
+    bs = _c(block_sizes, np.int32)
+    lib().orc_getlocalW(m, nnz, _c(indices, np.int32), _c(values, np.float64), int(trash), int(lcount), bs.size,
+                        bs, int(lam), _c(invtt, np.float64), W, hits)
+    return W[: n_valid * nnz].reshape(-1, nnz, nnz), hits[: n_valid // nnz]
+
+
+@dataclass
+class SolveResult:
+    x: np.ndarray
+    lindices: np.ndarray
+    cond: np.ndarray
+    hits: np.ndarray
+    res_hist: np.ndarray
+    iterations: int
+    converged: bool
+    res0: float
+    res: float
+    n_cut: int
+    trash_pix: int
+    pix_local: np.ndarray
+    weights: np.ndarray
+    signal: np.ndarray
+
+
+def gls_solve(pix, weights, signal, noise, block_sizes, lam, invtt, nnz=3, tol=1e-8, maxiter=500,
+              stop_after=0):
+    """MLmap setup + PCG_GLS_true on one rank (copies; inputs are not modified)."""
+    idx = _c(pix, np.int32).copy()
+    w = _c(weights, np.float64).copy()
+    s = _c(signal, np.float64).copy()
+    nz = _c(noise, np.float64)
+    m = idx.size // nnz
+    bs = _c(block_sizes, np.int32)
+    cap = max(idx.size, 1)
+    x = np.zeros(cap, np.float64)
+    lind = np.zeros(cap, np.int32)
+    cond = np.zeros(max(m, 1) + cap // nnz, np.float64)
+    hits = np.zeros(max(m, 1) + cap // nnz, np.int32)
+    hist = np.zeros(max(maxiter, 1), np.float64)
+    rep = OracleReport()
+    rc = lib().orc_gls_solve(m, nnz, idx, w, s, nz, bs.size, bs, int(lam), _c(invtt, np.float64), float(tol),
+                             int(maxiter), int(stop_after), x, lind, cond, hits, hist, ct.byref(rep))
+    if rc:
+        raise RuntimeError(f"orc_gls_solve rc={rc}")
+    n = rep.n_map
+    nh = min(rep.iterations + 1, maxiter) if maxiter > 0 else 0
+    return SolveResult(x=x[:n].copy(), lindices=lind[: rep.lcount].copy(), cond=cond[: n // nnz].copy(),
+                       hits=hits[: n // nnz].copy(), res_hist=hist[:nh].copy(), iterations=rep.iterations,
+                       converged=bool(rep.converged), res0=rep.res0, res=rep.res, n_cut=rep.n_cut,
+                       trash_pix=rep.trash_pix, pix_local=idx, weights=w, signal=s)
