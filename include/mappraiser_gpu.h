@@ -45,8 +45,31 @@ typedef struct mgpu_options {
     int refresh_every;  /* MGPU_MIXED: recompute the FP64 residual every k iterations (default 10) */
     int tile_log2;      /* real FFT tile length 2^k of the Toeplitz product; 0 = autotuned choice */
     int poll_every;     /* PCG: iterations launched between host convergence polls (default 8) */
-    int reserved[4];
+    int profile;        /* 1: time every kernel class with HIP events on the handle's stream */
+    int offset_mode;    /* per-block offset handling of N^-1 (MGPU_OFFSET_*), default REFERENCE */
+    int reserved[2];
 } mgpu_options;
+
+/* Per-block offset handling in N^-1 (block b: C = I - 1 1^T / n_b, T = Toeplitz band):
+ *   MGPU_OFFSET_REFERENCE  N^-1 = T C, the reference's stbmmProd (toeplitz_wizard.c:86-95):
+ *                          mean removal before the product.  Not symmetric: on workloads
+ *                          with steep 1/f noise PCG can stagnate or break down.
+ *   MGPU_OFFSET_SYMMETRIC  N^-1 = C T C: the same offset marginalisation as a symmetric
+ *                          projection; equals T C except within lambda of block edges.
+ *   MGPU_OFFSET_NONE       N^-1 = T (no offset removal).
+ * lambda == 1 (white noise) never removes offsets, as in apply_weights (pcg_true.c:583-593). */
+#define MGPU_OFFSET_REFERENCE 0
+#define MGPU_OFFSET_SYMMETRIC 1
+#define MGPU_OFFSET_NONE 2
+
+/* kernel classes reported by mappraiser_gpu_kernel_times */
+#define MGPU_KT_P 0        /* P (MatVecProd) with the fused per-block sums */
+#define MGPU_KT_MEAN 1     /* per-block mean of N^-1 */
+#define MGPU_KT_TILES 2    /* Toeplitz FFT tiles of N^-1 */
+#define MGPU_KT_PT 3       /* P^T (TrMatVecProd) with the fused dot */
+#define MGPU_KT_MAP 4      /* PCG map-domain updates, dots and preconditioner */
+#define MGPU_KT_REFRESH 5  /* FP64 residual recomputation (MGPU_MIXED) */
+#define MGPU_NKT 6
 
 typedef struct mgpu_report {
     int    iterations;  /* value of the loop counter k when PCG stopped (maxiter if not converged) */
@@ -129,6 +152,13 @@ int mappraiser_gpu_pcg(mgpu_problem *pb, const double *signal, const double *noi
 int mappraiser_gpu_solve_device(mgpu_problem *pb, double *d_signal, const double *d_noise,
                                 double tol, int maxiter, double *x, double *res_hist,
                                 double *cond, int *lhits, mgpu_report *rep);
+
+/* Kernel timing of a handle created with opt.profile = 1: accumulated device
+ * milliseconds and launch counts per kernel class (MGPU_KT_*), measured with
+ * HIP events on the handle's stream; reset != 0 clears them afterwards.  No
+ * reference counterpart (the reference prints MPI_Wtime splits,
+ * pcg_true.c:27,98-99,178-179). */
+int mappraiser_gpu_kernel_times(mgpu_problem *pb, double *ms, long long *counts, int reset);
 
 /* Drop-in for PCG_GLS_true(outpath, ref, &A, &Nm1, x, signal, noise, cond,
  * lhits, tol, maxiter, precond=0, Z_2lvl) at mappraiser.c:211 for one rank,

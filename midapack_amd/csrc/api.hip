@@ -47,6 +47,7 @@ static int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt) {
     MGPU_HIP(hipStreamCreateWithFlags(&pb->stream, hipStreamNonBlocking));
     mappraiser_gpu_options_default(&pb->opt);
     if (opt) pb->opt = *opt;
+    pb->prof.on = pb->opt.profile != 0;
     MGPU_ARG(pb->opt.precision == MGPU_FP64 || pb->opt.precision == MGPU_MIXED, "unknown precision %d",
              pb->opt.precision);
     return MGPU_OK;
@@ -266,6 +267,23 @@ int mappraiser_gpu_solve_device(mgpu_problem *h, double *d_signal, const double 
     if (lhits) MGPU_HIP(hipMemcpyAsync(lhits, pb.pc.d_hits, sizeof(int) * np, hipMemcpyDeviceToHost, pb.stream));
     MGPU_HIP(hipStreamSynchronize(pb.stream));
     if (rep) *rep = r;
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_kernel_times(mgpu_problem *h, double *ms, long long *counts, int reset) {
+    MGPU_ARG(h, "NULL handle");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    MGPU_HIP(hipStreamSynchronize(pb.stream));
+    pb.prof.collect();
+    for (int i = 0; i < MGPU_NKT; i++) {
+        if (ms) ms[i] = pb.prof.ms[i];
+        if (counts) counts[i] = pb.prof.cnt[i];
+        if (reset) {
+            pb.prof.ms[i] = 0.0;
+            pb.prof.cnt[i] = 0;
+        }
+    }
     return MGPU_OK;
 }
 

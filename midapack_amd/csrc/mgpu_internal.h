@@ -151,7 +151,13 @@ struct ToeplitzPlan {
     // partial sums for the per-block mean (toeplitz_wizard.c:86-95)
     int64_t nchunks = 0;
     double *d_partials = nullptr;    // nchunks + nb
-    double *d_mean = nullptr;        // nb
+    double *d_mean = nullptr;        // nb (per block: [0] input mean, see d_mean_out)
+    // MGPU_OFFSET_SYMMETRIC: mean of T (v - mean) per block, from the block-edge terms
+    int offset_mode = MGPU_OFFSET_REFERENCE;
+    double *d_mean_out = nullptr;    // nb
+    double *d_tail = nullptr;        // nfilters * lambda: tail_f[j] = sum_{k>j} T_f[k]
+    int *d_block_filter = nullptr;   // nb
+    int *d_block_kind = nullptr;     // nb: 0 Toeplitz product, 1 short block (identity)
 };
 
 // ---------------------------------------------------------------------------
@@ -191,9 +197,74 @@ struct Precond {
 };
 
 // ---------------------------------------------------------------------------
+// Kernel timing with HIP events on the problem's stream (opt.profile)
+// ---------------------------------------------------------------------------
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        int cat;
+        int tag;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> free_ev;
+    double ms[MGPU_NKT] = {0};
+    long long cnt[MGPU_NKT] = {0};
+    int open_cat = -1;
+    int tag = 0;  // PCG iteration the next launches belong to (0 = outside the loop)
+    hipEvent_t open_a = nullptr;
+
+    hipEvent_t get() {
+        if (!free_ev.empty()) {
+            hipEvent_t e = free_ev.back();
+            free_ev.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void) hipEventCreate(&e);
+        return e;
+    }
+    void begin(int cat, hipStream_t s) {
+        if (!on) return;
+        open_cat = cat;
+        open_a = get();
+        (void) hipEventRecord(open_a, s);
+    }
+    void end(hipStream_t s) {
+        if (!on || open_cat < 0) return;
+        hipEvent_t b = get();
+        (void) hipEventRecord(b, s);
+        pending.push_back(Rec{open_cat, tag, open_a, b});
+        open_cat = -1;
+    }
+    // call after the stream has been synchronised; launches of iterations past
+    // `last_iter` were queued after convergence and returned at once: not counted
+    void collect(int last_iter = 0x7fffffff) {
+        for (auto &r : pending) {
+            float t = 0.f;
+            if (r.tag <= last_iter && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+                ms[r.cat] += t;
+                cnt[r.cat] += 1;
+            }
+            free_ev.push_back(r.a);
+            free_ev.push_back(r.b);
+        }
+        pending.clear();
+    }
+    ~Profiler() {
+        for (auto &r : pending) {
+            (void) hipEventDestroy(r.a);
+            (void) hipEventDestroy(r.b);
+        }
+        for (auto e : free_ev) (void) hipEventDestroy(e);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Problem
 // ---------------------------------------------------------------------------
 struct Problem {
+    Profiler prof;
     int device = 0;
     hipStream_t stream = nullptr;
     mgpu_options opt{};

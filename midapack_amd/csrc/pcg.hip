@@ -323,19 +323,27 @@ int ensure_work(Problem &pb) {
 
 // A v = P^T N^-1 P v in FP64 (map -> map), fused <v, Av> partials optional
 static int apply_A64(Problem &pb, const double *d_v, double *d_out, double *d_dot) {
+    pb.prof.begin(MGPU_KT_P, pb.stream);
     MGPU_RET(launch_matvec(pb, d_v, pb.d_tod, true));
+    pb.prof.end(pb.stream);
     MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, false));
+    pb.prof.begin(MGPU_KT_PT, pb.stream);
     MGPU_RET(launch_trmatvec(pb, pb.d_tod2, d_out, d_dot ? d_v : nullptr, d_dot, nullptr));
+    pb.prof.end(pb.stream);
     return MGPU_OK;
 }
 
 static int apply_A32(Problem &pb, const double *d_v, double *d_out, double *d_dot, const int *d_flag) {
     const int n = 3 * pb.pt.npix;
+    pb.prof.begin(MGPU_KT_P, pb.stream);
     k_map_to_f32<<<grid_for(n, 256), 256, 0, pb.stream>>>(d_v, pb.d_mapf, n, d_flag);
     MGPU_LAUNCHED();
     MGPU_RET(launch_matvec_f32(pb, pb.d_mapf, pb.d_todf, true));
+    pb.prof.end(pb.stream);
     MGPU_RET(toeplitz_apply_f32(pb, pb.d_todf, pb.d_tod2f, false));
+    pb.prof.begin(MGPU_KT_PT, pb.stream);
     MGPU_RET(launch_trmatvec_f32(pb, pb.d_tod2f, d_out, d_dot ? d_v : nullptr, d_dot, nullptr));
+    pb.prof.end(pb.stream);
     return MGPU_OK;
 }
 
@@ -391,6 +399,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     while (!done) {
         for (int it = 0; it < poll && launched + 1 < maxiter; it++) {
             ++launched;
+            pb.prof.tag = launched;
             std::swap(g, gp);  // pcg_true.c:140-142
             // q = A h with fused <h, q>
             if (mixed)
@@ -398,30 +407,41 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             else
                 MGPU_RET(apply_A64(pb, h, qv, part));
             nparts_pt = trmatvec_nparts(pb);
+            pb.prof.begin(MGPU_KT_MAP, s);
             k_coeff<<<1, 256, 0, s>>>(part, nparts_pt, S, flag);
             MGPU_LAUNCHED();
             k_update<<<gmap, kMB, 0, s>>>(x, h, g, gp, qv, pb.pc.d_inv, Cg, npix, S, part, flag);
             MGPU_LAUNCHED();
+            pb.prof.end(s);
             if (refresh && launched % refresh == 0) {
                 // g = b - A x in FP64, then Cg = M g and g2pix on the refreshed residual
+                pb.prof.begin(MGPU_KT_REFRESH, s);
+                const bool keep = pb.prof.on;
+                pb.prof.on = false;  // the FP64 operator inside is accounted as a refresh
                 MGPU_RET(apply_A64(pb, x, Ax, nullptr));
                 k_refresh<<<grid_for(n, kMB), kMB, 0, s>>>(bvec, Ax, g, n, flag);
                 MGPU_LAUNCHED();
                 // the three dots of the stop test and of gamma on the refreshed g
                 k_refresh_three<<<gmap, kMB, 0, s>>>(pb.pc.d_inv, g, gp, Cg, npix, part, flag);
                 MGPU_LAUNCHED();
+                pb.prof.on = keep;
+                pb.prof.end(s);
                 refreshes++;
             }
+            pb.prof.begin(MGPU_KT_MAP, s);
             k_scalars<<<1, 256, 0, s>>>(part, (int) gmap, S, d_hist, flag);
             MGPU_LAUNCHED();
             k_hupdate<<<grid_for(n, kMB), kMB, 0, s>>>(h, Cg, n, S, flag);
             MGPU_LAUNCHED();
+            pb.prof.end(s);
         }
         int hf[4];
         MGPU_HIP(hipMemcpyAsync(hf, flag, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
         MGPU_HIP(hipStreamSynchronize(s));
+        pb.prof.collect(hf[2]);
         if (hf[0] || launched + 1 >= maxiter) done = true;
     }
+    pb.prof.tag = 0;
     MGPU_HIP(hipEventRecord(pb.ev1, s));
     // results
     int hf[4] = {0, 0, 0, 0};
@@ -452,8 +472,12 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         rep->true_res_rel = -1.0;
     }
     if (mixed && rep) {
-        // true residual of the returned x in FP64: ||b - A x|| / ||b||
-        MGPU_RET(apply_A64(pb, x, Ax, nullptr));
+        // true residual of the returned x in FP64: ||b - A x|| / ||b|| (not timed)
+        const bool keep = pb.prof.on;
+        pb.prof.on = false;
+        const int rc = apply_A64(pb, x, Ax, nullptr);
+        pb.prof.on = keep;
+        MGPU_RET(rc);
         k_refresh<<<grid_for(n, kMB), kMB, 0, s>>>(bvec, Ax, qv, n, pb.d_flag + 4);
         MGPU_LAUNCHED();
         const unsigned gd = grid_for(n, 256);

@@ -565,7 +565,7 @@ int launch_matvec(Problem &pb, const double *d_x, double *d_y, bool partials) {
     const Pointing &P = pb.pt;
     const ToeplitzPlan &T = pb.tp;
     const unsigned g = grid_for(P.m, kChunk);
-    partials = partials && T.nb > 0 && T.d_partials;
+    partials = partials && T.nb > 0 && T.d_partials && T.lambda > 1 && T.offset_mode != MGPU_OFFSET_NONE;
     if (P.unit_w0) {
         if (partials)
             k_matvec<double, double, false, true><<<g, kPBlock, 0, pb.stream>>>(
@@ -589,7 +589,7 @@ int launch_matvec_f32(Problem &pb, const float *d_x, float *d_y, bool partials) 
     const Pointing &P = pb.pt;
     const ToeplitzPlan &T = pb.tp;
     const unsigned g = grid_for(P.m, kChunk);
-    partials = partials && T.nb > 0 && T.d_partials;
+    partials = partials && T.nb > 0 && T.d_partials && T.lambda > 1 && T.offset_mode != MGPU_OFFSET_NONE;
     if (P.unit_w0) {
         if (partials)
             k_matvec<float, float, false, true><<<g, kPBlock, 0, pb.stream>>>(

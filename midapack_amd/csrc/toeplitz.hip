@@ -343,6 +343,7 @@ struct TileArgs {
     const void *coef;        // Cx<R> [nfilters * NC]
     const void *tw;          // Cx<R> [NC]
     const double *mean;      // [nb]
+    const double *mean_out;  // [nb] subtracted from the outputs (0 unless MGPU_OFFSET_SYMMETRIC)
     int lambda;
     int L;
 };
@@ -389,6 +390,7 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV) void k_tiles(TileArgs a) {
     const Cx<R> *__restrict__ tw = reinterpret_cast<const Cx<R> *>(a.tw);
     const Cx<R> *__restrict__ cf = reinterpret_cast<const Cx<R> *>(a.coef) + (int64_t) d.filter * NC;
     const R mean = (R) a.mean[d.block];
+    const R mean_out = (R) a.mean_out[d.block];
     const int64_t in0 = d.out_lo - (a.lambda - 1);
 
     // block-relative 32-bit offsets from a uniform base: loads are clamped into
@@ -453,8 +455,8 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV) void k_tiles(TileArgs a) {
             const int n = j + r * (NC / RAD0);
             const int i0 = 2 * n;
             const int o0 = rel0 + i0;
-            if (i0 >= lo && i0 < hi && o0 < nblk) zb[(unsigned) o0] = re[b * RAD0 + r];
-            if (i0 + 1 >= lo && i0 + 1 < hi && o0 + 1 < nblk) zb[(unsigned) (o0 + 1)] = -im[b * RAD0 + r];
+            if (i0 >= lo && i0 < hi && o0 < nblk) zb[(unsigned) o0] = re[b * RAD0 + r] - mean_out;
+            if (i0 + 1 >= lo && i0 + 1 < hi && o0 + 1 < nblk) zb[(unsigned) (o0 + 1)] = -im[b * RAD0 + r] - mean_out;
         }
     }
     (void) NP;
@@ -500,23 +502,58 @@ __global__ __launch_bounds__(256) void k_partials(const R *__restrict__ y, int64
     }
 }
 
-// mean[b] = (sum of the block's partials, fixed order) / n_b
+__device__ __forceinline__ double tree256(double v, double *red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// mean[b] = (sum of the block's partials, fixed order) / n_b.
+// With mean_out (MGPU_OFFSET_SYMMETRIC) also the mean of T (v - mean) over the
+// block: T 1 equals its interior value c = T0 + 2 sum_k T_k except within
+// lambda-1 of the block ends, where it is c + e_i with
+// e_i = -tail(i) - tail(n-1-i), tail(j) = sum_{k>j} T_k; since v - mean sums
+// to zero, mean(T (v - mean)) = (1/n) sum_i e_i (v_i - mean), a sum over the
+// 2(lambda-1) edge samples only.
+template <typename R>
 __global__ __launch_bounds__(256) void k_block_mean(const double *__restrict__ partials,
                                                     const int64_t *__restrict__ start, int nb,
-                                                    double *__restrict__ mean) {
+                                                    double *__restrict__ mean, const R *__restrict__ v,
+                                                    const double *__restrict__ tail,
+                                                    const int *__restrict__ block_filter,
+                                                    const int *__restrict__ block_kind, int lambda,
+                                                    double *__restrict__ mean_out) {
     __shared__ double red[256];
     const int b = blockIdx.x;
     const int64_t lo = start[b], hi = start[b + 1];
     const int64_t first = lo / kChunk + b, last = (hi - 1) / kChunk + b;
     double s = 0.0;
     for (int64_t i = first + threadIdx.x; i <= last; i += 256) s += partials[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+    const double mu = tree256(s, red) / (double) (hi - lo);
+    if (threadIdx.x == 0) mean[b] = mu;
+    if (!mean_out) return;
+    if (block_kind[b] != 0) {  // short block: identity product
+        if (threadIdx.x == 0) mean_out[b] = 0.0;
+        return;
     }
-    if (threadIdx.x == 0) mean[b] = red[0] / (double) (hi - lo);
+    const int n = (int) (hi - lo), L1 = lambda - 1;
+    const double *tl = tail + (int64_t) block_filter[b] * lambda;
+    const int a_end = min(L1, n);
+    const int b_beg = max(n - L1, a_end);
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < a_end + (n - b_beg); i += 256) {
+        const int k = i < a_end ? i : b_beg + (i - a_end);
+        const double e = -(k < L1 ? tl[k] : 0.0) - (n - 1 - k < L1 ? tl[n - 1 - k] : 0.0);
+        acc += e * ((double) v[lo + k] - mu);
+    }
+    const double tot = tree256(acc, red);
+    if (threadIdx.x == 0) mean_out[b] = tot / (double) n;
 }
 
 // z = y - mean over the listed (short) blocks
@@ -551,17 +588,17 @@ __global__ void k_diag(const R *__restrict__ y, R *__restrict__ z, int64_t m, co
 template <typename R>
 __global__ void k_direct(const R *__restrict__ y, R *__restrict__ z, const int64_t *__restrict__ start,
                          const int *__restrict__ blocks, const double *__restrict__ mean,
-                         const double *__restrict__ invtt, int lambda) {
+                         const double *__restrict__ mean_out, const double *__restrict__ invtt, int lambda) {
     const int b = blocks[blockIdx.y];
     const int64_t lo = start[b], hi = start[b + 1];
-    const double mu = mean[b];
+    const double mu = mean[b], mo = mean_out[b];
     const double *T = invtt + (int64_t) b * lambda;
     for (int64_t i = lo + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < hi;
          i += (int64_t) gridDim.x * blockDim.x) {
         const int64_t j0 = max(lo, i - (lambda - 1)), j1 = min(hi, i + lambda);
         double acc = 0.0;
         for (int64_t j = j0; j < j1; j++) acc += T[j > i ? j - i : i - j] * ((double) y[j] - mu);
-        z[i] = (R) acc;
+        z[i] = (R) (acc - mo);
     }
 }
 
@@ -695,6 +732,12 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
     T.nchunks = (T.m + kChunk - 1) / kChunk;
     MGPU_RET(pb.arena.alloc(&T.d_partials, (size_t) (T.nchunks + nb)));
     MGPU_RET(pb.arena.alloc(&T.d_mean, (size_t) nb));
+    MGPU_RET(pb.arena.alloc(&T.d_mean_out, (size_t) nb));
+    MGPU_HIP(hipMemsetAsync(T.d_mean, 0, sizeof(double) * nb, s));
+    MGPU_HIP(hipMemsetAsync(T.d_mean_out, 0, sizeof(double) * nb, s));
+    T.offset_mode = pb.opt.offset_mode;
+    MGPU_ARG(T.offset_mode >= MGPU_OFFSET_REFERENCE && T.offset_mode <= MGPU_OFFSET_NONE, "unknown offset_mode %d",
+             T.offset_mode);
     if (lambda == 1) {
         MGPU_HIP(hipStreamSynchronize(s));
         return MGPU_OK;
@@ -734,6 +777,29 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
             }
             T.h_block_filter[b] = fid;
         }
+    }
+    // block-edge tails for the symmetric offset projection: tail_f[j] = sum_{k>j} T_f[k]
+    {
+        const int nf = (int) filter_src.size();
+        std::vector<double> tails((size_t) nf * lambda, 0.0);
+        for (int f = 0; f < nf; f++) {
+            const double *Tf = inv + (int64_t) filter_src[f] * lambda;
+            double *tl = tails.data() + (int64_t) f * lambda;
+            double acc = 0.0;
+            for (int j = lambda - 1; j >= 0; j--) {
+                tl[j] = acc;
+                acc += Tf[j];
+            }
+        }
+        std::vector<int> kind(nb);
+        for (int b = 0; b < nb; b++) kind[b] = block_sizes[b] < lambda - 1 ? 1 : 0;
+        MGPU_RET(pb.arena.alloc(&T.d_tail, tails.size()));
+        MGPU_RET(pb.arena.alloc(&T.d_block_filter, (size_t) nb));
+        MGPU_RET(pb.arena.alloc(&T.d_block_kind, (size_t) nb));
+        MGPU_HIP(hipMemcpyAsync(T.d_tail, tails.data(), sizeof(double) * tails.size(), hipMemcpyHostToDevice, s));
+        MGPU_HIP(hipMemcpyAsync(T.d_block_filter, T.h_block_filter.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+        MGPU_HIP(hipMemcpyAsync(T.d_block_kind, kind.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+        MGPU_HIP(hipStreamSynchronize(s));
     }
     std::vector<TileDesc> tiles;
     if (l2 > 0) {
@@ -825,7 +891,7 @@ static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *co
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
         attr_set = true;
     }
-    TileArgs a{d_in, d_out, T.d_tiles, coef, tw, T.d_mean, T.lambda, T.L};
+    TileArgs a{d_in, d_out, T.d_tiles, coef, tw, T.d_mean, T.d_mean_out, T.lambda, T.L};
     hipLaunchKernelGGL(kern, dim3(T.ntiles), dim3(NT), lds, pb.stream, a);
     MGPU_LAUNCHED();
     return MGPU_OK;
@@ -859,16 +925,26 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
         MGPU_LAUNCHED();
         return MGPU_OK;
     }
-    if (compute_partials) {
-        k_partials<R><<<(unsigned) T.nchunks, 256, 0, s>>>(d_in, T.m, T.d_start, T.nb, T.d_partials);
+    if (T.offset_mode != MGPU_OFFSET_NONE) {  // MGPU_OFFSET_NONE: means stay 0
+        pb.prof.begin(MGPU_KT_MEAN, s);
+        if (compute_partials) {
+            k_partials<R><<<(unsigned) T.nchunks, 256, 0, s>>>(d_in, T.m, T.d_start, T.nb, T.d_partials);
+            MGPU_LAUNCHED();
+        }
+        k_block_mean<R><<<T.nb, 256, 0, s>>>(T.d_partials, T.d_start, T.nb, T.d_mean, d_in, T.d_tail,
+                                             T.d_block_filter, T.d_block_kind, T.lambda,
+                                             T.offset_mode == MGPU_OFFSET_SYMMETRIC ? T.d_mean_out : nullptr);
         MGPU_LAUNCHED();
+        pb.prof.end(s);
     }
-    k_block_mean<<<T.nb, 256, 0, s>>>(T.d_partials, T.d_start, T.nb, T.d_mean);
-    MGPU_LAUNCHED();
-    if (T.ntiles) MGPU_RET(launch_tiles<R>(pb, d_in, d_out));
+    if (T.ntiles) {
+        pb.prof.begin(MGPU_KT_TILES, s);
+        MGPU_RET(launch_tiles<R>(pb, d_in, d_out));
+        pb.prof.end(s);
+    }
     if (!T.h_direct_blocks.empty()) {
         k_direct<R><<<dim3(64, (unsigned) T.h_direct_blocks.size()), 256, 0, s>>>(
-                d_in, d_out, T.d_start, T.d_direct_blocks, T.d_mean, T.d_direct_invtt, T.lambda);
+                d_in, d_out, T.d_start, T.d_direct_blocks, T.d_mean, T.d_mean_out, T.d_direct_invtt, T.lambda);
         MGPU_LAUNCHED();
     }
     if (!T.h_short_blocks.empty()) {

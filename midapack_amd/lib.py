@@ -19,6 +19,9 @@ LIB_PATH = os.path.join(_HERE, "libmappraiser_gpu.so")
 
 MGPU_FP64 = 0
 MGPU_MIXED = 1
+MGPU_OFFSET_REFERENCE = 0
+MGPU_OFFSET_SYMMETRIC = 1
+MGPU_OFFSET_NONE = 2
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -31,13 +34,17 @@ EXPORTS = [
     "mappraiser_gpu_problem_create_device", "mappraiser_gpu_problem_destroy",
     "mappraiser_gpu_local_shape", "mappraiser_gpu_matvec", "mappraiser_gpu_trmatvec",
     "mappraiser_gpu_apply_weights", "mappraiser_gpu_precond_build", "mappraiser_gpu_precond_apply",
-    "mappraiser_gpu_pcg", "mappraiser_gpu_solve_device", "mappraiser_gpu_pcg_gls",
+    "mappraiser_gpu_pcg", "mappraiser_gpu_solve_device", "mappraiser_gpu_kernel_times", "mappraiser_gpu_pcg_gls",
 ]
+
+# kernel classes of mappraiser_gpu_kernel_times (MGPU_KT_*)
+KERNEL_CLASSES = ["P", "mean", "tiles", "Pt", "map", "refresh"]
 
 
 class Options(ct.Structure):
     _fields_ = [("precision", ct.c_int), ("refresh_every", ct.c_int), ("tile_log2", ct.c_int),
-                ("poll_every", ct.c_int), ("reserved", ct.c_int * 4)]
+                ("poll_every", ct.c_int), ("profile", ct.c_int), ("offset_mode", ct.c_int),
+                ("reserved", ct.c_int * 2)]
 
 
 class Report(ct.Structure):
@@ -84,6 +91,7 @@ def load():
                                      ct.POINTER(Report)]
     L.mappraiser_gpu_solve_device.argtypes = [_vp, _vp, _vp, ct.c_double, ct.c_int, _f64p, _vp, _vp, _vp,
                                               ct.POINTER(Report)]
+    L.mappraiser_gpu_kernel_times.argtypes = [_vp, _vp, _vp, ct.c_int]
     L.mappraiser_gpu_pcg_gls.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p,
                                          ct.c_int, _i32p, ct.c_int, _f64p, ct.c_double, ct.c_int, _f64p, _f64p,
                                          _i32p, _i32p, ct.POINTER(Options), ct.POINTER(Report)]
@@ -100,10 +108,14 @@ def _check(rc: int, what: str):
         raise GPUError(f"{what} failed (status {rc}): {last_error()}")
 
 
-def options(precision=MGPU_FP64, refresh_every=10, tile_log2=0, poll_every=8) -> Options:
+def options(precision=MGPU_FP64, refresh_every=10, tile_log2=0, poll_every=8, profile=False,
+            offset_mode=None) -> Options:
     o = Options()
     load().mappraiser_gpu_options_default(ct.byref(o))
     o.precision, o.refresh_every, o.tile_log2, o.poll_every = precision, refresh_every, tile_log2, poll_every
+    o.profile = int(bool(profile))
+    if offset_mode is not None:
+        o.offset_mode = int(offset_mode)
     return o
 
 
@@ -210,6 +222,25 @@ class Problem:
         out = np.empty(self.map_size(), np.float64)
         _check(load().mappraiser_gpu_precond_apply(self._h, _c(g, np.float64), out), "precond_apply")
         return out
+
+    def kernel_times(self, reset=False):
+        """{class: (total ms, launches)} measured with HIP events (needs options(profile=True))."""
+        ms = np.zeros(len(KERNEL_CLASSES), np.float64)
+        cnt = np.zeros(len(KERNEL_CLASSES), np.int64)
+        _check(load().mappraiser_gpu_kernel_times(self._h, ms.ctypes.data, cnt.ctypes.data, int(reset)),
+               "kernel_times")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+    def solve_device(self, d_signal, d_noise, tol=1e-8, maxiter=500):
+        """precond_build + PCG on device-resident signal/noise (d_signal is modified on cut samples)."""
+        npix0 = self.map_size() // 3
+        x = np.empty(max(3 * npix0, 1), np.float64)
+        hist = np.zeros(max(maxiter, 1), np.float64)
+        rep = Report()
+        _check(load().mappraiser_gpu_solve_device(self._h, _vp(d_signal), _vp(d_noise), float(tol), int(maxiter), x,
+                                                  hist.ctypes.data, None, None, ct.byref(rep)), "solve_device")
+        nh = min(rep.iterations + 1, maxiter)
+        return x[: rep.n_map], hist[:nh], rep
 
     def pcg(self, signal, noise, tol=1e-8, maxiter=500, stop_after=0, history=True):
         n = self.map_size()

@@ -87,10 +87,12 @@ def invtt_block(nb_samp, lam, sigma, fknee, alpha, fsamp=FSAMP):
 
 
 def make_workload(name, ndet, nsamp, nx, ny, lam, seed, block_len=None, fknee=0.1, alpha=1.0,
-                  sigma=1.0, fknee_range=None, alpha_range=None, det_lengths=None, flag_frac=0.0):
+                  sigma=1.0, fknee_range=None, alpha_range=None, det_lengths=None, flag_frac=0.0,
+                  crosslink=False):
     """Build one rank's data.  ``det_lengths`` overrides ``nsamp`` per detector;
     ``block_len`` splits every detector into Toeplitz blocks of that length
-    (the last block takes the remainder); all blocks of a detector share its T."""
+    (the last block takes the remainder); all blocks of a detector share its T.
+    ``crosslink`` makes every odd detector scan along columns instead of rows."""
     lens = [int(nsamp)] * ndet if det_lengths is None else [int(v) for v in det_lengths]
     m = int(sum(lens))
     npix_grid = nx * ny
@@ -111,7 +113,11 @@ def make_workload(name, ndet, nsamp, nx, ny, lam, seed, block_len=None, fknee=0.
         ox, oy = int(rd.integers(0, nx)), int(rd.integers(0, ny))
         fk = fknee if fknee_range is None else float(rd.uniform(*fknee_range))
         al = alpha if alpha_range is None else float(rd.uniform(*alpha_range))
-        p = raster_pixels(n_d, nx, ny, ox, oy)
+        if crosslink and d % 2 == 1:
+            pt = raster_pixels(n_d, ny, nx, oy, ox)  # raster of the transposed grid
+            p = (pt % ny) * nx + pt // ny
+        else:
+            p = raster_pixels(n_d, nx, ny, ox, oy)
         psi = _rng(seed, 2, d).random(n_d) * np.pi
         c2, s2 = np.cos(2 * psi), np.sin(2 * psi)
         sl = slice(t0, t0 + n_d)

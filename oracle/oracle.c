@@ -369,12 +369,28 @@ static void circ_spectrum(const double *T, int bs, int lambda, cplx *Tfft, cplx 
 /* toeplitz.c:651-700 (scmm_basic) + 569-616 (scmm_direct), one column, nfft=1:
  * V_rfft = V_bloc; r2c; V_fft <- C_fft * V_fft (in place, the updated real part
  * is reused for the imaginary part as at lines 587-588); c2r; TV = V_rfft/bs. */
+static int g_ninv_mode = ORC_NINV_REFERENCE;
+
+int orc_set_ninv_mode(int mode) {
+    int old = g_ninv_mode;
+    if (mode >= ORC_NINV_REFERENCE && mode <= ORC_NINV_DIRECT) g_ninv_mode = mode;
+    return old;
+}
+
 static void scmm(const double *Vb, double *TVb, int bs, const cplx *Cf, cplx *Vf, cplx *work) {
     int sizeT = bs / 2 + 1;
     rfft_any(bs, Vb, Vf, work);
-    for (int i = 0; i < sizeT; i++) {
-        Vf[i].re = Cf[i].re * Vf[i].re - Cf[i].im * Vf[i].im;
-        Vf[i].im = Cf[i].re * Vf[i].im + Cf[i].im * Vf[i].re;
+    if (g_ninv_mode == ORC_NINV_REFERENCE) {
+        for (int i = 0; i < sizeT; i++) {
+            Vf[i].re = Cf[i].re * Vf[i].re - Cf[i].im * Vf[i].im;
+            Vf[i].im = Cf[i].re * Vf[i].im + Cf[i].im * Vf[i].re;
+        }
+    } else {
+        /* the spectrum of the real symmetric circulant embedding is real */
+        for (int i = 0; i < sizeT; i++) {
+            Vf[i].re = Cf[i].re * Vf[i].re;
+            Vf[i].im = Cf[i].re * Vf[i].im;
+        }
     }
     irfft_any(bs, Vf, TVb, work);
     const double norm = 1.0 / ((double) bs);
@@ -405,7 +421,7 @@ static void stmm_direct(double *V, int n, const double *T, int lambda) {
 int orc_stmm_block(double *V, int n, const double *T, int lambda, int use_fft) {
     const int dcm = lambda - 1;
     if (n < dcm) return 1; /* stmm_main:912-913: block left unchanged */
-    if (!use_fft) {
+    if (!use_fft || g_ninv_mode == ORC_NINV_DIRECT) {
         stmm_direct(V, n, T, lambda);
         return ORC_OK;
     }
@@ -457,6 +473,14 @@ int orc_stmm_block(double *V, int n, const double *T, int lambda, int use_fft) {
  * mpi_stbmm (toeplitz_block.c:115-636), which on one rank with contiguous
  * blocks processes every block [idv, idv+n) independently.  Samples past the
  * last block are left untouched. */
+static int g_offset_mode = ORC_OFFSET_REFERENCE;
+
+int orc_set_offset_mode(int mode) {
+    int old = g_offset_mode;
+    if (mode >= ORC_OFFSET_REFERENCE && mode <= ORC_OFFSET_NONE) g_offset_mode = mode;
+    return old;
+}
+
 int orc_stbmm(int nb, const int *block_sizes, int lambda, const double *invtt, int64_t m,
               double *V, int use_fft) {
     int64_t tot = 0;
@@ -466,19 +490,29 @@ int orc_stbmm(int nb, const int *block_sizes, int lambda, const double *invtt, i
     }
     if (tot > m) return ORC_EINVAL;
     int64_t t_id = 0;
-    for (int b = 0; b < nb; b++) {
-        const int n = block_sizes[b];
-        double offset = 0;
-        for (int j = 0; j < n; j++) offset += V[t_id + j];
-        offset /= n;
-        for (int j = 0; j < n; j++) V[t_id + j] -= offset;
-        t_id += n;
+    if (g_offset_mode != ORC_OFFSET_NONE) {
+        for (int b = 0; b < nb; b++) {
+            const int n = block_sizes[b];
+            double offset = 0;
+            for (int j = 0; j < n; j++) offset += V[t_id + j];
+            offset /= n;
+            for (int j = 0; j < n; j++) V[t_id + j] -= offset;
+            t_id += n;
+        }
     }
     t_id = 0;
     for (int b = 0; b < nb; b++) {
-        int rc = orc_stmm_block(V + t_id, block_sizes[b], invtt + (int64_t) b * lambda, lambda, use_fft);
+        const int n = block_sizes[b];
+        int rc = orc_stmm_block(V + t_id, n, invtt + (int64_t) b * lambda, lambda, use_fft);
         if (rc < 0) return rc;
-        t_id += block_sizes[b];
+        if (g_offset_mode == ORC_OFFSET_SYMMETRIC) {
+            /* the projection C applied again after the product: C T C */
+            double mo = 0;
+            for (int j = 0; j < n; j++) mo += V[t_id + j];
+            mo /= n;
+            for (int j = 0; j < n; j++) V[t_id + j] -= mo;
+        }
+        t_id += n;
     }
     return ORC_OK;
 }

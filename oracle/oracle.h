@@ -63,6 +63,29 @@ int orc_fft(int n, const double *in, double *out, int sign);
 /* ---- Toeplitz (toeplitz.c:168-253, 441-470, 569-700, 731-865; toeplitz_block.c:115-636;
  *      toeplitz_wizard.c:80-111; pcg_true.c:578-600) ---- */
 int orc_define_blocksize(int n, int lambda);
+/* How the Toeplitz tiles are computed (all other steps are unchanged):
+ *   ORC_NINV_REFERENCE  FFT shift-and-overlap with the reference's in-place
+ *                       complex multiply (toeplitz.c:587-588), whose imaginary
+ *                       line reuses the updated real part: the rounding noise
+ *                       of Im FFT(T_circ) enters multiplied by Re FFT(T_circ),
+ *                       an error of order eps*|T|^2 (about 6e-10 relative on
+ *                       the reference's own test data, T0 ~ 1e7).  Default.
+ *   ORC_NINV_REAL_SPECTRUM  same tiles, multiplied by the real spectrum.
+ *   ORC_NINV_DIRECT     direct banded product (the closed form).
+ * Returns the previous mode. */
+#define ORC_NINV_REFERENCE 0
+#define ORC_NINV_REAL_SPECTRUM 1
+#define ORC_NINV_DIRECT 2
+int orc_set_ninv_mode(int mode);
+/* Per-block offset handling of N^-1 (C = I - 1 1^T/n per block):
+ *   ORC_OFFSET_REFERENCE  T C, the reference (toeplitz_wizard.c:86-95).  Default.
+ *   ORC_OFFSET_SYMMETRIC  C T C, the projection applied again after the product.
+ *   ORC_OFFSET_NONE       T.
+ * Returns the previous mode. */
+#define ORC_OFFSET_REFERENCE 0
+#define ORC_OFFSET_SYMMETRIC 1
+#define ORC_OFFSET_NONE 2
+int orc_set_offset_mode(int mode);
 /* one block, in place: banded symmetric Toeplitz product with zero boundaries.
  * use_fft=1: shift-and-overlap with the reference's tile rule; 0: direct O(n lambda). */
 int orc_stmm_block(double *V, int n, const double *T, int lambda, int use_fft);

@@ -52,6 +52,8 @@ def lib():
         L.orc_pixel_lists.restype = None
         L.orc_fft.argtypes = [ct.c_int, _f64p, _f64p, ct.c_int]
         L.orc_define_blocksize.argtypes = [ct.c_int, ct.c_int]
+        L.orc_set_ninv_mode.argtypes = [ct.c_int]
+        L.orc_set_offset_mode.argtypes = [ct.c_int]
         L.orc_stmm_block.argtypes = [_f64p, ct.c_int, _f64p, ct.c_int, ct.c_int]
         L.orc_stbmm.argtypes = [ct.c_int, _i32p, ct.c_int, _f64p, ct.c_int64, _f64p, ct.c_int]
         L.orc_apply_weights.argtypes = [ct.c_int, _i32p, ct.c_int, _f64p, ct.c_int64, _f64p, ct.c_int]
@@ -173,6 +175,43 @@ def fft(z, sign=-1):
     if rc:
         raise RuntimeError(rc)
     return out.view(np.complex128)
+
+
+NINV_REFERENCE, NINV_REAL_SPECTRUM, NINV_DIRECT = 0, 1, 2
+
+
+def set_ninv_mode(mode: int) -> int:
+    """Select how the oracle computes the Toeplitz tiles (see oracle.h); returns the old mode."""
+    return int(lib().orc_set_ninv_mode(int(mode)))
+
+
+class ninv_mode:
+    """Context manager: ``with ninv_mode(NINV_DIRECT): ...``"""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        self.old = set_ninv_mode(self.mode)
+
+    def __exit__(self, *a):
+        set_ninv_mode(self.old)
+
+
+OFFSET_REFERENCE, OFFSET_SYMMETRIC, OFFSET_NONE = 0, 1, 2
+
+
+class offset_mode:
+    """Context manager selecting N^-1's per-block offset handling (see oracle.h)."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        self.old = int(lib().orc_set_offset_mode(int(self.mode)))
+
+    def __exit__(self, *a):
+        lib().orc_set_offset_mode(self.old)
 
 
 def define_blocksize(n, lam):

@@ -1,0 +1,113 @@
+"""Writes the committed golden fixtures under tests/golden/ (run in the build
+container, where /root/reference and oracle/_ref exist):
+
+  fixture_rank1.npz    inputs of rank 1 of the reference's own C test data
+                       (mappraiser/test/data_bin/*_1.bin: 48,000 samples, two
+                       24,000-sample Toeplitz blocks, lambda = 8192), stored as
+                       p = pix[3t]/3 and the Q/U weights after checking that
+                       pix[3t+k] = 3p+k and that the I weight is exactly 1.
+  ref_mapmat_rank1.npz outputs of the reference's mapmat.c/csort.c/cindex.c
+                       compiled from source (oracle/_ref): lindices, local
+                       indices, MatVecProd(x) and TrMatVecProd(y) for seeded x, y.
+  ref_mapmat_flags.npz the same on a synthetic pointing with flagged samples
+                       (-3,-2,-1 rows), which exercises the trash-pixel branches.
+  lapack_3x3.npz       dlange('1'), dgetrf, dgecon('1'), dgetri from scipy's
+                       LAPACK on random, SPD, map-like and near-singular 3x3 blocks.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+DATA = "/root/reference/mappraiser/test/data_bin"
+sys.path.insert(0, ROOT)
+
+
+def fixture_rank1():
+    r = 1
+    pix = np.fromfile(f"{DATA}/pixels_{r}.bin", dtype=np.int32)
+    w = np.fromfile(f"{DATA}/pixweights_{r}.bin", dtype=np.float64)
+    sig = np.fromfile(f"{DATA}/signal_{r}.bin", dtype=np.float64)
+    noise = np.fromfile(f"{DATA}/noise_{r}.bin", dtype=np.float64)
+    inv = np.fromfile(f"{DATA}/invtt_{r}.bin", dtype=np.float64)
+    blocks = np.fromfile(f"{DATA}/local_blocks_sizes_{r}.bin", dtype=np.int32)
+    m = sig.size
+    assert pix.size == 3 * m and w.size == 3 * m and blocks.sum() == m
+    assert np.all(pix[0::3] % 3 == 0) and np.all(pix[1::3] == pix[0::3] + 1) and np.all(pix[2::3] == pix[0::3] + 2)
+    assert np.all(w[0::3] == 1.0)
+    lam = inv.size // blocks.size
+    np.savez_compressed(os.path.join(HERE, "fixture_rank1.npz"), p=(pix[0::3] // 3).astype(np.int32),
+                        wq=w[1::3].copy(), wu=w[2::3].copy(), signal=sig, noise=noise, invtt=inv,
+                        block_sizes=blocks, lam=np.int32(lam))
+    return pix, w
+
+
+def ref_goldens(name, pix, w, seed):
+    from oracle import oracle as O
+
+    lind, loc, trash = O.ref_local_shape(pix, w, 3)
+    n = lind.size - (3 if trash else 0)
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(pix.size // 3)
+    Px, _ = O.ref_products(pix, w, 3, x=x)
+    _, Pty = O.ref_products(pix, w, 3, y=y)
+    np.savez_compressed(os.path.join(HERE, name), lindices=lind, indices=loc, trash=np.int32(trash), x=x, y=y,
+                        Px=Px, Pty=Pty)
+
+
+def flagged_pointing():
+    from midapack_amd import workloads as W
+
+    wl = W.config_small(seed=21, ndet=2, nsamp=3000, nx=12, ny=12, lam=32, flag_frac=0.05)
+    return wl.pix, wl.weights
+
+
+def lapack_vectors():
+    import scipy.linalg.lapack as L
+
+    rng = np.random.default_rng(12345)
+    mats = []
+    for i in range(300):
+        A = rng.standard_normal((3, 3))
+        mats.append(A)
+        mats.append(A @ A.T + 1e-3 * np.eye(3))
+    # map-like blocks: sum over hits of w w^T T0 with w = (1, cos 2psi, sin 2psi)
+    for nh in (1, 2, 3, 4, 6, 16, 64):
+        for _ in range(40):
+            psi = rng.random(nh) * np.pi
+            Wm = np.stack([np.ones(nh), np.cos(2 * psi), np.sin(2 * psi)], 1)
+            mats.append(1.3e7 * (Wm.T @ Wm))
+    mats.append(np.zeros((3, 3)))
+    mats.append(np.diag([1.0, 0.0, 2.0]))
+    mats = np.array(mats)
+    out = {k: [] for k in ("anorm", "lu", "ipiv", "info", "rcond", "inv", "inv_info")}
+    for A in mats:
+        an = L.dlange("1", A)
+        lu, piv, info = L.dgetrf(A)
+        rc, _ = L.dgecon(lu, an, norm="1")
+        inv, info2 = L.dgetri(lu, piv) if info == 0 else (np.zeros((3, 3)), info)
+        out["anorm"].append(an)
+        out["lu"].append(lu)
+        out["ipiv"].append(piv)
+        out["info"].append(info)
+        out["rcond"].append(rc)
+        out["inv"].append(inv)
+        out["inv_info"].append(info2)
+    np.savez_compressed(os.path.join(HERE, "lapack_3x3.npz"), A=mats, **{k: np.array(v) for k, v in out.items()})
+
+
+if __name__ == "__main__":
+    pix, w = fixture_rank1()
+    ref_goldens("ref_mapmat_rank1.npz", pix, w, seed=101)
+    fp, fw = flagged_pointing()
+    ref_goldens("ref_mapmat_flags.npz", fp, fw, seed=202)
+    lapack_vectors()
+    for f in sorted(os.listdir(HERE)):
+        print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -1,0 +1,281 @@
+"""GPU parity: the HIP path, called through the C-ABI, against the reference's
+own outputs (committed goldens of the compiled mapmat code) and the pinned CPU
+oracle on the same seeded inputs.
+
+Bars: integer tables (lindices, local indices, hits, cut set) and the pointing
+products bit-exact; FP64 operators within 1e-12 relative of the exact product;
+PCG iterates x_k within 1e-12 relative for small k.
+"""
+import numpy as np
+import pytest
+
+from midapack_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _flagged():
+    wl = W.config_small(seed=21, ndet=2, nsamp=3000, nx=12, ny=12, lam=32, flag_frac=0.05)
+    return wl.pix, wl.weights
+
+
+# ----------------------------------------------------------------------------
+# index tables and P / P^T against the reference's compiled mapmat (goldens)
+# ----------------------------------------------------------------------------
+def test_local_shape_and_products_match_reference_fixture(gpu, fixture_rank1, ref_rank1):
+    f = fixture_rank1
+    print("case: reference data_bin rank 1", flush=True)
+    with gpu.Problem(f["pix"], f["weights"], f["block_sizes"], f["lam"], f["invtt"]) as pb:
+        lind, idx, tr = pb.local_shape()
+        assert np.array_equal(lind, ref_rank1["lindices"])
+        assert np.array_equal(idx, ref_rank1["indices"])
+        assert tr == int(ref_rank1["trash"])
+        assert np.array_equal(pb.matvec(ref_rank1["x"]), ref_rank1["Px"])
+        assert np.array_equal(pb.trmatvec(ref_rank1["y"]), ref_rank1["Pty"])
+
+
+def test_local_shape_and_products_match_reference_flags(gpu, ref_flags):
+    pix, w = _flagged()
+    print("case: flagged pointing (trash rows)", flush=True)
+    with gpu.Problem(pix, w, np.array([pix.size // 3], np.int32), 4, np.ones(4)) as pb:
+        lind, idx, tr = pb.local_shape()
+        assert tr == 1 and np.array_equal(lind, ref_flags["lindices"]) and np.array_equal(idx, ref_flags["indices"])
+        assert np.array_equal(pb.matvec(ref_flags["x"]), ref_flags["Px"])
+        assert np.array_equal(pb.trmatvec(ref_flags["y"]), ref_flags["Pty"])
+
+
+@pytest.mark.parametrize("w0", [1.0, 0.5])
+def test_products_bitexact_vs_oracle_c1(gpu, w0):
+    wl = W.config_c1()
+    w = wl.weights.copy()
+    w[0::3] = w0  # non-unit I weight exercises the general sample-major/ELL path
+    print(f"case: C1 pointing, I weight {w0}", flush=True)
+    lind, loc, tr = O.local_shape(wl.pix)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(lind.size)
+    y = rng.standard_normal(wl.m)
+    with gpu.Problem(wl.pix, w, wl.block_sizes, wl.lam, wl.invtt) as pb:
+        assert np.array_equal(pb.matvec(x), O.matvec(loc, w, tr, 3, x))
+        assert np.array_equal(pb.trmatvec(y), O.trmatvec(loc, w, tr, 3, lind.size, y))
+
+
+# ----------------------------------------------------------------------------
+# N^-1: mean removal + banded Toeplitz (exact closed form = oracle direct mode)
+# ----------------------------------------------------------------------------
+NINV_CASES = [
+    dict(name="lambda=1 (diagonal), tail", blocks=[3000, 1000], lam=1, tail=17),
+    dict(name="lambda=16, short block, tail", blocks=[5000, 10, 4000], lam=16, tail=9),
+    dict(name="lambda=100 (odd band)", blocks=[20000, 7001], lam=100, tail=0),
+    dict(name="lambda=1024", blocks=[65536, 30000], lam=1024, tail=0),
+    dict(name="lambda=4096", blocks=[40000], lam=4096, tail=0),
+    dict(name="lambda=8192, 2^17 blocks", blocks=[131072, 131072], lam=8192, tail=0),
+    dict(name="lambda=8192, block shorter than 2 lambda", blocks=[12000, 70000], lam=8192, tail=0),
+]
+
+
+@pytest.mark.parametrize("case", NINV_CASES, ids=[c["name"] for c in NINV_CASES])
+def test_ninv_matches_closed_form(gpu, case):
+    print(f"case: {case['name']}", flush=True)
+    rng = np.random.default_rng(case["lam"])
+    blocks = np.array(case["blocks"], np.int32)
+    lam = case["lam"]
+    m = int(blocks.sum()) + case["tail"]
+    # realistic bands: the windowed inverse 1/f noise of each block length
+    invtt = np.concatenate([W.invtt_block(max(int(n), 2 * lam), lam, 1.0, 0.2, 1.5) for n in blocks])
+    invtt[lam:] *= 1.3  # blocks carry different filters
+    pix = np.repeat(3 * (np.arange(m) % 97), 3).astype(np.int32) + np.tile(np.arange(3, dtype=np.int32), m)
+    w = np.tile([1.0, 0.3, -0.2], m)
+    v = rng.standard_normal(m) + 0.5
+    with gpu.Problem(pix, w, blocks, lam, invtt) as pb:
+        out = pb.apply_weights(v)
+    with O.ninv_mode(O.NINV_DIRECT):
+        ref = O.apply_weights(v, blocks, lam, invtt)
+    assert _rel(out, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("mode", ["symmetric", "none"])
+@pytest.mark.parametrize("case", [NINV_CASES[1], NINV_CASES[3], NINV_CASES[5], NINV_CASES[6]],
+                         ids=[NINV_CASES[i]["name"] for i in (1, 3, 5, 6)])
+def test_ninv_offset_modes_match_closed_form(gpu, case, mode):
+    print(f"case: {case['name']}, offset {mode}", flush=True)
+    gmode = gpu.MGPU_OFFSET_SYMMETRIC if mode == "symmetric" else gpu.MGPU_OFFSET_NONE
+    omode = O.OFFSET_SYMMETRIC if mode == "symmetric" else O.OFFSET_NONE
+    rng = np.random.default_rng(case["lam"] + 1)
+    blocks = np.array(case["blocks"], np.int32)
+    lam = case["lam"]
+    m = int(blocks.sum()) + case["tail"]
+    invtt = np.concatenate([W.invtt_block(max(int(n), 2 * lam), lam, 1.0, 0.3, 1.2) for n in blocks])
+    pix = np.repeat(3 * (np.arange(m) % 97), 3).astype(np.int32) + np.tile(np.arange(3, dtype=np.int32), m)
+    w = np.tile([1.0, 0.3, -0.2], m)
+    v = rng.standard_normal(m) + 0.5
+    with gpu.Problem(pix, w, blocks, lam, invtt, opts=gpu.options(offset_mode=gmode)) as pb:
+        out = pb.apply_weights(v)
+    with O.ninv_mode(O.NINV_DIRECT), O.offset_mode(omode):
+        ref = O.apply_weights(v, blocks, lam, invtt)
+    assert _rel(out, ref) <= 1e-12
+
+
+def test_symmetric_offset_pcg_iterates_and_convergence(gpu):
+    wl = W.make_workload("c2-shape", 8, 1 << 18, 512, 512, 8192, 2, block_len=1 << 17,
+                         fknee_range=(0.05, 0.5), alpha_range=(1.0, 2.0))
+    print("case: 8 x 2^18 on 512^2, lambda 8192, 1/f alpha 1-2, symmetric offsets", flush=True)
+    opts = gpu.options(offset_mode=gpu.MGPU_OFFSET_SYMMETRIC)
+    with O.ninv_mode(O.NINV_REAL_SPECTRUM), O.offset_mode(O.OFFSET_SYMMETRIC):
+        r = O.gls_solve(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-30,
+                        maxiter=20, stop_after=5)
+    with gpu.Problem(wl.pix, wl.weights, wl.block_sizes, wl.lam, wl.invtt, opts=opts) as pb:
+        sig, _, _ = pb.precond_build(wl.signal)
+        x5, _, _ = pb.pcg(sig, wl.noise, tol=1e-30, maxiter=20, stop_after=5)
+    assert _rel(x5, r.x) <= 1e-11
+    g = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                    maxiter=3000, opts=opts)
+    print(f"symmetric offsets: converged={g.report['converged']} in {g.report['iterations']} iterations",
+          flush=True)
+    assert g.report["converged"]
+
+
+def test_ninv_reference_fixture(gpu, fixture_rank1):
+    f = fixture_rank1
+    v = np.random.default_rng(2).standard_normal(f["signal"].size)
+    with gpu.Problem(f["pix"], f["weights"], f["block_sizes"], f["lam"], f["invtt"]) as pb:
+        out = pb.apply_weights(v)
+    with O.ninv_mode(O.NINV_DIRECT):
+        exact = O.apply_weights(v, f["block_sizes"], f["lam"], f["invtt"])
+    quirk = O.apply_weights(v, f["block_sizes"], f["lam"], f["invtt"])
+    print(f"N^-1 on reference data: GPU vs exact {_rel(out, exact):.2e}, reference FFT path vs exact "
+          f"{_rel(quirk, exact):.2e}", flush=True)
+    assert _rel(out, exact) <= 1e-12
+
+
+# ----------------------------------------------------------------------------
+# preconditioner: cond, hits and the cut set bit-exact
+# ----------------------------------------------------------------------------
+def _cut_workload():
+    wl = W.make_workload("cut", 2, 65536, 64, 64, 1024, 3)
+    pix = wl.pix.copy()
+    for t, p in zip((5, 5000, 70000, 100000), (9000, 9001, 9002, 9003)):
+        pix[3 * t:3 * t + 3] = [3 * p, 3 * p + 1, 3 * p + 2]
+    return wl, pix
+
+
+def test_precond_bitexact_with_cut(gpu):
+    wl, pix = _cut_workload()
+    print("case: forced single-hit pixels (cut)", flush=True)
+    with O.ninv_mode(O.NINV_REAL_SPECTRUM):
+        r = O.gls_solve(pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                        maxiter=2)
+    with gpu.Problem(pix, wl.weights, wl.block_sizes, wl.lam, wl.invtt) as pb:
+        sig, cond, hits = pb.precond_build(wl.signal)
+        lind, idx, tr = pb.local_shape()
+    assert r.n_cut >= 4
+    assert np.array_equal(lind, r.lindices) and np.array_equal(idx, r.pix_local) and tr == r.trash_pix
+    assert np.array_equal(hits, r.hits)
+    assert np.array_equal(cond, r.cond)
+    assert np.array_equal(sig, r.signal)
+
+
+def test_precond_reference_fixture(gpu, fixture_rank1):
+    f = fixture_rank1
+    r = O.gls_solve(f["pix"], f["weights"], f["signal"], f["noise"], f["block_sizes"], f["lam"], f["invtt"],
+                    tol=1e-8, maxiter=2)
+    with gpu.Problem(f["pix"], f["weights"], f["block_sizes"], f["lam"], f["invtt"]) as pb:
+        sig, cond, hits = pb.precond_build(f["signal"])
+        lind, idx, tr = pb.local_shape()
+    print(f"reference data: {r.n_cut} of {r.n_cut + cond.size} pixels cut", flush=True)
+    assert np.array_equal(lind, r.lindices) and np.array_equal(idx, r.pix_local)
+    assert np.array_equal(cond, r.cond) and np.array_equal(hits, r.hits) and np.array_equal(sig, r.signal)
+
+
+# ----------------------------------------------------------------------------
+# PCG
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("k", [1, 2, 5, 10])
+def test_pcg_iterates(gpu, k):
+    wl, pix = _cut_workload()
+    print(f"case: x_{k}", flush=True)
+    with O.ninv_mode(O.NINV_REAL_SPECTRUM):
+        r = O.gls_solve(pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-30,
+                        maxiter=50, stop_after=k)
+    with gpu.Problem(pix, wl.weights, wl.block_sizes, wl.lam, wl.invtt) as pb:
+        sig, _, _ = pb.precond_build(wl.signal)
+        x, hist, rep = pb.pcg(sig, wl.noise, tol=1e-30, maxiter=50, stop_after=k)
+    assert rep.iterations == r.iterations == k
+    assert _rel(x, r.x) <= 1e-12 * (1 + k)
+    np.testing.assert_allclose(hist, r.res_hist, rtol=1e-10)
+
+
+def test_pcg_full_solve_and_determinism(gpu):
+    wl = W.make_workload("c1-shape", 2, 65536, 64, 64, 1024, 3)
+    print("case: 2 x 65536, lambda 1024, tol 1e-8", flush=True)
+    with O.ninv_mode(O.NINV_REAL_SPECTRUM):
+        r = O.gls_solve(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                        maxiter=300)
+    res = []
+    for _ in range(2):
+        g = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                        maxiter=300)
+        res.append(g)
+    g = res[0]
+    assert np.array_equal(res[0].x, res[1].x), "two runs differ"
+    assert g.report["converged"] and r.converged
+    assert abs(g.report["iterations"] - r.iterations) <= 1
+    print(f"iterations GPU {g.report['iterations']} oracle {r.iterations}; map rel diff {_rel(g.x, r.x):.2e}",
+          flush=True)
+    assert _rel(g.x, r.x) <= 1e-7
+
+
+def test_dropin_side_effects_match_oracle(gpu, fixture_rank1, tmp_path):
+    f = fixture_rank1
+    with O.ninv_mode(O.NINV_REAL_SPECTRUM):
+        r = O.gls_solve(f["pix"], f["weights"], f["signal"], f["noise"], f["block_sizes"], f["lam"], f["invtt"],
+                        tol=1e-8, maxiter=100)
+    g = gpu.pcg_gls(f["pix"], f["weights"], f["signal"], f["noise"], f["block_sizes"], f["lam"], f["invtt"],
+                    tol=1e-8, maxiter=100, outpath=str(tmp_path), ref="fixture")
+    assert np.array_equal(g.pix_local, r.pix_local)
+    assert np.array_equal(g.weights, r.weights)
+    assert np.array_equal(g.signal, r.signal)
+    assert np.array_equal(g.lindices, r.lindices)
+    assert np.array_equal(g.cond, r.cond) and np.array_equal(g.hits, r.hits)
+    assert g.report["converged"] and abs(g.report["iterations"] - r.iterations) <= 1
+    assert g.res_hist is not None and g.res_hist.size == g.report["iterations"] + 1
+    print(f"reference data: {g.report['iterations']} iterations, map rel diff {_rel(g.x, r.x):.2e}", flush=True)
+    assert _rel(g.x, r.x) <= 1e-6
+
+
+def test_mixed_precision_map_error(gpu):
+    wl = W.config_c1()
+    print("case: C1 mixed precision", flush=True)
+    with O.ninv_mode(O.NINV_REAL_SPECTRUM):
+        r = O.gls_solve(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                        maxiter=300)
+    g = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                    maxiter=300, opts=gpu.options(precision=gpu.MGPU_MIXED, refresh_every=10))
+    err = _rel(g.x, r.x)
+    print(f"mixed: {g.report['iterations']} iterations (FP64 oracle {r.iterations}), map error vs FP64 "
+          f"{err:.2e}, true residual {g.report['true_res_rel']:.2e}, refreshes {g.report['refreshes']}",
+          flush=True)
+    assert g.report["converged"]
+    assert err <= 1e-6
+
+
+# ----------------------------------------------------------------------------
+# error handling (no kernel ever faults: inputs are rejected on the host)
+# ----------------------------------------------------------------------------
+def test_invalid_inputs_are_rejected(gpu):
+    wl = W.config_small(ndet=1, nsamp=256, nx=4, ny=4, lam=8)
+    bad = wl.pix.copy()
+    bad[4] += 1  # breaks the 3p+k encoding
+    with pytest.raises(gpu.GPUError, match="3p\\+k"):
+        gpu.Problem(bad, wl.weights, wl.block_sizes, wl.lam, wl.invtt)
+    with pytest.raises(gpu.GPUError, match="exceeds m"):
+        gpu.Problem(wl.pix, wl.weights, np.array([300], np.int32), wl.lam, np.ones(8))
+    with pytest.raises(gpu.GPUError, match="nnz"):
+        gpu.Problem(wl.pix[: 2 * 200], wl.weights[: 2 * 200], np.array([200], np.int32), wl.lam, wl.invtt, nnz=2)
+    with gpu.Problem(wl.pix, wl.weights, wl.block_sizes, wl.lam, wl.invtt) as pb:
+        with pytest.raises(gpu.GPUError, match="before precond_build"):
+            pb.precond_apply(np.zeros(pb.map_size()))
