@@ -173,15 +173,19 @@ struct Pointing {
     bool unit_w0 = true;   // I weight identically 1
     int64_t nh = 0;        // sliced-ELL length (hits + padding)
     int *d_q = nullptr;    // [m]
-    double *d_w0 = nullptr, *d_w1 = nullptr, *d_w2 = nullptr;   // [m]
-    float *d_w0f = nullptr, *d_w1f = nullptr, *d_w2f = nullptr; // [m] (mixed)
+    double *d_w0 = nullptr;   // [m] I weight (only when not identically 1)
+    double2 *d_w12 = nullptr; // [m] (Q, U) weights interleaved: one 16-byte load per sample
+    float *d_w0f = nullptr;   // FP32 copies (mixed precision)
+    float2 *d_w12f = nullptr;
     int64_t nvalid = 0;    // samples on valid pixels (CSR length)
     int *d_off = nullptr;  // [npix+1] CSR offsets (hits of pixel q: off[q]..off[q+1])
     int *d_cst = nullptr;  // [nvalid] CSR sample indices, increasing within a pixel
     int64_t *d_ell_base = nullptr; // [nslice+1] sliced-ELL slice offsets (elements)
     int *d_ct = nullptr;   // [nh] sliced-ELL sample index (-1 = padding)
-    double *d_cw0 = nullptr, *d_cw1 = nullptr, *d_cw2 = nullptr;
-    float *d_cw0f = nullptr, *d_cw1f = nullptr, *d_cw2f = nullptr;
+    double *d_cw0 = nullptr;    // [nh] I weight in ELL order (only when not identically 1)
+    double2 *d_cw12 = nullptr;  // [nh] (Q, U) weights in ELL order
+    float *d_cw0f = nullptr;
+    float2 *d_cw12f = nullptr;
     int *d_pixg = nullptr; // [npix0] global pixel number p (valid pixels, increasing)
 };
 

@@ -8,11 +8,13 @@
 //   TrMatVecProd   mapmat.c:819-851: x[i] += w y[t] in increasing t.
 //
 // Layout choices for HBM:
-//   * P runs sample-major (one lane per sample, 4 samples per lane, 1024 per
-//     workgroup): q/w loads and the y store are fully coalesced; the map gather
-//     x[3q+k] is L2-resident (map << TOD).  The per-block sums needed by the
-//     mean removal of N^-1 are fused into the epilogue (one partial per
-//     (chunk, block) pair, fixed-order tree, deterministic).
+//   * P runs sample-major (one lane per sample, 4 samples per lane strided by
+//     the workgroup size, so every load instruction reads a contiguous span):
+//     q (4 B), the interleaved (Q, U) weights (one 16-byte load) and the y
+//     store are fully coalesced; the map gather x[3q+k] is L2-resident
+//     (map << TOD).  The per-block sums needed by the mean removal of N^-1 are
+//     fused into the epilogue (one partial per (chunk, block) pair, fixed-order
+//     tree, deterministic).
 //   * P^T runs pixel-major over a sliced-ELL layout with slice height 64 (one
 //     wavefront): slice s stores hit i of lane l at base_s + 64 i + l, hits of a
 //     pixel in increasing t.  Each lane accumulates its pixel sequentially, so
@@ -20,6 +22,9 @@
 //     deterministic, and no atomics are used.  All loads of the ELL arrays are
 //     coalesced; the gather of y[t] is coalesced for scans whose neighbouring
 //     pixels are neighbours in time (rasters).
+//   * The I weight is stored only when some sample has w0 != 1 (the usual
+//     pointing has w0 = 1 everywhere; x0 * 1 == x0 exactly, so skipping it is
+//     bit-exact).
 //
 // Every kernel is compiled with -ffp-contract=off: products and sums are
 // separately rounded like the reference's C build.
@@ -29,42 +34,61 @@
 
 namespace mgpu {
 
-constexpr int kSlice = 64;   // ELL slice height (one wavefront)
-constexpr int kPBlock = 256; // P workgroup size (kChunk / 4 samples per lane)
+constexpr int kSlice = 64;          // ELL slice height (one wavefront)
+constexpr int kPBlock = 256;        // P workgroup size (kChunk / 4 samples per lane)
+constexpr int kClassifyGrid = 2048; // grid-stride validation: 8 workgroups per CU
 
 // ---------------------------------------------------------------------------
 // validation + pixel extraction
 // ---------------------------------------------------------------------------
-// status bits: 1 = malformed row, 2 = trash row present, 4 = I weight != 1
-__global__ void k_classify(const int *__restrict__ pix, const double *__restrict__ w, int64_t m,
-                           int *__restrict__ p_of_t, int *__restrict__ status,
-                           int *__restrict__ pmin, int *__restrict__ pmax) {
-    int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+// status bits: 1 = malformed row, 2 = trash row present, 4 = I weight != 1.
+// Grid-stride over the samples, one reduction per workgroup: the three
+// device-wide atomics are issued once per workgroup instead of once per wave.
+__global__ __launch_bounds__(256) void k_classify(const int *__restrict__ pix, const double *__restrict__ w,
+                                                  int64_t m, int *__restrict__ p_of_t, int *__restrict__ status,
+                                                  int *__restrict__ pmin, int *__restrict__ pmax) {
+    __shared__ int slo[4], shi[4], sst[4];
     int lo = INT32_MAX, hi = INT32_MIN, st = 0;
-    if (t < m) {
-        int a = pix[3 * t], b = pix[3 * t + 1], c = pix[3 * t + 2];
+    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < m; t += stride) {
+        const int a = pix[3 * t], b = pix[3 * t + 1], c = pix[3 * t + 2];
         int p = -1;
         if (a >= 0) {
-            if (a % 3 != 0 || b != a + 1 || c != a + 2) st |= 1;
-            else p = a / 3;
+            if (a % 3 != 0 || b != a + 1 || c != a + 2)
+                st |= 1;
+            else
+                p = a / 3;
         } else {
-            if (a != -3 || b != -2 || c != -1) st |= 1;
-            else st |= 2;
+            if (a != -3 || b != -2 || c != -1)
+                st |= 1;
+            else
+                st |= 2;
         }
         if (w[3 * t] != 1.0) st |= 4;
         p_of_t[t] = p;
         if (p >= 0) {
-            lo = p;
-            hi = p;
+            lo = min(lo, p);
+            hi = max(hi, p);
         }
     }
-    // wavefront reduce, then one atomic per wave
     for (int o = 32; o > 0; o >>= 1) {
         lo = min(lo, __shfl_xor(lo, o));
         hi = max(hi, __shfl_xor(hi, o));
         st |= __shfl_xor(st, o);
     }
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        slo[wave] = lo;
+        shi[wave] = hi;
+        sst[wave] = st;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int) (blockDim.x >> 6); i++) {
+            lo = min(lo, slo[i]);
+            hi = max(hi, shi[i]);
+            st |= sst[i];
+        }
         if (lo != INT32_MAX) {
             atomicMin(pmin, lo);
             atomicMax(pmax, hi);
@@ -73,8 +97,7 @@ __global__ void k_classify(const int *__restrict__ pix, const double *__restrict
     }
 }
 
-__global__ void k_mark(const int *__restrict__ p_of_t, int64_t m, int pmin,
-                       unsigned *__restrict__ flags) {
+__global__ void k_mark(const int *__restrict__ p_of_t, int64_t m, int pmin, unsigned *__restrict__ flags) {
     int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t < m) {
         int p = p_of_t[t];
@@ -82,10 +105,9 @@ __global__ void k_mark(const int *__restrict__ p_of_t, int64_t m, int pmin,
     }
 }
 
-__global__ void k_assign(const int *__restrict__ p_of_t, int64_t m, int pmin,
-                         const unsigned *__restrict__ rank, const double *__restrict__ w,
-                         int *__restrict__ q, double *__restrict__ w0, double *__restrict__ w1,
-                         double *__restrict__ w2, unsigned *__restrict__ key, int *__restrict__ tval,
+__global__ void k_assign(const int *__restrict__ p_of_t, int64_t m, int pmin, const unsigned *__restrict__ rank,
+                         const double *__restrict__ w, int *__restrict__ q, double *__restrict__ w0,
+                         double2 *__restrict__ w12, unsigned *__restrict__ key, int *__restrict__ tval,
                          unsigned trash_key) {
     int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t < m) {
@@ -93,22 +115,20 @@ __global__ void k_assign(const int *__restrict__ p_of_t, int64_t m, int pmin,
         int qq = p >= 0 ? (int) rank[p - pmin] : -1;
         q[t] = qq;
         if (w0) w0[t] = w[3 * t];
-        w1[t] = w[3 * t + 1];
-        w2[t] = w[3 * t + 2];
+        w12[t] = make_double2(w[3 * t + 1], w[3 * t + 2]);
         key[t] = qq >= 0 ? (unsigned) qq : trash_key;
         tval[t] = (int) t;
     }
 }
 
-__global__ void k_pixg(const unsigned *__restrict__ flags, const unsigned *__restrict__ rank,
-                       int64_t span, int pmin, int *__restrict__ pixg) {
+__global__ void k_pixg(const unsigned *__restrict__ flags, const unsigned *__restrict__ rank, int64_t span, int pmin,
+                       int *__restrict__ pixg) {
     int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i < span && flags[i]) pixg[rank[i]] = pmin + (int) i;
 }
 
 // CSR offsets from the sorted keys: off[k] = first position of key k
-__global__ void k_offsets(const unsigned *__restrict__ skey, int64_t nh, int npix,
-                          int *__restrict__ off) {
+__global__ void k_offsets(const unsigned *__restrict__ skey, int64_t nh, int npix, int *__restrict__ off) {
     int64_t h = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (h < nh) {
         unsigned k = skey[h];
@@ -118,8 +138,7 @@ __global__ void k_offsets(const unsigned *__restrict__ skey, int64_t nh, int npi
 }
 
 // slice widths: width[s] = max hits over the 64 pixels of slice s
-__global__ void k_slice_width(const int *__restrict__ off, int npix, int nslice,
-                              int64_t *__restrict__ width) {
+__global__ void k_slice_width(const int *__restrict__ off, int npix, int nslice, int64_t *__restrict__ width) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nslice) {
         int wmax = 0;
@@ -133,11 +152,9 @@ __global__ void k_slice_width(const int *__restrict__ off, int npix, int nslice,
 
 // Scatter the CSR into sliced-ELL (pixel q = 64 s + lane; hit i at base_s + 64 i + lane)
 template <bool W0>
-__global__ void k_fill_ell(const int *__restrict__ off, const int *__restrict__ st,
-                           const int64_t *__restrict__ base, int npix, int nslice,
-                           const double *__restrict__ w0, const double *__restrict__ w1,
-                           const double *__restrict__ w2, int *__restrict__ ct, double *__restrict__ cw0,
-                           double *__restrict__ cw1, double *__restrict__ cw2) {
+__global__ void k_fill_ell(const int *__restrict__ off, const int *__restrict__ st, const int64_t *__restrict__ base,
+                           int npix, int nslice, const double *__restrict__ w0, const double2 *__restrict__ w12,
+                           int *__restrict__ ct, double *__restrict__ cw0, double2 *__restrict__ cw12) {
     int qq = blockIdx.x * blockDim.x + threadIdx.x;
     int s = qq / kSlice, lane = qq % kSlice;
     if (s >= nslice) return;
@@ -154,13 +171,11 @@ __global__ void k_fill_ell(const int *__restrict__ off, const int *__restrict__ 
             int t = st[h];
             ct[dst] = t;
             if (W0) cw0[dst] = w0[t];
-            cw1[dst] = w1[t];
-            cw2[dst] = w2[t];
+            cw12[dst] = w12[t];
         } else {
             ct[dst] = -1;
             if (W0) cw0[dst] = 0.0;
-            cw1[dst] = 0.0;
-            cw2[dst] = 0.0;
+            cw12[dst] = make_double2(0.0, 0.0);
         }
     }
 }
@@ -171,8 +186,7 @@ __global__ void k_scale64(int64_t *a, int64_t n, int64_t f) {
 }
 
 // number of valid samples: keys < npix are sorted before the trash keys
-__global__ void k_count_valid(const unsigned *__restrict__ skey, int64_t m, unsigned npix,
-                              int64_t *__restrict__ cnt) {
+__global__ void k_count_valid(const unsigned *__restrict__ skey, int64_t m, unsigned npix, int64_t *__restrict__ cnt) {
     int64_t h = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (h < m && skey[h] < npix && (h + 1 == m || skey[h + 1] >= npix)) *cnt = h + 1;
 }
@@ -180,6 +194,14 @@ __global__ void k_count_valid(const unsigned *__restrict__ skey, int64_t m, unsi
 __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
     int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float) a[i];
+}
+
+__global__ void k_to_f32x2(const double2 *__restrict__ a, float2 *__restrict__ b, int64_t n) {
+    int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double2 v = a[i];
+        b[i] = make_float2((float) v.x, (float) v.y);
+    }
 }
 
 // compaction after the pixel cut
@@ -227,18 +249,27 @@ __global__ void k_remap_q(int *__restrict__ q, int64_t m, const unsigned *__rest
 // ---------------------------------------------------------------------------
 // build
 // ---------------------------------------------------------------------------
+static void release_fp32(Problem &pb) {
+    Pointing &P = pb.pt;
+    for (void *p : {(void *) P.d_cw12f, (void *) P.d_cw0f, (void *) P.d_w12f, (void *) P.d_w0f})
+        if (p) pb.arena.release(p);
+    P.d_cw12f = nullptr;
+    P.d_cw0f = nullptr;
+    P.d_w12f = nullptr;
+    P.d_w0f = nullptr;
+}
+
 static int build_ell(Problem &pb) {
     Pointing &P = pb.pt;
     hipStream_t s = pb.stream;
     const int npix = P.npix;
     const int nslice = (npix + kSlice - 1) / kSlice;
-    if (P.d_ell_base) pb.arena.release(P.d_ell_base);
-    if (P.d_ct) pb.arena.release(P.d_ct);
-    if (P.d_cw0) pb.arena.release(P.d_cw0);
-    if (P.d_cw1) pb.arena.release(P.d_cw1);
-    if (P.d_cw2) pb.arena.release(P.d_cw2);
+    for (void *p : {(void *) P.d_ell_base, (void *) P.d_ct, (void *) P.d_cw0, (void *) P.d_cw12})
+        if (p) pb.arena.release(p);
+    P.d_ell_base = nullptr;
     P.d_ct = nullptr;
-    P.d_cw0 = P.d_cw1 = P.d_cw2 = nullptr;
+    P.d_cw0 = nullptr;
+    P.d_cw12 = nullptr;
     int64_t *d_w, *d_base;
     MGPU_RET(pb.arena.alloc(&d_w, (size_t) nslice + 1));
     MGPU_RET(pb.arena.alloc(&d_base, (size_t) nslice + 1));
@@ -259,32 +290,20 @@ static int build_ell(Problem &pb) {
     pb.arena.release(d_w);
     MGPU_RET(pb.arena.alloc(&P.d_ct, (size_t) nell));
     if (!P.unit_w0) MGPU_RET(pb.arena.alloc(&P.d_cw0, (size_t) nell));
-    MGPU_RET(pb.arena.alloc(&P.d_cw1, (size_t) nell));
-    MGPU_RET(pb.arena.alloc(&P.d_cw2, (size_t) nell));
+    MGPU_RET(pb.arena.alloc(&P.d_cw12, (size_t) nell));
     P.nh = nell;
     P.d_ell_base = d_base;
     const int nthr = nslice * kSlice;
     if (nthr > 0) {
         if (P.unit_w0)
             k_fill_ell<false><<<grid_for(nthr, 256), 256, 0, s>>>(P.d_off, P.d_cst, d_base, npix, nslice, nullptr,
-                                                                  P.d_w1, P.d_w2, P.d_ct, nullptr, P.d_cw1,
-                                                                  P.d_cw2);
+                                                                  P.d_w12, P.d_ct, nullptr, P.d_cw12);
         else
-            k_fill_ell<true><<<grid_for(nthr, 256), 256, 0, s>>>(P.d_off, P.d_cst, d_base, npix, nslice, P.d_w0, P.d_w1,
-                                                                 P.d_w2, P.d_ct, P.d_cw0, P.d_cw1, P.d_cw2);
+            k_fill_ell<true><<<grid_for(nthr, 256), 256, 0, s>>>(P.d_off, P.d_cst, d_base, npix, nslice, P.d_w0,
+                                                                 P.d_w12, P.d_ct, P.d_cw0, P.d_cw12);
         MGPU_LAUNCHED();
     }
-    // FP32 copies are stale now
-    if (P.d_cw1f) {
-        pb.arena.release(P.d_cw1f);
-        pb.arena.release(P.d_cw2f);
-        if (P.d_cw0f) pb.arena.release(P.d_cw0f);
-        pb.arena.release(P.d_w1f);
-        pb.arena.release(P.d_w2f);
-        if (P.d_w0f) pb.arena.release(P.d_w0f);
-        P.d_cw0f = P.d_cw1f = P.d_cw2f = nullptr;
-        P.d_w0f = P.d_w1f = P.d_w2f = nullptr;
-    }
+    release_fp32(pb);  // the FP32 copies are stale now
     return MGPU_OK;
 }
 
@@ -301,7 +320,8 @@ int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m
     MGPU_RET(pb.arena.alloc(&d_stat, 3));
     int init[3] = {0, INT32_MAX, INT32_MIN};
     MGPU_HIP(hipMemcpyAsync(d_stat, init, sizeof(init), hipMemcpyHostToDevice, s));
-    k_classify<<<grid_for(m, 256), 256, 0, s>>>(d_pix, d_wts, m, d_p, d_stat, d_stat + 1, d_stat + 2);
+    const unsigned gcl = (unsigned) std::min<int64_t>(kClassifyGrid, (m + 255) / 256);
+    k_classify<<<gcl, 256, 0, s>>>(d_pix, d_wts, m, d_p, d_stat, d_stat + 1, d_stat + 2);
     MGPU_LAUNCHED();
     int hst[3];
     MGPU_HIP(hipMemcpyAsync(hst, d_stat, sizeof(hst), hipMemcpyDeviceToHost, s));
@@ -332,8 +352,7 @@ int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m
 
     MGPU_RET(pb.arena.alloc(&P.d_q, (size_t) m));
     if (!P.unit_w0) MGPU_RET(pb.arena.alloc(&P.d_w0, (size_t) m));
-    MGPU_RET(pb.arena.alloc(&P.d_w1, (size_t) m));
-    MGPU_RET(pb.arena.alloc(&P.d_w2, (size_t) m));
+    MGPU_RET(pb.arena.alloc(&P.d_w12, (size_t) m));
     MGPU_RET(pb.arena.alloc(&P.d_pixg, (size_t) P.npix0));
     unsigned *d_key, *d_skey;
     int *d_t;
@@ -341,8 +360,8 @@ int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m
     MGPU_RET(pb.arena.alloc(&d_skey, (size_t) m));
     MGPU_RET(pb.arena.alloc(&d_t, (size_t) m));
     MGPU_RET(pb.arena.alloc(&P.d_cst, (size_t) m));
-    k_assign<<<grid_for(m, 256), 256, 0, s>>>(d_p, m, pmin, d_rank, d_wts, P.d_q, P.d_w0, P.d_w1, P.d_w2,
-                                              d_key, d_t, (unsigned) P.npix0);
+    k_assign<<<grid_for(m, 256), 256, 0, s>>>(d_p, m, pmin, d_rank, d_wts, P.d_q, P.d_w0, P.d_w12, d_key, d_t,
+                                              (unsigned) P.npix0);
     MGPU_LAUNCHED();
     k_pixg<<<grid_for(span, 256), 256, 0, s>>>(d_flags, d_rank, span, pmin, P.d_pixg);
     MGPU_LAUNCHED();
@@ -351,11 +370,10 @@ int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m
     int end_bit = 1;
     while (end_bit < 32 && ((unsigned) P.npix0 >> end_bit) != 0) end_bit++;
     tb = 0;
-    MGPU_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_key, d_skey, d_t, P.d_cst, (int) m, 0, end_bit,
-                                                s));
+    MGPU_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_key, d_skey, d_t, P.d_cst, (int) m, 0, end_bit, s));
     MGPU_RET(pb.cub_tmp.ensure(tb));
-    MGPU_HIP(hipcub::DeviceRadixSort::SortPairs(pb.cub_tmp.p, tb, d_key, d_skey, d_t, P.d_cst, (int) m, 0,
-                                                end_bit, s));
+    MGPU_HIP(hipcub::DeviceRadixSort::SortPairs(pb.cub_tmp.p, tb, d_key, d_skey, d_t, P.d_cst, (int) m, 0, end_bit,
+                                                s));
     int64_t *d_cnt;
     MGPU_RET(pb.arena.alloc(&d_cnt, 1));
     MGPU_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int64_t), s));
@@ -370,13 +388,9 @@ int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m
     MGPU_LAUNCHED();
     MGPU_RET(build_ell(pb));
     MGPU_HIP(hipStreamSynchronize(s));
-    pb.arena.release(d_cnt);
-    pb.arena.release(d_key);
-    pb.arena.release(d_skey);
-    pb.arena.release(d_t);
-    pb.arena.release(d_flags);
-    pb.arena.release(d_rank);
-    pb.arena.release(d_p);
+    for (void *p : {(void *) d_cnt, (void *) d_key, (void *) d_skey, (void *) d_t, (void *) d_flags, (void *) d_rank,
+                    (void *) d_p})
+        pb.arena.release(p);
     return MGPU_OK;
 }
 
@@ -407,8 +421,8 @@ int pointing_compact(Problem &pb, const unsigned *d_kept) {
     MGPU_RET(pb.arena.alloc(&d_pixg2, (size_t) nnew));
     MGPU_RET(pb.arena.alloc(&d_off2, (size_t) nnew + 1));
     MGPU_RET(pb.arena.alloc(&d_cst2, (size_t) nvalid_new));
-    k_compact_pixels<<<grid_for(npix + 1, 256), 256, 0, s>>>(d_kept, d_new, d_hscan, P.d_pixg, npix, d_pixg2,
-                                                             d_off2, (int) nnew);
+    k_compact_pixels<<<grid_for(npix + 1, 256), 256, 0, s>>>(d_kept, d_new, d_hscan, P.d_pixg, npix, d_pixg2, d_off2,
+                                                             (int) nnew);
     MGPU_LAUNCHED();
     if (npix > 0) {
         k_compact_hits<<<npix, 64, 0, s>>>(d_kept, d_new, P.d_off, P.d_cst, d_off2, npix, d_cst2);
@@ -427,29 +441,22 @@ int pointing_compact(Problem &pb, const unsigned *d_kept) {
     P.npix = (int) nnew;
     MGPU_RET(build_ell(pb));
     MGPU_HIP(hipStreamSynchronize(s));
-    pb.arena.release(d_keep);
-    pb.arena.release(d_new);
-    pb.arena.release(d_cnt);
-    pb.arena.release(d_hscan);
+    for (void *p : {(void *) d_keep, (void *) d_new, (void *) d_cnt, (void *) d_hscan}) pb.arena.release(p);
     return MGPU_OK;
 }
 
 int pointing_make_fp32(Problem &pb) {
     Pointing &P = pb.pt;
-    if (P.d_w1f) return MGPU_OK;
-    MGPU_RET(pb.arena.alloc(&P.d_w1f, (size_t) P.m));
-    MGPU_RET(pb.arena.alloc(&P.d_w2f, (size_t) P.m));
-    MGPU_RET(pb.arena.alloc(&P.d_cw1f, (size_t) P.nh));
-    MGPU_RET(pb.arena.alloc(&P.d_cw2f, (size_t) P.nh));
-    k_to_f32<<<grid_for(P.m, 256), 256, 0, pb.stream>>>(P.d_w1, P.d_w1f, P.m);
-    k_to_f32<<<grid_for(P.m, 256), 256, 0, pb.stream>>>(P.d_w2, P.d_w2f, P.m);
-    k_to_f32<<<grid_for(P.nh, 256), 256, 0, pb.stream>>>(P.d_cw1, P.d_cw1f, P.nh);
-    k_to_f32<<<grid_for(P.nh, 256), 256, 0, pb.stream>>>(P.d_cw2, P.d_cw2f, P.nh);
+    if (P.d_w12f) return MGPU_OK;
+    MGPU_RET(pb.arena.alloc(&P.d_w12f, (size_t) P.m));
+    MGPU_RET(pb.arena.alloc(&P.d_cw12f, (size_t) P.nh));
+    k_to_f32x2<<<grid_for(P.m, 256), 256, 0, pb.stream>>>(P.d_w12, P.d_w12f, P.m);
+    if (P.nh > 0) k_to_f32x2<<<grid_for(P.nh, 256), 256, 0, pb.stream>>>(P.d_cw12, P.d_cw12f, P.nh);
     if (!P.unit_w0) {
         MGPU_RET(pb.arena.alloc(&P.d_w0f, (size_t) P.m));
         MGPU_RET(pb.arena.alloc(&P.d_cw0f, (size_t) P.nh));
         k_to_f32<<<grid_for(P.m, 256), 256, 0, pb.stream>>>(P.d_w0, P.d_w0f, P.m);
-        k_to_f32<<<grid_for(P.nh, 256), 256, 0, pb.stream>>>(P.d_cw0, P.d_cw0f, P.nh);
+        if (P.nh > 0) k_to_f32<<<grid_for(P.nh, 256), 256, 0, pb.stream>>>(P.d_cw0, P.d_cw0f, P.nh);
     }
     MGPU_LAUNCHED();
     return MGPU_OK;
@@ -503,7 +510,7 @@ __device__ __forceinline__ double wg_sum_256(double v, double *red) {
     return s;  // valid in thread 0
 }
 
-// first block whose range contains t (start[nb] = covered); returns nb if t >= covered
+// block whose range contains t (start[nb] = covered); nb if t >= covered
 __device__ __forceinline__ int find_block(const int64_t *__restrict__ start, int nb, int64_t t) {
     int lo = 0, hi = nb;  // invariant: start[lo] <= t < start[hi] (if t < covered)
     if (t >= start[nb]) return nb;
@@ -517,31 +524,41 @@ __device__ __forceinline__ int find_block(const int64_t *__restrict__ start, int
     return lo;
 }
 
-template <typename TX, typename TW, bool W0, bool PARTIALS>
-__global__ __launch_bounds__(kPBlock) void k_matvec(
-        const int *__restrict__ q, const TW *__restrict__ w0, const TW *__restrict__ w1,
-        const TW *__restrict__ w2, const TX *__restrict__ x, TX *__restrict__ y, int64_t m,
-        const int64_t *__restrict__ start, int nb, double *__restrict__ partials) {
+template <typename TX, typename TW, typename TW2, bool W0, bool PARTIALS>
+__global__ __launch_bounds__(kPBlock) void k_matvec(const int *__restrict__ q, const TW *__restrict__ w0,
+                                                    const TW2 *__restrict__ w12, const TX *__restrict__ x,
+                                                    TX *__restrict__ y, int64_t m, const int64_t *__restrict__ start,
+                                                    int nb, double *__restrict__ partials) {
     __shared__ double red[kPBlock / 64];
+    constexpr int PER = kChunk / kPBlock;
     const int64_t c = blockIdx.x;
     const int64_t c0 = c * kChunk;
-    TX yv[kChunk / kPBlock];
+    int qq[PER];
+    TW2 wv[PER];
+    // issue every streaming load of the chunk before the map gathers
 #pragma unroll
-    for (int i = 0; i < kChunk / kPBlock; i++) {
+    for (int i = 0; i < PER; i++) {
+        const int64_t t = c0 + threadIdx.x + (int64_t) i * kPBlock;
+        qq[i] = -1;
+        if (t < m) {
+            qq[i] = q[t];
+            wv[i] = w12[t];
+        }
+    }
+    TX yv[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
         const int64_t t = c0 + threadIdx.x + (int64_t) i * kPBlock;
         TX v = (TX) 0;
-        if (t < m) {
-            const int qq = q[t];
-            if (qq >= 0) {
-                const TX x0 = x[3 * qq], x1 = x[3 * qq + 1], x2 = x[3 * qq + 2];
-                TX acc = (TX) 0;
-                acc += (W0 ? (TX) w0[t] * x0 : x0);
-                acc += (TX) w1[t] * x1;
-                acc += (TX) w2[t] * x2;
-                v = acc;
-            }
-            y[t] = v;
+        if (qq[i] >= 0) {
+            const TX x0 = x[3 * qq[i]], x1 = x[3 * qq[i] + 1], x2 = x[3 * qq[i] + 2];
+            TX acc = (TX) 0;
+            acc += (W0 ? (TX) w0[t] * x0 : x0);
+            acc += (TX) wv[i].x * x1;
+            acc += (TX) wv[i].y * x2;
+            v = acc;
         }
+        if (t < m) y[t] = v;
         yv[i] = v;
     }
     if (PARTIALS) {
@@ -551,7 +568,7 @@ __global__ __launch_bounds__(kPBlock) void k_matvec(
             const int64_t lo = start[b], hi = start[b + 1];
             double s = 0.0;
 #pragma unroll
-            for (int i = 0; i < kChunk / kPBlock; i++) {
+            for (int i = 0; i < PER; i++) {
                 const int64_t t = c0 + threadIdx.x + (int64_t) i * kPBlock;
                 if (t >= lo && t < hi) s += (double) yv[i];
             }
@@ -561,52 +578,39 @@ __global__ __launch_bounds__(kPBlock) void k_matvec(
     }
 }
 
-int launch_matvec(Problem &pb, const double *d_x, double *d_y, bool partials) {
+template <typename TX, typename TW, typename TW2>
+static int launch_matvec_t(Problem &pb, const TX *d_x, TX *d_y, bool partials, const TW *w0, const TW2 *w12) {
     const Pointing &P = pb.pt;
     const ToeplitzPlan &T = pb.tp;
     const unsigned g = grid_for(P.m, kChunk);
     partials = partials && T.nb > 0 && T.d_partials && T.lambda > 1 && T.offset_mode != MGPU_OFFSET_NONE;
+    const int *q = P.d_q;
+    const int64_t *st = T.d_start;
     if (P.unit_w0) {
         if (partials)
-            k_matvec<double, double, false, true><<<g, kPBlock, 0, pb.stream>>>(
-                    P.d_q, nullptr, P.d_w1, P.d_w2, d_x, d_y, P.m, T.d_start, T.nb, T.d_partials);
+            k_matvec<TX, TW, TW2, false, true>
+                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, T.nb, T.d_partials);
         else
-            k_matvec<double, double, false, false><<<g, kPBlock, 0, pb.stream>>>(
-                    P.d_q, nullptr, P.d_w1, P.d_w2, d_x, d_y, P.m, T.d_start, T.nb, nullptr);
+            k_matvec<TX, TW, TW2, false, false>
+                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, T.nb, nullptr);
     } else {
         if (partials)
-            k_matvec<double, double, true, true><<<g, kPBlock, 0, pb.stream>>>(
-                    P.d_q, P.d_w0, P.d_w1, P.d_w2, d_x, d_y, P.m, T.d_start, T.nb, T.d_partials);
+            k_matvec<TX, TW, TW2, true, true>
+                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, T.nb, T.d_partials);
         else
-            k_matvec<double, double, true, false><<<g, kPBlock, 0, pb.stream>>>(
-                    P.d_q, P.d_w0, P.d_w1, P.d_w2, d_x, d_y, P.m, T.d_start, T.nb, nullptr);
+            k_matvec<TX, TW, TW2, true, false><<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, T.nb,
+                                                                             nullptr);
     }
     MGPU_LAUNCHED();
     return MGPU_OK;
 }
 
+int launch_matvec(Problem &pb, const double *d_x, double *d_y, bool partials) {
+    return launch_matvec_t<double, double, double2>(pb, d_x, d_y, partials, pb.pt.d_w0, pb.pt.d_w12);
+}
+
 int launch_matvec_f32(Problem &pb, const float *d_x, float *d_y, bool partials) {
-    const Pointing &P = pb.pt;
-    const ToeplitzPlan &T = pb.tp;
-    const unsigned g = grid_for(P.m, kChunk);
-    partials = partials && T.nb > 0 && T.d_partials && T.lambda > 1 && T.offset_mode != MGPU_OFFSET_NONE;
-    if (P.unit_w0) {
-        if (partials)
-            k_matvec<float, float, false, true><<<g, kPBlock, 0, pb.stream>>>(
-                    P.d_q, nullptr, P.d_w1f, P.d_w2f, d_x, d_y, P.m, T.d_start, T.nb, T.d_partials);
-        else
-            k_matvec<float, float, false, false><<<g, kPBlock, 0, pb.stream>>>(
-                    P.d_q, nullptr, P.d_w1f, P.d_w2f, d_x, d_y, P.m, T.d_start, T.nb, nullptr);
-    } else {
-        if (partials)
-            k_matvec<float, float, true, true><<<g, kPBlock, 0, pb.stream>>>(
-                    P.d_q, P.d_w0f, P.d_w1f, P.d_w2f, d_x, d_y, P.m, T.d_start, T.nb, T.d_partials);
-        else
-            k_matvec<float, float, true, false><<<g, kPBlock, 0, pb.stream>>>(
-                    P.d_q, P.d_w0f, P.d_w1f, P.d_w2f, d_x, d_y, P.m, T.d_start, T.nb, nullptr);
-    }
-    MGPU_LAUNCHED();
-    return MGPU_OK;
+    return launch_matvec_t<float, float, float2>(pb, d_x, d_y, partials, pb.pt.d_w0f, pb.pt.d_w12f);
 }
 
 // ---------------------------------------------------------------------------
@@ -614,12 +618,12 @@ int launch_matvec_f32(Problem &pb, const float *d_x, float *d_y, bool partials) 
 // ---------------------------------------------------------------------------
 constexpr int kPtBlock = 256;
 
-template <typename TY, typename TW, bool W0, bool DOT>
-__global__ __launch_bounds__(kPtBlock) void k_trmatvec(
-        const int64_t *__restrict__ base, const int *__restrict__ ct, const TW *__restrict__ cw0,
-        const TW *__restrict__ cw1, const TW *__restrict__ cw2, const TY *__restrict__ y,
-        double *__restrict__ x, int npix, const double *__restrict__ hvec,
-        double *__restrict__ dotpart) {
+template <typename TY, typename TW, typename TW2, bool W0, bool DOT>
+__global__ __launch_bounds__(kPtBlock) void k_trmatvec(const int64_t *__restrict__ base, const int *__restrict__ ct,
+                                                       const TW *__restrict__ cw0, const TW2 *__restrict__ cw12,
+                                                       const TY *__restrict__ y, double *__restrict__ x, int npix,
+                                                       const double *__restrict__ hvec,
+                                                       double *__restrict__ dotpart) {
     __shared__ double red[kPtBlock / 64];
     const int qq = blockIdx.x * kPtBlock + threadIdx.x;
     const int s = qq / kSlice, lane = qq % kSlice;
@@ -631,10 +635,11 @@ __global__ __launch_bounds__(kPtBlock) void k_trmatvec(
             const int64_t e = b0 + i * kSlice + lane;
             const int t = ct[e];
             if (t < 0) break;  // padding: hits of this lane are exhausted
+            const TW2 w = cw12[e];
             const double yt = (double) y[t];
             a0 += (W0 ? (double) cw0[e] * yt : yt);
-            a1 += (double) cw1[e] * yt;
-            a2 += (double) cw2[e] * yt;
+            a1 += (double) w.x * yt;
+            a2 += (double) w.y * yt;
         }
         x[3 * qq] = a0;
         x[3 * qq + 1] = a1;
@@ -654,43 +659,41 @@ __global__ __launch_bounds__(kPtBlock) void k_trmatvec(
 
 int trmatvec_nparts(const Problem &pb) { return (int) grid_for(pb.pt.npix, kPtBlock); }
 
-template <typename TY, typename TW>
+template <typename TY, typename TW, typename TW2>
 static int launch_trmatvec_t(Problem &pb, const TY *d_y, double *d_x, const double *d_h, double *d_dot,
-                             const TW *cw0, const TW *cw1, const TW *cw2, int *nparts) {
+                             const TW *cw0, const TW2 *cw12, int *nparts) {
     const Pointing &P = pb.pt;
     const unsigned g = grid_for(P.npix, kPtBlock);
     if (nparts) *nparts = (int) g;
     if (g == 0) return MGPU_OK;
     const bool dot = d_h && d_dot;
+    const int64_t *b = P.d_ell_base;
     if (P.unit_w0) {
         if (dot)
-            k_trmatvec<TY, TW, false, true><<<g, kPtBlock, 0, pb.stream>>>(P.d_ell_base, P.d_ct, cw0, cw1, cw2,
-                                                                          d_y, d_x, P.npix, d_h, d_dot);
+            k_trmatvec<TY, TW, TW2, false, true>
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, P.npix, d_h, d_dot);
         else
-            k_trmatvec<TY, TW, false, false><<<g, kPtBlock, 0, pb.stream>>>(P.d_ell_base, P.d_ct, cw0, cw1, cw2,
-                                                                           d_y, d_x, P.npix, nullptr, nullptr);
+            k_trmatvec<TY, TW, TW2, false, false>
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, P.npix, nullptr, nullptr);
     } else {
         if (dot)
-            k_trmatvec<TY, TW, true, true><<<g, kPtBlock, 0, pb.stream>>>(P.d_ell_base, P.d_ct, cw0, cw1, cw2,
-                                                                         d_y, d_x, P.npix, d_h, d_dot);
+            k_trmatvec<TY, TW, TW2, true, true>
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, P.npix, d_h, d_dot);
         else
-            k_trmatvec<TY, TW, true, false><<<g, kPtBlock, 0, pb.stream>>>(P.d_ell_base, P.d_ct, cw0, cw1, cw2,
-                                                                          d_y, d_x, P.npix, nullptr, nullptr);
+            k_trmatvec<TY, TW, TW2, true, false>
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, P.npix, nullptr, nullptr);
     }
     MGPU_LAUNCHED();
     return MGPU_OK;
 }
 
-int launch_trmatvec(Problem &pb, const double *d_y, double *d_x, const double *d_h, double *d_dot,
-                    int *nparts) {
-    const Pointing &P = pb.pt;
-    return launch_trmatvec_t<double, double>(pb, d_y, d_x, d_h, d_dot, P.d_cw0, P.d_cw1, P.d_cw2, nparts);
+int launch_trmatvec(Problem &pb, const double *d_y, double *d_x, const double *d_h, double *d_dot, int *nparts) {
+    return launch_trmatvec_t<double, double, double2>(pb, d_y, d_x, d_h, d_dot, pb.pt.d_cw0, pb.pt.d_cw12, nparts);
 }
 
 int launch_trmatvec_f32(Problem &pb, const float *d_y, double *d_x, const double *d_h, double *d_dot,
                         int *nparts) {
-    const Pointing &P = pb.pt;
-    return launch_trmatvec_t<float, float>(pb, d_y, d_x, d_h, d_dot, P.d_cw0f, P.d_cw1f, P.d_cw2f, nparts);
+    return launch_trmatvec_t<float, float, float2>(pb, d_y, d_x, d_h, d_dot, pb.pt.d_cw0f, pb.pt.d_cw12f, nparts);
 }
 
 }  // namespace mgpu

@@ -369,8 +369,8 @@ __device__ __forceinline__ int dgetri_3(double *a, const int *ipiv) {
 // one lane per pixel: accumulate B over the ELL hits, then rcond / inverse
 template <bool W0>
 __global__ __launch_bounds__(256) void k_blocks(const int64_t *__restrict__ base, const int *__restrict__ ct,
-                                                const double *__restrict__ cw0, const double *__restrict__ cw1,
-                                                const double *__restrict__ cw2, int npix,
+                                                const double *__restrict__ cw0, const double2 *__restrict__ cw12,
+                                                int npix,
                                                 const int64_t *__restrict__ start, int nb,
                                                 const double *__restrict__ T0, double *__restrict__ blocks_out,
                                                 double *__restrict__ inv, double *__restrict__ cond,
@@ -388,7 +388,8 @@ __global__ __launch_bounds__(256) void k_blocks(const int64_t *__restrict__ base
         const int t = ct[e];
         if (t < 0) break;
         while (blk < nb && start[blk + 1] <= t) blk++;
-        const double w[3] = {W0 ? cw0[e] : 1.0, cw1[e], cw2[e]};
+        const double2 wqu = cw12[e];
+        const double w[3] = {W0 ? cw0[e] : 1.0, wqu.x, wqu.y};
         nh++;
         if (t < covered) {
             const double d = T0[blk];
@@ -460,11 +461,11 @@ int precond_build(Problem &pb, double *d_signal) {
     MGPU_RET(pb.arena.alloc(&d_hits, (size_t) npix));
     MGPU_RET(pb.arena.alloc(&d_kept, (size_t) npix + 1));
     if (P.unit_w0)
-        k_blocks<false><<<grid_for(npix, 256), 256, 0, s>>>(P.d_ell_base, P.d_ct, nullptr, P.d_cw1, P.d_cw2, npix,
+        k_blocks<false><<<grid_for(npix, 256), 256, 0, s>>>(P.d_ell_base, P.d_ct, nullptr, P.d_cw12, npix,
                                                             T.d_start, T.nb, T.d_T0, nullptr, d_inv, d_cond, d_hits,
                                                             d_kept);
     else
-        k_blocks<true><<<grid_for(npix, 256), 256, 0, s>>>(P.d_ell_base, P.d_ct, P.d_cw0, P.d_cw1, P.d_cw2, npix,
+        k_blocks<true><<<grid_for(npix, 256), 256, 0, s>>>(P.d_ell_base, P.d_ct, P.d_cw0, P.d_cw12, npix,
                                                            T.d_start, T.nb, T.d_T0, nullptr, d_inv, d_cond, d_hits,
                                                            d_kept);
     MGPU_LAUNCHED();
