@@ -47,7 +47,14 @@ typedef struct mgpu_options {
     int poll_every;     /* PCG: iterations launched between host convergence polls (default 8) */
     int profile;        /* 1: time every kernel class with HIP events on the handle's stream */
     int offset_mode;    /* per-block offset handling of N^-1 (MGPU_OFFSET_*), default REFERENCE */
-    int reserved[2];
+    int tile_variant;   /* Toeplitz tile kernel variant: 0 = autotuned (each variant of the tile
+                           length is timed on the first product, once per process, device,
+                           precision and tile length); k > 0 forces variant k (1 = the default
+                           shape) of the precision of the PCG iterations (FP32 under
+                           MGPU_MIXED); see mappraiser_gpu_tile_variants */
+    int graph;          /* 1: capture each block of PCG iterations once into a HIP graph and
+                           replay it (same kernels, bit-identical results; ignored when
+                           profile = 1) */
 } mgpu_options;
 
 /* Per-block offset handling in N^-1 (block b: C = I - 1 1^T / n_b, T = Toeplitz band):
@@ -84,7 +91,7 @@ typedef struct mgpu_report {
     double t_solve_ms;  /* device time: PCG iterations */
     double true_res_rel;/* ||b - A x||/||b - A x0|| recomputed in FP64 at exit (MGPU_MIXED), else -1 */
     int    refreshes;   /* FP64 residual recomputations performed (MGPU_MIXED) */
-    int    pad;
+    int    tile_variant;  /* tile kernel variant used by the solve's Toeplitz product (0: none) */
 } mgpu_report;
 
 /* ---- library ---- */
@@ -159,6 +166,48 @@ int mappraiser_gpu_solve_device(mgpu_problem *pb, double *d_signal, const double
  * reference counterpart (the reference prints MPI_Wtime splits,
  * pcg_true.c:27,98-99,178-179). */
 int mappraiser_gpu_kernel_times(mgpu_problem *pb, double *ms, long long *counts, int reset);
+
+/* Tile kernel variants of the Toeplitz product for a precision (MGPU_FP64 /
+ * MGPU_MIXED) and real tile length 2^tile_log2: *count = number of variants
+ * (0 if the length has no tile kernel); if desc != NULL, a text description of
+ * variant `variant` (1-based) is written (at most desc_len bytes, terminated).
+ * No reference counterpart (the reference's FFT is FFTW's, toeplitz.c:418-423). */
+int mappraiser_gpu_tile_variants(int precision, int tile_log2, int variant, int *count, char *desc, int desc_len);
+
+/* ---- batched solves ----
+ * One item is one independent GLS problem (a Monte Carlo noise realization, a
+ * detector subset, a serving request) with its own pointing, signal, noise,
+ * Toeplitz blocks (their sizes must sum to m) and lambda.  Items of equal
+ * lambda are solved together as one segmented problem: every kernel of a PCG
+ * iteration is launched once for all of them, each item keeps its own PCG
+ * scalars and stop test, and converged items are frozen while the others go
+ * on.  With the same tile length (opt->tile_log2) and tile variant, each
+ * item's map, iteration count and residuals are bit-identical to its solo
+ * solve with the same options.  Groups of different lambda run concurrently
+ * on their own streams.  No reference counterpart: the reference solves one
+ * problem per MLmap call (mappraiser.c:34-232). */
+typedef struct mgpu_batch_item {
+    /* inputs: host memory, or device memory when inputs_on_device != 0 */
+    int m;                          /* samples */
+    const int *pix;                 /* [3m] global indices 3p+k (0 <= p < grid_npix), flagged -3,-2,-1 */
+    const double *pixweights;       /* [3m] */
+    const double *signal;           /* [m] (read only: the pixel cut acts on an internal copy) */
+    const double *noise;            /* [m] */
+    int nb;                         /* Toeplitz blocks */
+    const int *local_blocks_sizes;  /* [nb], host memory, summing to m */
+    int lambda;
+    const double *invtt;            /* [nb*lambda] */
+    /* outputs (host memory; NULL pointers are skipped) */
+    double *x;                      /* [x_cap] map of the kept pixels, (I,Q,U) per pixel, pixel order */
+    int *pixels;                    /* [x_cap/3] global pixel p of each map entry */
+    double *cond;                   /* [x_cap/3] rcond of each kept pixel */
+    int *lhits;                     /* [x_cap/3] hits of each kept pixel */
+    int x_cap;                      /* capacity of x in doubles (>= 3 * pixels kept) */
+    mgpu_report rep;                /* per-item report (t_setup_ms/t_solve_ms: the item's group) */
+} mgpu_batch_item;
+
+int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int grid_npix, int inputs_on_device, double tol,
+                             int maxiter, const mgpu_options *opt);
 
 /* Drop-in for PCG_GLS_true(outpath, ref, &A, &Nm1, x, signal, noise, cond,
  * lhits, tol, maxiter, precond=0, Z_2lvl) at mappraiser.c:211 for one rank,

@@ -24,7 +24,6 @@ void set_error(const char *fmt, ...) {
 }
 
 Problem::~Problem() {
-    if (h_pinned) (void) hipHostFree(h_pinned);
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (stream) {
@@ -40,7 +39,7 @@ static int upload(Problem &pb, T **d, const T *h, size_t n) {
     return MGPU_OK;
 }
 
-static int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt) {
+int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt) {
     pb.reset(new Problem());
     pb->device = g_device;
     MGPU_HIP(hipSetDevice(g_device));
@@ -53,7 +52,7 @@ static int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt) {
     return MGPU_OK;
 }
 
-static int check_common(int m, int nnz, int nb, const int *bs, int lambda) {
+int check_common(int m, int nnz, int nb, const int *bs, int lambda) {
     MGPU_ARG(m > 0, "m=%d must be positive", m);
     MGPU_ARG(nnz == 3, "nnz=%d: only the I,Q,U pointing (nnz=3) is supported", nnz);
     MGPU_ARG(nb > 0 && bs, "nb_blocks=%d / local_blocks_sizes missing", nb);
@@ -283,6 +282,18 @@ int mappraiser_gpu_kernel_times(mgpu_problem *h, double *ms, long long *counts, 
             pb.prof.ms[i] = 0.0;
             pb.prof.cnt[i] = 0;
         }
+    }
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_tile_variants(int precision, int tile_log2, int variant, int *count, char *desc, int desc_len) {
+    MGPU_ARG(count, "NULL count");
+    MGPU_ARG(precision == MGPU_FP64 || precision == MGPU_MIXED, "unknown precision %d", precision);
+    const bool fp32 = precision == MGPU_MIXED;
+    *count = tile_variant_count(fp32, tile_log2);
+    if (desc && desc_len > 0) {
+        MGPU_ARG(variant >= 1 && variant <= *count, "variant %d out of range 1..%d", variant, *count);
+        std::snprintf(desc, (size_t) desc_len, "%s", tile_variant_desc(fp32, tile_log2, variant));
     }
     return MGPU_OK;
 }

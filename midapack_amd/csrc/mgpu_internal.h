@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/mappraiser_gpu.h"
@@ -154,6 +155,8 @@ struct ToeplitzPlan {
     double *d_mean = nullptr;        // nb (per block: [0] input mean, see d_mean_out)
     // MGPU_OFFSET_SYMMETRIC: mean of T (v - mean) per block, from the block-edge terms
     int offset_mode = MGPU_OFFSET_REFERENCE;
+    int *d_chunk_block = nullptr;    // nchunks: first block overlapping chunk c (partial-sum epilogues)
+    int variant[2] = {0, 0};         // tile kernel variant used for FP64 [0] and FP32 [1] (0 = not chosen)
     double *d_mean_out = nullptr;    // nb
     double *d_tail = nullptr;        // nfilters * lambda: tail_f[j] = sum_{k>j} T_f[k]
     int *d_block_filter = nullptr;   // nb
@@ -267,6 +270,30 @@ struct Profiler {
 // ---------------------------------------------------------------------------
 // Problem
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Map segments: independent problems solved together (batched realizations).
+// Segment r owns the local pixels [h_q[r], h_q[r+1]).  Every map-domain
+// workgroup (kMapWG pixels) lies inside one segment and segment r's workgroups
+// are [h_wg0[r], h_wg0[r+1]), so per-workgroup dot partials never mix
+// segments and each segment's sums are grouped exactly as in a solo solve.
+// A solo problem is one segment.
+// ---------------------------------------------------------------------------
+constexpr int kMapWG = 256;
+
+struct Segments {
+    int nseg = 1;
+    int64_t stride = 0;       // segment r owns global pixels [r*stride, (r+1)*stride); 0: one segment
+    int npix = -1;            // pixel count the tables describe (-1: not built)
+    std::vector<int> h_q;     // nseg+1 first local pixel of each segment
+    std::vector<int> h_wg0;   // nseg+1 first workgroup of each segment
+    int nwg = 0;              // map-domain workgroups
+    int *d_q = nullptr;       // nseg+1
+    int *d_wg0 = nullptr;     // nseg+1
+    int *d_wg_seg = nullptr;  // nwg: segment of each workgroup
+    int *d_wg_q0 = nullptr;   // nwg: first local pixel of each workgroup
+    int *d_state = nullptr;   // nseg: PCG state (0 running, 1 converged, 2 stopped)
+};
+
 struct Problem {
     Profiler prof;
     int device = 0;
@@ -277,6 +304,7 @@ struct Problem {
     Pointing pt;
     ToeplitzPlan tp;
     Precond pc;
+    Segments sg;
     // working TOD / map vectors (allocated lazily)
     double *d_tod = nullptr, *d_tod2 = nullptr;   // [m] FP64
     float *d_todf = nullptr, *d_tod2f = nullptr;  // [m] FP32 (mixed)
@@ -285,8 +313,8 @@ struct Problem {
     double *d_map[8] = {nullptr};                 // map work vectors (3*npix0 each)
     float *d_mapf = nullptr;                      // FP32 copy of a map vector (mixed P)
     double *d_red = nullptr;                      // reduction partials
-    double *d_scal = nullptr;                     // device scalars of PCG
-    double *h_pinned = nullptr;                   // pinned host scalars
+    double *d_scal = nullptr;                     // device scalars of PCG (per segment)
+    int64_t nscal = 0;
     int *d_flag = nullptr;                        // PCG stop flag
     int64_t nred = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -319,14 +347,30 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
 // compute_partials) -> d_out (may not alias d_in)
 int toeplitz_apply(Problem &pb, const double *d_in, double *d_out, bool compute_partials);
 int toeplitz_apply_f32(Problem &pb, const float *d_in, float *d_out, bool compute_partials);
+int tile_variant_count(bool fp32, int log2n);
+const char *tile_variant_desc(bool fp32, int log2n, int variant);
 
 int precond_build(Problem &pb, double *d_signal);
 int launch_precond_apply(Problem &pb, const double *d_g, double *d_Cg);
 
+// per-segment PCG outcome (batched problems)
+struct SegReport {
+    int iterations = 0, converged = 0;
+    double res0 = 0.0, res = 0.0, true_res_rel = -1.0;
+    int q0 = 0, q1 = 0;  // local pixel range of the segment
+};
+// h_hist: maxiter entries per segment (segment-major), may be NULL
 int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double tol, int maxiter,
-            int stop_after, double *h_x, double *h_hist, mgpu_report *rep);
+            int stop_after, double *h_x, double *h_hist, mgpu_report *rep,
+            std::vector<SegReport> *segs = nullptr);
 
 int ensure_work(Problem &pb);
+// a problem on the current device with its own stream and the given options
+int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt);
+// common argument checks of the problem constructors
+int check_common(int m, int nnz, int nb, const int *bs, int lambda);
+// (re)build the segment tables for the current pixel set (no-op when current)
+int segments_ensure(Problem &pb);
 
 // ---------------------------------------------------------------------------
 // small device helpers

@@ -11,22 +11,32 @@
 //
 // Device design.  All vectors stay in HBM; the host only launches.  Every
 // kernel of an iteration first reads a device stop flag and returns when it is
-// set, so the host queues `poll_every` iterations at a time and reads the flag
-// once per batch.  Dot products are fused into the producing kernels as
-// per-workgroup partials and reduced by a single workgroup in a fixed order:
-// results are deterministic and independent of what else runs on the GPU.
+// set, so the host queues a block of iterations at a time and reads the flag
+// once per block.  Dot products are fused into the producing kernels as
+// per-workgroup partials and reduced by one workgroup per segment in a fixed
+// order: results are deterministic and independent of what else runs on the
+// GPU.  With opt.graph the block of iterations is captured once into a HIP
+// graph and replayed (the kernels and their order are unchanged, so the
+// results are bit-identical to the eager launches).
+//
+// Segments (batched problems): the map is split into independent segments,
+// each with its own scalars (ro, gamma, residuals), stop test and iteration
+// count; a converged segment is frozen while the others continue.  Every
+// map-domain workgroup lies in one segment (Segments in mgpu_internal.h), so
+// a segment's dot products are summed exactly as in a solo solve.
 //
 // MGPU_MIXED: P, N^-1 and P^T run in FP32 on FP32 copies of the pointing
 // weights and TOD; map vectors, dot products and the 3x3 blocks stay FP64,
 // and every `refresh_every` iterations g is recomputed in FP64 as b - A x.
 #include "mgpu_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
 namespace mgpu {
 
-// scalar slots in d_scal
+// scalar slots of one segment in d_scal (segment r at r * S_NSCAL)
 enum {
     S_G2PIX = 0,
     S_RES,
@@ -41,11 +51,16 @@ enum {
     S_TOL,
     S_STOPK,    // stop_after (0 = off)
     S_MAXK,
-    S_NSCAL
+    S_TRUE_R2,  // ||b - A x||^2 of the returned x (mixed precision)
+    S_TRUE_B2,  // ||b||^2
+    S_NSCAL = 16
 };
 
-constexpr int kMB = 256;   // map-domain workgroup size
-constexpr int kRedMax = 1 << 16;
+// device flags: [0] every segment stopped, [1] segment 0 converged, [2] largest
+// iteration count reached, [3] number of stopped segments; [4..7] never set
+enum { F_STOP = 0, F_CONV0, F_KMAX, F_NDONE, F_NEVER = 4, F_NFLAGS = 8 };
+
+constexpr int kMB = kMapWG;  // map-domain workgroup size (pixels)
 
 __device__ __forceinline__ bool stopped(const int *flag) { return *(volatile const int *) flag != 0; }
 
@@ -73,16 +88,29 @@ __device__ __forceinline__ double reduce_parts(const double *__restrict__ p, int
     return red[0];
 }
 
-// tod = s + n (the first residual with x0 = 0, pcg_true.c:78-80), fused partials
+// pixel handled by this thread of a map-domain workgroup, its segment and the segment end
+struct SegPix {
+    int r, q, qend;
+};
+__device__ __forceinline__ SegPix seg_pixel(const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
+                                            const int *__restrict__ seg_q) {
+    SegPix p;
+    p.r = wg_seg[blockIdx.x];
+    p.q = wg_q0[blockIdx.x] + (int) threadIdx.x;
+    p.qend = seg_q[p.r + 1];
+    return p;
+}
+
+// a segment is finished: count it, and raise the global stop flag with the last one
+__device__ __forceinline__ void segment_done(int *flag, int nseg) {
+    if (atomicAdd(&flag[F_NDONE], 1) + 1 == nseg) atomicExch(&flag[F_STOP], 1);
+}
+
+// tod = s + n (the first residual with x0 = 0, pcg_true.c:78-80)
 __global__ __launch_bounds__(256) void k_rhs_tod(const double *__restrict__ s, const double *__restrict__ n,
                                                  double *__restrict__ tod, int64_t m) {
     const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t < m) tod[t] = (s[t] + n[t]) - 0.0;
-}
-
-__global__ __launch_bounds__(256) void k_to_f32_tod(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
-    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (float) a[i];
 }
 
 __global__ __launch_bounds__(256) void k_map_to_f32(const double *__restrict__ a, float *__restrict__ b, int n,
@@ -94,12 +122,14 @@ __global__ __launch_bounds__(256) void k_map_to_f32(const double *__restrict__ a
 
 // init: Cg = M g, h = Cg, partials of <Cg,g> and <g,g>
 __global__ __launch_bounds__(kMB) void k_init_dots(const double *__restrict__ inv, const double *__restrict__ g,
-                                                   double *__restrict__ Cg, double *__restrict__ h, int npix,
-                                                   double *__restrict__ part) {
+                                                   double *__restrict__ Cg, double *__restrict__ h,
+                                                   const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
+                                                   const int *__restrict__ seg_q, double *__restrict__ part) {
     __shared__ double red[kMB / 64];
-    const int q = blockIdx.x * kMB + threadIdx.x;
+    const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
+    const int q = sp.q;
     double a = 0.0, c = 0.0;
-    if (q < npix) {
+    if (q < sp.qend) {
         const double g0 = g[3 * q], g1 = g[3 * q + 1], g2 = g[3 * q + 2];
         const double *M = inv + 9 * (int64_t) q;
         for (int r = 0; r < 3; r++) {
@@ -122,35 +152,47 @@ __global__ __launch_bounds__(kMB) void k_init_dots(const double *__restrict__ in
     }
 }
 
-__global__ __launch_bounds__(256) void k_init_scalars(const double *__restrict__ part, int nparts,
-                                                      double *__restrict__ S, double *__restrict__ hist,
-                                                      int *__restrict__ flag) {
+// one workgroup per segment
+__global__ __launch_bounds__(256) void k_init_scalars(const double *__restrict__ part, int nwg,
+                                                      const int *__restrict__ wg0, double *__restrict__ S,
+                                                      double *__restrict__ hist, int maxiter,
+                                                      int *__restrict__ state, int nseg, int *__restrict__ flag) {
     __shared__ double red[256];
-    const double g2pix = reduce_parts(part, nparts, red);
+    const int r = blockIdx.x;
+    const int p0 = wg0[r], np = wg0[r + 1] - p0;
+    const double g2pix = reduce_parts(part + p0, np, red);
     __syncthreads();
-    const double res = reduce_parts(part + nparts, nparts, red);
+    const double res = reduce_parts(part + nwg + p0, np, red);
     if (threadIdx.x == 0) {
-        S[S_G2PIX] = g2pix;
-        S[S_RES] = res;
-        S[S_RES0] = res;
-        S[S_TOL2] = S[S_TOL] * S[S_TOL] * res;
-        S[S_K] = 0.0;
-        if (hist) hist[0] = sqrt(res) / sqrt(res);
-        flag[0] = 0;
-        flag[1] = 0;  // converged
-        flag[2] = 0;  // iterations done
+        double *Sr = S + (int64_t) r * S_NSCAL;
+        Sr[S_G2PIX] = g2pix;
+        Sr[S_RES] = res;
+        Sr[S_RES0] = res;
+        Sr[S_TOL2] = Sr[S_TOL] * Sr[S_TOL] * res;
+        Sr[S_K] = 0.0;
+        if (hist) hist[(int64_t) r * maxiter] = sqrt(res) / sqrt(res);
+        if (np == 0) {  // empty segment (every pixel cut): nothing to solve
+            state[r] = 2;
+            segment_done(flag, nseg);
+        } else {
+            state[r] = 0;
+        }
     }
 }
 
-// ro = g2pix / coeff
-__global__ __launch_bounds__(256) void k_coeff(const double *__restrict__ part, int nparts, double *__restrict__ S,
+// ro = g2pix / coeff, one workgroup per segment
+__global__ __launch_bounds__(256) void k_coeff(const double *__restrict__ part, const int *__restrict__ wg0,
+                                               double *__restrict__ S, const int *__restrict__ state,
                                                const int *__restrict__ flag) {
     if (stopped(flag)) return;
+    const int r = blockIdx.x;
+    if (state[r]) return;
     __shared__ double red[256];
-    const double coeff = reduce_parts(part, nparts, red);
+    const double coeff = reduce_parts(part + wg0[r], wg0[r + 1] - wg0[r], red);
     if (threadIdx.x == 0) {
-        S[S_COEFF] = coeff;
-        S[S_RO] = S[S_G2PIX] / coeff;
+        double *Sr = S + (int64_t) r * S_NSCAL;
+        Sr[S_COEFF] = coeff;
+        Sr[S_RO] = Sr[S_G2PIX] / coeff;
     }
 }
 
@@ -158,14 +200,18 @@ __global__ __launch_bounds__(256) void k_coeff(const double *__restrict__ part, 
 __global__ __launch_bounds__(kMB) void k_update(double *__restrict__ x, const double *__restrict__ h,
                                                 double *__restrict__ g, const double *__restrict__ gp,
                                                 const double *__restrict__ qv, const double *__restrict__ inv,
-                                                double *__restrict__ Cg, int npix, const double *__restrict__ S,
+                                                double *__restrict__ Cg, const int *__restrict__ wg_seg,
+                                                const int *__restrict__ wg_q0, const int *__restrict__ seg_q,
+                                                const double *__restrict__ S, const int *__restrict__ state,
                                                 double *__restrict__ part, const int *__restrict__ flag) {
     if (stopped(flag)) return;
+    const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
+    if (state[sp.r]) return;  // frozen segment (uniform over the workgroup)
     __shared__ double red[kMB / 64];
-    const int q = blockIdx.x * kMB + threadIdx.x;
+    const int q = sp.q;
     double a = 0.0, b = 0.0, c = 0.0;
-    if (q < npix) {
-        const double ro = S[S_RO];
+    if (q < sp.qend) {
+        const double ro = S[(int64_t) sp.r * S_NSCAL + S_RO];
         double gn[3], gpv[3];
         for (int r = 0; r < 3; r++) {
             const int i = 3 * q + r;
@@ -196,78 +242,89 @@ __global__ __launch_bounds__(kMB) void k_update(double *__restrict__ x, const do
     }
 }
 
-// dots -> g2pix, polak, res; history; stop test; gamma
-__global__ __launch_bounds__(256) void k_scalars(const double *__restrict__ part, int nparts, double *__restrict__ S,
-                                                 double *__restrict__ hist, int *__restrict__ flag) {
+// dots -> g2pix, polak, res; history; stop test; gamma.  One workgroup per segment.
+__global__ __launch_bounds__(256) void k_scalars(const double *__restrict__ part, int nwg,
+                                                 const int *__restrict__ wg0, double *__restrict__ S,
+                                                 double *__restrict__ hist, int maxiter, int *__restrict__ state,
+                                                 int nseg, int *__restrict__ flag) {
     if (stopped(flag)) return;
+    const int r = blockIdx.x;
+    if (state[r]) return;
     __shared__ double red[256];
-    const double g2pix = reduce_parts(part, nparts, red);
+    const int p0 = wg0[r], np = wg0[r + 1] - p0;
+    const double g2pix = reduce_parts(part + p0, np, red);
     __syncthreads();
-    const double polak = reduce_parts(part + nparts, nparts, red);
+    const double polak = reduce_parts(part + nwg + p0, np, red);
     __syncthreads();
-    const double res = reduce_parts(part + 2 * nparts, nparts, red);
+    const double res = reduce_parts(part + 2 * nwg + p0, np, red);
     if (threadIdx.x == 0) {
-        const double k = S[S_K] + 1.0;
-        S[S_K] = k;
-        S[S_G2PIXP] = S[S_G2PIX];
-        S[S_G2PIX] = g2pix;
-        S[S_POLAK] = polak;
-        S[S_RES] = res;
+        double *Sr = S + (int64_t) r * S_NSCAL;
+        const double k = Sr[S_K] + 1.0;
+        Sr[S_K] = k;
+        Sr[S_G2PIXP] = Sr[S_G2PIX];
+        Sr[S_G2PIX] = g2pix;
+        Sr[S_POLAK] = polak;
+        Sr[S_RES] = res;
         const int ki = (int) k;
-        if (hist && ki < (int) S[S_MAXK]) hist[ki] = sqrt(res) / sqrt(S[S_RES0]);
-        flag[2] = ki;
-        if (res <= S[S_TOL2]) {
-            flag[1] = 1;
-            flag[0] = 1;
-        } else if ((S[S_STOPK] > 0.0 && k >= S[S_STOPK]) || k + 1.0 >= S[S_MAXK]) {
-            flag[0] = 1;
+        if (hist && ki < maxiter) hist[(int64_t) r * maxiter + ki] = sqrt(res) / sqrt(Sr[S_RES0]);
+        atomicMax(&flag[F_KMAX], ki);
+        int st = 0;
+        if (res <= Sr[S_TOL2])
+            st = 1;
+        else if ((Sr[S_STOPK] > 0.0 && k >= Sr[S_STOPK]) || k + 1.0 >= Sr[S_MAXK])
+            st = 2;
+        if (st) {
+            state[r] = st;
+            if (r == 0 && st == 1) flag[F_CONV0] = 1;
+            segment_done(flag, nseg);
         }
-        S[S_GAMMA] = polak / S[S_G2PIXP];
+        Sr[S_GAMMA] = polak / Sr[S_G2PIXP];
     }
 }
 
 // h = h gamma + Cg
-__global__ __launch_bounds__(kMB) void k_hupdate(double *__restrict__ h, const double *__restrict__ Cg, int n,
-                                                 const double *__restrict__ S, const int *__restrict__ flag) {
+__global__ __launch_bounds__(kMB) void k_hupdate(double *__restrict__ h, const double *__restrict__ Cg,
+                                                 const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
+                                                 const int *__restrict__ seg_q, const double *__restrict__ S,
+                                                 const int *__restrict__ state, const int *__restrict__ flag) {
     if (stopped(flag)) return;
-    const int i = blockIdx.x * kMB + threadIdx.x;
-    if (i < n) h[i] = h[i] * S[S_GAMMA] + Cg[i];
+    const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
+    if (state[sp.r] || sp.q >= sp.qend) return;
+    const double gamma = S[(int64_t) sp.r * S_NSCAL + S_GAMMA];
+    for (int c = 0; c < 3; c++) {
+        const int i = 3 * sp.q + c;
+        h[i] = h[i] * gamma + Cg[i];
+    }
 }
 
-// FP64 refresh of the residual: g = b - Ax (mixed precision)
+// FP64 refresh of the residual: g = b - Ax (mixed precision); state == nullptr
+// computes every segment (final true residual)
 __global__ __launch_bounds__(kMB) void k_refresh(const double *__restrict__ b, const double *__restrict__ Ax,
-                                                 double *__restrict__ g, int n, const int *__restrict__ flag) {
+                                                 double *__restrict__ g, const int *__restrict__ wg_seg,
+                                                 const int *__restrict__ wg_q0, const int *__restrict__ seg_q,
+                                                 const int *__restrict__ state, const int *__restrict__ flag) {
     if (stopped(flag)) return;
-    const int i = blockIdx.x * kMB + threadIdx.x;
-    if (i < n) g[i] = b[i] - Ax[i];
-}
-
-__global__ __launch_bounds__(256) void k_dot_plain(const double *__restrict__ a, const double *__restrict__ b, int n,
-                                                   double *__restrict__ part) {
-    __shared__ double red[4];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    double v = i < n ? a[i] * b[i] : 0.0;
-    v = block_sum(v, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = v;
-}
-
-__global__ __launch_bounds__(256) void k_sum_parts(const double *__restrict__ part, int nparts,
-                                                   double *__restrict__ out) {
-    __shared__ double red[256];
-    const double s = reduce_parts(part, nparts, red);
-    if (threadIdx.x == 0) *out = s;
+    const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
+    if ((state && state[sp.r]) || sp.q >= sp.qend) return;
+    for (int c = 0; c < 3; c++) {
+        const int i = 3 * sp.q + c;
+        g[i] = b[i] - Ax[i];
+    }
 }
 
 // refreshed residual: Cg = M g and the three dots <Cg,g>, <Cg,g-gp>, <g,g>
 __global__ __launch_bounds__(kMB) void k_refresh_three(const double *__restrict__ inv, const double *__restrict__ g,
                                                        const double *__restrict__ gp, double *__restrict__ Cg,
-                                                       int npix, double *__restrict__ part,
-                                                       const int *__restrict__ flag) {
+                                                       const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
+                                                       const int *__restrict__ seg_q, const int *__restrict__ state,
+                                                       double *__restrict__ part, const int *__restrict__ flag) {
     if (stopped(flag)) return;
+    const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
+    if (state[sp.r]) return;
     __shared__ double red[kMB / 64];
-    const int q = blockIdx.x * kMB + threadIdx.x;
+    const int q = sp.q;
     double a = 0.0, b = 0.0, c = 0.0;
-    if (q < npix) {
+    if (q < sp.qend) {
         const double g0 = g[3 * q], g1 = g[3 * q + 1], g2 = g[3 * q + 2];
         const double *M = inv + 9 * (int64_t) q;
         for (int r = 0; r < 3; r++) {
@@ -292,9 +349,120 @@ __global__ __launch_bounds__(kMB) void k_refresh_three(const double *__restrict_
     }
 }
 
+// partials of <r,r> and <b,b> (true residual of the returned map)
+__global__ __launch_bounds__(kMB) void k_two_norms(const double *__restrict__ rv, const double *__restrict__ b,
+                                                   const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
+                                                   const int *__restrict__ seg_q, double *__restrict__ part) {
+    __shared__ double red[kMB / 64];
+    const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
+    double a = 0.0, c = 0.0;
+    if (sp.q < sp.qend)
+        for (int k = 0; k < 3; k++) {
+            const int i = 3 * sp.q + k;
+            a += rv[i] * rv[i];
+            c += b[i] * b[i];
+        }
+    a = block_sum(a, red);
+    c = block_sum(c, red);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[gridDim.x + blockIdx.x] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_seg_norms(const double *__restrict__ part, int nwg,
+                                                   const int *__restrict__ wg0, double *__restrict__ S) {
+    __shared__ double red[256];
+    const int r = blockIdx.x;
+    const int p0 = wg0[r], np = wg0[r + 1] - p0;
+    const double a = reduce_parts(part + p0, np, red);
+    __syncthreads();
+    const double c = reduce_parts(part + nwg + p0, np, red);
+    if (threadIdx.x == 0) {
+        S[(int64_t) r * S_NSCAL + S_TRUE_R2] = a;
+        S[(int64_t) r * S_NSCAL + S_TRUE_B2] = c;
+    }
+}
+
+// first local pixel of every segment: lower bound of r * stride in the sorted pixel ids
+__global__ void k_seg_bounds(const int *__restrict__ pixg, int npix, int64_t stride, int nseg, int *__restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nseg) return;
+    if (r == nseg) {
+        out[r] = npix;
+        return;
+    }
+    const int64_t key = (int64_t) r * stride;
+    int lo = 0, hi = npix;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t) pixg[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    out[r] = lo;
+}
+
 // ---------------------------------------------------------------------------
-// host driver
+// host side
 // ---------------------------------------------------------------------------
+int segments_ensure(Problem &pb) {
+    Segments &G = pb.sg;
+    const int npix = pb.pt.npix;
+    if (G.npix == npix && G.d_wg_seg) return MGPU_OK;
+    hipStream_t s = pb.stream;
+    const int nseg = G.stride > 0 ? G.nseg : 1;
+    G.h_q.assign((size_t) nseg + 1, 0);
+    if (nseg == 1) {
+        G.h_q[1] = npix;
+    } else {
+        int *d_b;
+        MGPU_RET(pb.arena.alloc(&d_b, (size_t) nseg + 1));
+        k_seg_bounds<<<grid_for(nseg + 1, 64), 64, 0, s>>>(pb.pt.d_pixg, npix, G.stride, nseg, d_b);
+        MGPU_LAUNCHED();
+        MGPU_HIP(hipMemcpyAsync(G.h_q.data(), d_b, sizeof(int) * (nseg + 1), hipMemcpyDeviceToHost, s));
+        MGPU_HIP(hipStreamSynchronize(s));
+        pb.arena.release(d_b);
+    }
+    G.h_wg0.assign((size_t) nseg + 1, 0);
+    std::vector<int> wseg, wq0;
+    for (int r = 0; r < nseg; r++) {
+        G.h_wg0[r] = (int) wseg.size();
+        for (int a = G.h_q[r]; a < G.h_q[r + 1]; a += kMapWG) {
+            wseg.push_back(r);
+            wq0.push_back(a);
+        }
+    }
+    G.h_wg0[nseg] = (int) wseg.size();
+    G.nwg = (int) wseg.size();
+    for (int **p : {&G.d_q, &G.d_wg0, &G.d_wg_seg, &G.d_wg_q0})
+        if (*p) {
+            pb.arena.release(*p);
+            *p = nullptr;
+        }
+    if (G.d_state && G.nseg != nseg) {
+        pb.arena.release(G.d_state);
+        G.d_state = nullptr;
+    }
+    G.nseg = nseg;
+    const size_t nw = std::max<size_t>(wseg.size(), 1);
+    MGPU_RET(pb.arena.alloc(&G.d_q, (size_t) nseg + 1));
+    MGPU_RET(pb.arena.alloc(&G.d_wg0, (size_t) nseg + 1));
+    MGPU_RET(pb.arena.alloc(&G.d_wg_seg, nw));
+    MGPU_RET(pb.arena.alloc(&G.d_wg_q0, nw));
+    if (!G.d_state) MGPU_RET(pb.arena.alloc(&G.d_state, (size_t) nseg));
+    MGPU_HIP(hipMemcpyAsync(G.d_q, G.h_q.data(), sizeof(int) * (nseg + 1), hipMemcpyHostToDevice, s));
+    MGPU_HIP(hipMemcpyAsync(G.d_wg0, G.h_wg0.data(), sizeof(int) * (nseg + 1), hipMemcpyHostToDevice, s));
+    if (!wseg.empty()) {
+        MGPU_HIP(hipMemcpyAsync(G.d_wg_seg, wseg.data(), sizeof(int) * wseg.size(), hipMemcpyHostToDevice, s));
+        MGPU_HIP(hipMemcpyAsync(G.d_wg_q0, wq0.data(), sizeof(int) * wq0.size(), hipMemcpyHostToDevice, s));
+    }
+    MGPU_HIP(hipStreamSynchronize(s));
+    G.npix = npix;
+    return MGPU_OK;
+}
+
 int ensure_work(Problem &pb) {
     const int64_t m = pb.pt.m;
     const int n0 = 3 * pb.pt.npix0;
@@ -304,12 +472,27 @@ int ensure_work(Problem &pb) {
     }
     if (!pb.d_map[0])
         for (int i = 0; i < 8; i++) MGPU_RET(pb.arena.alloc(&pb.d_map[i], (size_t) n0));
+    // partials: 3 per map workgroup (at most one extra workgroup per segment)
+    const int64_t need_red = 3 * ((int64_t) grid_for(pb.pt.npix0, kMB) + pb.sg.nseg + 1) + 64;
+    if (pb.d_red && pb.nred < need_red) {
+        pb.arena.release(pb.d_red);
+        pb.d_red = nullptr;
+    }
     if (!pb.d_red) {
-        pb.nred = 3 * ((int64_t) grid_for(pb.pt.npix0, kMB) + 1) + 4 * (int64_t) grid_for(n0, 256) + 64;
+        pb.nred = need_red;
         MGPU_RET(pb.arena.alloc(&pb.d_red, (size_t) pb.nred));
-        MGPU_RET(pb.arena.alloc(&pb.d_scal, (size_t) 32));
-        MGPU_RET(pb.arena.alloc(&pb.d_flag, (size_t) 8));
-        MGPU_HIP(hipHostMalloc((void **) &pb.h_pinned, 64 * sizeof(double), hipHostMallocDefault));
+    }
+    const int64_t nscal = (int64_t) S_NSCAL * pb.sg.nseg;
+    if (pb.d_scal && pb.nscal < nscal) {
+        pb.arena.release(pb.d_scal);
+        pb.d_scal = nullptr;
+    }
+    if (!pb.d_scal) {
+        pb.nscal = nscal;
+        MGPU_RET(pb.arena.alloc(&pb.d_scal, (size_t) nscal));
+    }
+    if (!pb.d_flag) {
+        MGPU_RET(pb.arena.alloc(&pb.d_flag, (size_t) F_NFLAGS));
         MGPU_HIP(hipEventCreate(&pb.ev0));
         MGPU_HIP(hipEventCreate(&pb.ev1));
     }
@@ -347,33 +530,53 @@ static int apply_A32(Problem &pb, const double *d_v, double *d_out, double *d_do
     return MGPU_OK;
 }
 
+// owns a captured graph of one block of iterations
+struct GraphBlock {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphBlock() {
+        if (exec) (void) hipGraphExecDestroy(exec);
+        if (graph) (void) hipGraphDestroy(graph);
+    }
+};
+
 int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double tol, int maxiter, int stop_after,
-            double *h_x, double *h_hist, mgpu_report *rep) {
+            double *h_x, double *h_hist, mgpu_report *rep, std::vector<SegReport> *segs) {
     MGPU_ARG(pb.pc.built, "pcg before precond_build");
     MGPU_ARG(maxiter >= 1, "maxiter=%d must be >= 1", maxiter);
+    MGPU_RET(segments_ensure(pb));
     MGPU_RET(ensure_work(pb));
     const bool mixed = pb.opt.precision == MGPU_MIXED;
     if (mixed) MGPU_RET(pointing_make_fp32(pb));
     hipStream_t s = pb.stream;
+    const Segments &G = pb.sg;
+    const int nseg = G.nseg, nwg = G.nwg;
     const int npix = pb.pt.npix, n = 3 * npix;
     const int64_t m = pb.pt.m;
-    const unsigned gmap = grid_for(npix, kMB);
+    MGPU_ARG(nwg > 0, "no pixel left to solve for");
     double *x = pb.d_map[0], *h = pb.d_map[1], *gA = pb.d_map[2], *gB = pb.d_map[3], *qv = pb.d_map[4],
            *Cg = pb.d_map[5], *bvec = pb.d_map[6], *Ax = pb.d_map[7];
     double *part = pb.d_red;
     double *S = pb.d_scal;
     int *flag = pb.d_flag;
+    int *state = G.d_state;
+    const int *wseg = G.d_wg_seg, *wq0 = G.d_wg_q0, *sq = G.d_q, *wg0 = G.d_wg0;
     double *d_hist = nullptr;
-    if (h_hist) MGPU_RET(pb.arena.alloc(&d_hist, (size_t) maxiter));
+    if (h_hist) {
+        MGPU_RET(pb.arena.alloc(&d_hist, (size_t) maxiter * nseg));
+        MGPU_HIP(hipMemsetAsync(d_hist, 0, sizeof(double) * (size_t) maxiter * nseg, s));
+    }
 
-    // scalars: tol, stop_after, maxiter
-    double init[S_NSCAL] = {0};
-    init[S_TOL] = tol;
-    init[S_STOPK] = stop_after > 0 ? (double) stop_after : 0.0;
-    init[S_MAXK] = (double) maxiter;
-    MGPU_HIP(hipMemcpyAsync(S, init, sizeof(init), hipMemcpyHostToDevice, s));
+    // scalars of every segment: tol, stop_after, maxiter
+    std::vector<double> init((size_t) S_NSCAL * nseg, 0.0);
+    for (int r = 0; r < nseg; r++) {
+        init[(size_t) r * S_NSCAL + S_TOL] = tol;
+        init[(size_t) r * S_NSCAL + S_STOPK] = stop_after > 0 ? (double) stop_after : 0.0;
+        init[(size_t) r * S_NSCAL + S_MAXK] = (double) maxiter;
+    }
+    MGPU_HIP(hipMemcpyAsync(S, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, s));
     MGPU_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
-    MGPU_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 8, s));
+    MGPU_HIP(hipMemsetAsync(flag, 0, sizeof(int) * F_NFLAGS, s));
     MGPU_HIP(hipEventRecord(pb.ev0, s));
 
     // g0 = P^T N^-1 (s + n) (FP64 in both modes: it is b, kept for refreshes)
@@ -382,85 +585,134 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, true));
     MGPU_RET(launch_trmatvec(pb, pb.d_tod2, gA, nullptr, nullptr, nullptr));
     MGPU_HIP(hipMemcpyAsync(bvec, gA, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    k_init_dots<<<gmap, kMB, 0, s>>>(pb.pc.d_inv, gA, Cg, h, npix, part);
+    k_init_dots<<<nwg, kMB, 0, s>>>(pb.pc.d_inv, gA, Cg, h, wseg, wq0, sq, part);
     MGPU_LAUNCHED();
-    k_init_scalars<<<1, 256, 0, s>>>(part, (int) gmap, S, d_hist, flag);
+    k_init_scalars<<<nseg, 256, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag);
     MGPU_LAUNCHED();
 
     const int poll = pb.opt.poll_every > 0 ? pb.opt.poll_every : 8;
     const int refresh = mixed ? (pb.opt.refresh_every > 0 ? pb.opt.refresh_every : 10) : 0;
+    // Iterations are queued in blocks between host polls.  A graph block must
+    // repeat exactly: an even length (g/gp swap parity) and, in mixed
+    // precision, a multiple of the refresh period.
+    const bool use_graph = pb.opt.graph != 0 && !pb.prof.on;
+    int block = poll;
+    if (use_graph) {
+        const int period = refresh ? (refresh % 2 ? 2 * refresh : refresh) : 2;
+        block = std::max(1, (poll + period - 1) / period) * period;
+    }
     int launched = 0;  // iterations queued (k = 1 .. launched)
-    int refreshes = 0;
     double *g = gA, *gp = gB;
-    int nparts_pt = 0;
-    bool done = false;
-    // maxiter == 1: no loop iteration (for k = 1; k < K)
-    if (maxiter <= 1) done = true;
-    while (!done) {
-        for (int it = 0; it < poll && launched + 1 < maxiter; it++) {
-            ++launched;
-            pb.prof.tag = launched;
-            std::swap(g, gp);  // pcg_true.c:140-142
-            // q = A h with fused <h, q>
-            if (mixed)
-                MGPU_RET(apply_A32(pb, h, qv, part, flag));
-            else
-                MGPU_RET(apply_A64(pb, h, qv, part));
-            nparts_pt = trmatvec_nparts(pb);
-            pb.prof.begin(MGPU_KT_MAP, s);
-            k_coeff<<<1, 256, 0, s>>>(part, nparts_pt, S, flag);
+    GraphBlock gb;
+
+    auto enqueue_iteration = [&]() -> int {
+        ++launched;
+        pb.prof.tag = launched;
+        std::swap(g, gp);  // pcg_true.c:140-142
+        // q = A h with fused <h, q>
+        if (mixed)
+            MGPU_RET(apply_A32(pb, h, qv, part, flag));
+        else
+            MGPU_RET(apply_A64(pb, h, qv, part));
+        pb.prof.begin(MGPU_KT_MAP, s);
+        k_coeff<<<nseg, 256, 0, s>>>(part, wg0, S, state, flag);
+        MGPU_LAUNCHED();
+        k_update<<<nwg, kMB, 0, s>>>(x, h, g, gp, qv, pb.pc.d_inv, Cg, wseg, wq0, sq, S, state, part, flag);
+        MGPU_LAUNCHED();
+        pb.prof.end(s);
+        if (refresh && launched % refresh == 0) {
+            // g = b - A x in FP64, then Cg = M g and the three dots on the refreshed residual
+            pb.prof.begin(MGPU_KT_REFRESH, s);
+            const bool keep = pb.prof.on;
+            pb.prof.on = false;  // the FP64 operator inside is accounted as a refresh
+            MGPU_RET(apply_A64(pb, x, Ax, nullptr));
+            k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, g, wseg, wq0, sq, state, flag);
             MGPU_LAUNCHED();
-            k_update<<<gmap, kMB, 0, s>>>(x, h, g, gp, qv, pb.pc.d_inv, Cg, npix, S, part, flag);
+            k_refresh_three<<<nwg, kMB, 0, s>>>(pb.pc.d_inv, g, gp, Cg, wseg, wq0, sq, state, part, flag);
             MGPU_LAUNCHED();
-            pb.prof.end(s);
-            if (refresh && launched % refresh == 0) {
-                // g = b - A x in FP64, then Cg = M g and g2pix on the refreshed residual
-                pb.prof.begin(MGPU_KT_REFRESH, s);
-                const bool keep = pb.prof.on;
-                pb.prof.on = false;  // the FP64 operator inside is accounted as a refresh
-                MGPU_RET(apply_A64(pb, x, Ax, nullptr));
-                k_refresh<<<grid_for(n, kMB), kMB, 0, s>>>(bvec, Ax, g, n, flag);
-                MGPU_LAUNCHED();
-                // the three dots of the stop test and of gamma on the refreshed g
-                k_refresh_three<<<gmap, kMB, 0, s>>>(pb.pc.d_inv, g, gp, Cg, npix, part, flag);
-                MGPU_LAUNCHED();
-                pb.prof.on = keep;
-                pb.prof.end(s);
-                refreshes++;
-            }
-            pb.prof.begin(MGPU_KT_MAP, s);
-            k_scalars<<<1, 256, 0, s>>>(part, (int) gmap, S, d_hist, flag);
-            MGPU_LAUNCHED();
-            k_hupdate<<<grid_for(n, kMB), kMB, 0, s>>>(h, Cg, n, S, flag);
-            MGPU_LAUNCHED();
+            pb.prof.on = keep;
             pb.prof.end(s);
         }
-        int hf[4];
-        MGPU_HIP(hipMemcpyAsync(hf, flag, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
+        pb.prof.begin(MGPU_KT_MAP, s);
+        k_scalars<<<nseg, 256, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag);
+        MGPU_LAUNCHED();
+        k_hupdate<<<nwg, kMB, 0, s>>>(h, Cg, wseg, wq0, sq, S, state, flag);
+        MGPU_LAUNCHED();
+        pb.prof.end(s);
+        return MGPU_OK;
+    };
+
+    bool done = maxiter <= 1;  // maxiter == 1: no loop iteration (for k = 1; k < K)
+    while (!done) {
+        if (gb.exec) {
+            MGPU_HIP(hipGraphLaunch(gb.exec, s));
+            launched += block;
+        } else if (use_graph && launched > 0) {
+            // the first block ran eagerly (tile autotuning, kernel attributes); capture the second
+            MGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = MGPU_OK;
+            for (int it = 0; it < block && rc == MGPU_OK; it++) rc = enqueue_iteration();
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(s, &graph);
+            gb.graph = graph;
+            MGPU_RET(rc);
+            MGPU_HIP(ec);
+            MGPU_HIP(hipGraphInstantiate(&gb.exec, gb.graph, nullptr, nullptr, 0));
+            MGPU_HIP(hipGraphLaunch(gb.exec, s));
+        } else {
+            for (int it = 0; it < block && launched + 1 < maxiter; it++) MGPU_RET(enqueue_iteration());
+        }
+        int hf[F_NFLAGS];
+        MGPU_HIP(hipMemcpyAsync(hf, flag, sizeof(hf), hipMemcpyDeviceToHost, s));
         MGPU_HIP(hipStreamSynchronize(s));
-        pb.prof.collect(hf[2]);
-        if (hf[0] || launched + 1 >= maxiter) done = true;
+        pb.prof.collect(hf[F_KMAX]);
+        if (hf[F_STOP] || launched + 1 >= maxiter) done = true;
     }
     pb.prof.tag = 0;
     MGPU_HIP(hipEventRecord(pb.ev1, s));
     // results
-    int hf[4] = {0, 0, 0, 0};
-    double hs[S_NSCAL];
-    MGPU_HIP(hipMemcpyAsync(hf, flag, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
-    MGPU_HIP(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, s));
+    int hf[F_NFLAGS] = {0};
+    std::vector<double> hs((size_t) S_NSCAL * nseg);
+    std::vector<int> hstate((size_t) nseg);
+    MGPU_HIP(hipMemcpyAsync(hf, flag, sizeof(hf), hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipMemcpyAsync(hstate.data(), state, sizeof(int) * nseg, hipMemcpyDeviceToHost, s));
     if (h_x) MGPU_HIP(hipMemcpyAsync(h_x, x, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     MGPU_HIP(hipStreamSynchronize(s));
-    const int kdone = hf[2];
-    if (h_hist) {
-        const int nh = std::min(maxiter, kdone + 1);
-        MGPU_HIP(hipMemcpy(h_hist, d_hist, sizeof(double) * nh, hipMemcpyDeviceToHost));
-    }
+    if (h_hist) MGPU_HIP(hipMemcpy(h_hist, d_hist, sizeof(double) * (size_t) maxiter * nseg, hipMemcpyDeviceToHost));
     float ms = 0.f;
     MGPU_HIP(hipEventElapsedTime(&ms, pb.ev0, pb.ev1));
-    if (rep) {
-        rep->converged = hf[1];
+    // refreshes performed: one per `refresh` iterations up to the last iteration that ran
+    const int refreshes = refresh ? hf[F_KMAX] / refresh : 0;
+
+    std::vector<double> true_rel((size_t) nseg, -1.0);
+    if (mixed && (rep || segs)) {
+        // true residual of the returned x in FP64: ||b - A x|| / ||b|| (not timed)
+        const bool keep = pb.prof.on;
+        pb.prof.on = false;
+        const int rc = apply_A64(pb, x, Ax, nullptr);
+        pb.prof.on = keep;
+        MGPU_RET(rc);
+        k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, qv, wseg, wq0, sq, nullptr, pb.d_flag + F_NEVER);
+        MGPU_LAUNCHED();
+        k_two_norms<<<nwg, kMB, 0, s>>>(qv, bvec, wseg, wq0, sq, part);
+        MGPU_LAUNCHED();
+        k_seg_norms<<<nseg, 256, 0, s>>>(part, nwg, wg0, S);
+        MGPU_LAUNCHED();
+        MGPU_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
+        MGPU_HIP(hipStreamSynchronize(s));
+        for (int r = 0; r < nseg; r++)
+            true_rel[r] = std::sqrt(hs[(size_t) r * S_NSCAL + S_TRUE_R2]) /
+                          std::sqrt(hs[(size_t) r * S_NSCAL + S_TRUE_B2]);
+    }
+    auto seg_iterations = [&](int r) {
         // reference: k at break, or K when the loop ran out (pcg_true.c:224)
-        rep->iterations = hf[1] || (stop_after > 0 && kdone >= stop_after) ? kdone : maxiter;
+        const int kr = (int) hs[(size_t) r * S_NSCAL + S_K];
+        return hstate[r] == 1 || (stop_after > 0 && kr >= stop_after) ? kr : maxiter;
+    };
+    if (rep) {
+        rep->converged = hstate[0] == 1;
+        rep->iterations = seg_iterations(0);
         rep->res0 = hs[S_RES0];
         rep->res = hs[S_RES];
         rep->n_map = n;
@@ -469,27 +721,21 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         rep->n_cut = pb.pc.n_cut;
         rep->t_solve_ms = ms;
         rep->refreshes = refreshes;
-        rep->true_res_rel = -1.0;
+        rep->tile_variant = pb.tp.variant[mixed ? 1 : 0];
+        rep->true_res_rel = true_rel[0];
     }
-    if (mixed && rep) {
-        // true residual of the returned x in FP64: ||b - A x|| / ||b|| (not timed)
-        const bool keep = pb.prof.on;
-        pb.prof.on = false;
-        const int rc = apply_A64(pb, x, Ax, nullptr);
-        pb.prof.on = keep;
-        MGPU_RET(rc);
-        k_refresh<<<grid_for(n, kMB), kMB, 0, s>>>(bvec, Ax, qv, n, pb.d_flag + 4);
-        MGPU_LAUNCHED();
-        const unsigned gd = grid_for(n, 256);
-        k_dot_plain<<<gd, 256, 0, s>>>(qv, qv, n, part);
-        k_sum_parts<<<1, 256, 0, s>>>(part, (int) gd, S + 20);
-        k_dot_plain<<<gd, 256, 0, s>>>(bvec, bvec, n, part);
-        k_sum_parts<<<1, 256, 0, s>>>(part, (int) gd, S + 21);
-        MGPU_LAUNCHED();
-        double two[2];
-        MGPU_HIP(hipMemcpyAsync(two, S + 20, sizeof(two), hipMemcpyDeviceToHost, s));
-        MGPU_HIP(hipStreamSynchronize(s));
-        rep->true_res_rel = std::sqrt(two[0]) / std::sqrt(two[1]);
+    if (segs) {
+        segs->assign((size_t) nseg, SegReport{});
+        for (int r = 0; r < nseg; r++) {
+            SegReport &o = (*segs)[r];
+            o.iterations = seg_iterations(r);
+            o.converged = hstate[r] == 1;
+            o.res0 = hs[(size_t) r * S_NSCAL + S_RES0];
+            o.res = hs[(size_t) r * S_NSCAL + S_RES];
+            o.true_res_rel = true_rel[r];
+            o.q0 = G.h_q[r];
+            o.q1 = G.h_q[r + 1];
+        }
     }
     if (d_hist) pb.arena.release(d_hist);
     return MGPU_OK;

@@ -510,25 +510,12 @@ __device__ __forceinline__ double wg_sum_256(double v, double *red) {
     return s;  // valid in thread 0
 }
 
-// block whose range contains t (start[nb] = covered); nb if t >= covered
-__device__ __forceinline__ int find_block(const int64_t *__restrict__ start, int nb, int64_t t) {
-    int lo = 0, hi = nb;  // invariant: start[lo] <= t < start[hi] (if t < covered)
-    if (t >= start[nb]) return nb;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (start[mid] <= t)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 template <typename TX, typename TW, typename TW2, bool W0, bool PARTIALS>
 __global__ __launch_bounds__(kPBlock) void k_matvec(const int *__restrict__ q, const TW *__restrict__ w0,
                                                     const TW2 *__restrict__ w12, const TX *__restrict__ x,
                                                     TX *__restrict__ y, int64_t m, const int64_t *__restrict__ start,
-                                                    int nb, double *__restrict__ partials) {
+                                                    const int *__restrict__ chunk_block, int nb,
+                                                    double *__restrict__ partials) {
     __shared__ double red[kPBlock / 64];
     constexpr int PER = kChunk / kPBlock;
     const int64_t c = blockIdx.x;
@@ -563,7 +550,7 @@ __global__ __launch_bounds__(kPBlock) void k_matvec(const int *__restrict__ q, c
     }
     if (PARTIALS) {
         const int64_t c1 = min(c0 + (int64_t) kChunk, m);
-        int b = find_block(start, nb, c0);
+        int b = chunk_block[c];
         for (; b < nb && start[b] < c1; b++) {
             const int64_t lo = start[b], hi = start[b + 1];
             double s = 0.0;
@@ -583,22 +570,24 @@ static int launch_matvec_t(Problem &pb, const TX *d_x, TX *d_y, bool partials, c
     const Pointing &P = pb.pt;
     const ToeplitzPlan &T = pb.tp;
     const unsigned g = grid_for(P.m, kChunk);
-    partials = partials && T.nb > 0 && T.d_partials && T.lambda > 1 && T.offset_mode != MGPU_OFFSET_NONE;
+    partials = partials && T.nb > 0 && T.d_partials && T.d_chunk_block && T.lambda > 1 &&
+               T.offset_mode != MGPU_OFFSET_NONE;
     const int *q = P.d_q;
     const int64_t *st = T.d_start;
+    const int *cb = T.d_chunk_block;
     if (P.unit_w0) {
         if (partials)
             k_matvec<TX, TW, TW2, false, true>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, T.nb, T.d_partials);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials);
         else
             k_matvec<TX, TW, TW2, false, false>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, T.nb, nullptr);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, nullptr);
     } else {
         if (partials)
             k_matvec<TX, TW, TW2, true, true>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, T.nb, T.d_partials);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials);
         else
-            k_matvec<TX, TW, TW2, true, false><<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, T.nb,
+            k_matvec<TX, TW, TW2, true, false><<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb,
                                                                              nullptr);
     }
     MGPU_LAUNCHED();
@@ -618,26 +607,53 @@ int launch_matvec_f32(Problem &pb, const float *d_x, float *d_y, bool partials) 
 // ---------------------------------------------------------------------------
 constexpr int kPtBlock = 256;
 
+// Workgroups follow the map segments (one segment per workgroup, first pixel
+// wg_q0[wg]), so the fused dot partials are those of the PCG map kernels.
 template <typename TY, typename TW, typename TW2, bool W0, bool DOT>
 __global__ __launch_bounds__(kPtBlock) void k_trmatvec(const int64_t *__restrict__ base, const int *__restrict__ ct,
                                                        const TW *__restrict__ cw0, const TW2 *__restrict__ cw12,
-                                                       const TY *__restrict__ y, double *__restrict__ x, int npix,
-                                                       const double *__restrict__ hvec,
+                                                       const TY *__restrict__ y, double *__restrict__ x,
+                                                       const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
+                                                       const int *__restrict__ seg_q, const double *__restrict__ hvec,
                                                        double *__restrict__ dotpart) {
     __shared__ double red[kPtBlock / 64];
-    const int qq = blockIdx.x * kPtBlock + threadIdx.x;
+    const int qq = wg_q0[blockIdx.x] + (int) threadIdx.x;
+    const int qend = seg_q[wg_seg[blockIdx.x] + 1];
     const int s = qq / kSlice, lane = qq % kSlice;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    if (qq < npix) {
+    if (qq < qend) {
+        // The slice width is uniform over the wave: every lane walks all of it,
+        // its padding entries (t = -1, weights 0) adding +0.0, which leaves the
+        // sums unchanged (they start at +0.0 and so are never -0.0).  Four hits
+        // per step keep eight independent loads in flight per lane.
         const int64_t b0 = base[s];
-        const int64_t width = (base[s + 1] - b0) / kSlice;
-        for (int64_t i = 0; i < width; i++) {
-            const int64_t e = b0 + i * kSlice + lane;
-            const int t = ct[e];
-            if (t < 0) break;  // padding: hits of this lane are exhausted
-            const TW2 w = cw12[e];
-            const double yt = (double) y[t];
-            a0 += (W0 ? (double) cw0[e] * yt : yt);
+        const int width = (int) ((base[s + 1] - b0) / kSlice);
+        const int *__restrict__ cte = ct + b0 + lane;
+        const TW2 *__restrict__ cwe = cw12 + b0 + lane;
+        int i = 0;
+        for (; i + 4 <= width; i += 4) {
+            int t[4];
+            TW2 w[4];
+            double yt[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                t[u] = cte[(i + u) * kSlice];
+                w[u] = cwe[(i + u) * kSlice];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) yt[u] = t[u] >= 0 ? (double) y[t[u]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                a0 += (W0 ? (double) cw0[b0 + lane + (int64_t) (i + u) * kSlice] * yt[u] : yt[u]);
+                a1 += (double) w[u].x * yt[u];
+                a2 += (double) w[u].y * yt[u];
+            }
+        }
+        for (; i < width; i++) {
+            const int t = cte[i * kSlice];
+            const TW2 w = cwe[i * kSlice];
+            const double yt = t >= 0 ? (double) y[t] : 0.0;
+            a0 += (W0 ? (double) cw0[b0 + lane + (int64_t) i * kSlice] * yt : yt);
             a1 += (double) w.x * yt;
             a2 += (double) w.y * yt;
         }
@@ -647,7 +663,7 @@ __global__ __launch_bounds__(kPtBlock) void k_trmatvec(const int64_t *__restrict
     }
     if (DOT) {
         double d = 0.0;
-        if (qq < npix) {
+        if (qq < qend) {
             d += hvec[3 * qq] * a0;
             d += hvec[3 * qq + 1] * a1;
             d += hvec[3 * qq + 2] * a2;
@@ -657,31 +673,35 @@ __global__ __launch_bounds__(kPtBlock) void k_trmatvec(const int64_t *__restrict
     }
 }
 
-int trmatvec_nparts(const Problem &pb) { return (int) grid_for(pb.pt.npix, kPtBlock); }
+int trmatvec_nparts(const Problem &pb) { return pb.sg.nwg; }
 
 template <typename TY, typename TW, typename TW2>
 static int launch_trmatvec_t(Problem &pb, const TY *d_y, double *d_x, const double *d_h, double *d_dot,
                              const TW *cw0, const TW2 *cw12, int *nparts) {
+    static_assert(kPtBlock == kMapWG, "P^T workgroups are the map-domain workgroups");
+    MGPU_RET(segments_ensure(pb));
     const Pointing &P = pb.pt;
-    const unsigned g = grid_for(P.npix, kPtBlock);
+    const Segments &G = pb.sg;
+    const unsigned g = (unsigned) G.nwg;
     if (nparts) *nparts = (int) g;
     if (g == 0) return MGPU_OK;
     const bool dot = d_h && d_dot;
     const int64_t *b = P.d_ell_base;
+    const int *ws = G.d_wg_seg, *wq = G.d_wg_q0, *sq = G.d_q;
     if (P.unit_w0) {
         if (dot)
             k_trmatvec<TY, TW, TW2, false, true>
-                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, P.npix, d_h, d_dot);
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, ws, wq, sq, d_h, d_dot);
         else
             k_trmatvec<TY, TW, TW2, false, false>
-                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, P.npix, nullptr, nullptr);
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, ws, wq, sq, nullptr, nullptr);
     } else {
         if (dot)
             k_trmatvec<TY, TW, TW2, true, true>
-                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, P.npix, d_h, d_dot);
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, ws, wq, sq, d_h, d_dot);
         else
             k_trmatvec<TY, TW, TW2, true, false>
-                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, P.npix, nullptr, nullptr);
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, ws, wq, sq, nullptr, nullptr);
     }
     MGPU_LAUNCHED();
     return MGPU_OK;

@@ -35,7 +35,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <unordered_map>
 
 namespace mgpu {
@@ -369,8 +373,8 @@ __device__ __forceinline__ void spectral_self(R &pr, R &pi, Cx<R> c) {
     pi = npi;
 }
 
-template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO>
-__global__ __launch_bounds__((1 << LOGNC) >> LOGV) void k_tiles(TileArgs a) {
+template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO, int MINW>
+__global__ __launch_bounds__((1 << LOGNC) >> LOGV, MINW) void k_tiles(TileArgs a) {
     constexpr int NC = 1 << LOGNC, V = 1 << LOGV, NT = NC / V;
     constexpr int NP = n_passes(LOGNC, LOGV, LOGR);
     constexpr int LR0 = fwd_bits(LOGNC, LOGV, LOGR, 0);
@@ -400,21 +404,40 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV) void k_tiles(TileArgs a) {
     const int nblk = (int) (d.blk_hi - d.blk_lo);
     const int rel0 = (int) (in0 - d.blk_lo);
 
+    // interior tile (the whole input window inside the block, uniform per
+    // workgroup): plain loads and stores without clamping or masking
+    const bool interior = rel0 >= 0 && rel0 + 2 * NC <= nblk;
+
     R re[V], im[V];
     // first forward pass inputs: z[n] = x[2n] + i x[2n+1], n = j + r NC/RAD0
+    if (interior) {
+        const R *__restrict__ yt = yb + rel0;
 #pragma unroll
-    for (int b = 0; b < B0; b++) {
-        const int j = t + b * NT;
+        for (int b = 0; b < B0; b++) {
+            const int j = t + b * NT;
 #pragma unroll
-        for (int r = 0; r < RAD0; r++) {
-            const int n = j + r * (NC / RAD0);
-            const int i0 = rel0 + 2 * n, i1 = i0 + 1;
-            const R v0 = yb[(unsigned) min(max(i0, 0), nblk - 1)];
-            const R v1 = yb[(unsigned) min(max(i1, 0), nblk - 1)];
-            re[b * RAD0 + r] = ((unsigned) i0 < (unsigned) nblk) ? v0 - mean : (R) 0;
-            im[b * RAD0 + r] = ((unsigned) i1 < (unsigned) nblk) ? v1 - mean : (R) 0;
+            for (int r = 0; r < RAD0; r++) {
+                const int n = j + r * (NC / RAD0);
+                re[b * RAD0 + r] = yt[2 * n] - mean;
+                im[b * RAD0 + r] = yt[2 * n + 1] - mean;
+            }
+            if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
         }
-        if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+    } else {
+#pragma unroll
+        for (int b = 0; b < B0; b++) {
+            const int j = t + b * NT;
+#pragma unroll
+            for (int r = 0; r < RAD0; r++) {
+                const int n = j + r * (NC / RAD0);
+                const int i0 = rel0 + 2 * n, i1 = i0 + 1;
+                const R v0 = yb[(unsigned) min(max(i0, 0), nblk - 1)];
+                const R v1 = yb[(unsigned) min(max(i1, 0), nblk - 1)];
+                re[b * RAD0 + r] = ((unsigned) i0 < (unsigned) nblk) ? v0 - mean : (R) 0;
+                im[b * RAD0 + r] = ((unsigned) i1 < (unsigned) nblk) ? v1 - mean : (R) 0;
+            }
+            if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+        }
     }
     Passes<R, LOGNC, LOGV, LOGR, TWO, false, 0>::run(re, im, lds, tw, t);
 
@@ -447,16 +470,33 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV) void k_tiles(TileArgs a) {
     // last inverse pass (radix RAD0, not paired): position n = j + r NC/RAD0 holds
     // conj(y[2n] + i y[2n+1]); tile sample i -> block offset rel0 + i
     const int lo = a.lambda - 1, hi = a.lambda - 1 + a.L;
+    if (interior) {
+        // tile sample i is an output iff (unsigned) (i - lo) < L; it lies inside the block
+        R *__restrict__ zt = zb + rel0;
+        const unsigned L = (unsigned) a.L;
 #pragma unroll
-    for (int b = 0; b < B0; b++) {
-        const int j = t + b * NT;
+        for (int b = 0; b < B0; b++) {
+            const int j = t + b * NT;
 #pragma unroll
-        for (int r = 0; r < RAD0; r++) {
-            const int n = j + r * (NC / RAD0);
-            const int i0 = 2 * n;
-            const int o0 = rel0 + i0;
-            if (i0 >= lo && i0 < hi && o0 < nblk) zb[(unsigned) o0] = re[b * RAD0 + r] - mean_out;
-            if (i0 + 1 >= lo && i0 + 1 < hi && o0 + 1 < nblk) zb[(unsigned) (o0 + 1)] = -im[b * RAD0 + r] - mean_out;
+            for (int r = 0; r < RAD0; r++) {
+                const int i0 = 2 * (j + r * (NC / RAD0));
+                if ((unsigned) (i0 - lo) < L) zt[i0] = re[b * RAD0 + r] - mean_out;
+                if ((unsigned) (i0 + 1 - lo) < L) zt[i0 + 1] = -im[b * RAD0 + r] - mean_out;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < B0; b++) {
+            const int j = t + b * NT;
+#pragma unroll
+            for (int r = 0; r < RAD0; r++) {
+                const int n = j + r * (NC / RAD0);
+                const int i0 = 2 * n;
+                const int o0 = rel0 + i0;
+                if (i0 >= lo && i0 < hi && o0 < nblk) zb[(unsigned) o0] = re[b * RAD0 + r] - mean_out;
+                if (i0 + 1 >= lo && i0 + 1 < hi && o0 + 1 < nblk)
+                    zb[(unsigned) (o0 + 1)] = -im[b * RAD0 + r] - mean_out;
+            }
         }
     }
     (void) NP;
@@ -480,12 +520,13 @@ __device__ __forceinline__ int find_block_t(const int64_t *__restrict__ start, i
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_partials(const R *__restrict__ y, int64_t m,
-                                                  const int64_t *__restrict__ start, int nb,
+                                                  const int64_t *__restrict__ start,
+                                                  const int *__restrict__ chunk_block, int nb,
                                                   double *__restrict__ partials) {
     __shared__ double red[4];
     const int64_t c = blockIdx.x, c0 = c * kChunk;
     const int64_t c1 = min(c0 + (int64_t) kChunk, m);
-    int b = find_block_t(start, nb, c0);
+    int b = chunk_block[c];
     for (; b < nb && start[b] < c1; b++) {
         const int64_t lo = start[b], hi = start[b + 1];
         double s = 0.0;
@@ -731,6 +772,19 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
 
     T.nchunks = (T.m + kChunk - 1) / kChunk;
     MGPU_RET(pb.arena.alloc(&T.d_partials, (size_t) (T.nchunks + nb)));
+    {
+        // first block overlapping each chunk of kChunk samples (nb past the covered range)
+        std::vector<int> cb((size_t) T.nchunks);
+        int b = 0;
+        for (int64_t c = 0; c < T.nchunks; c++) {
+            const int64_t t0 = c * kChunk;
+            while (b < nb && T.h_start[b + 1] <= t0) b++;
+            cb[(size_t) c] = b;
+        }
+        MGPU_RET(pb.arena.alloc(&T.d_chunk_block, (size_t) T.nchunks));
+        MGPU_HIP(hipMemcpyAsync(T.d_chunk_block, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice, s));
+        MGPU_HIP(hipStreamSynchronize(s));
+    }
     MGPU_RET(pb.arena.alloc(&T.d_mean, (size_t) nb));
     MGPU_RET(pb.arena.alloc(&T.d_mean_out, (size_t) nb));
     MGPU_HIP(hipMemsetAsync(T.d_mean, 0, sizeof(double) * nb, s));
@@ -879,17 +933,74 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
     return MGPU_OK;
 }
 
-template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO>
+// ---------------------------------------------------------------------------
+// tile kernel variants and their autotuning
+// ---------------------------------------------------------------------------
+// A variant fixes the values per lane (2^LOGV), the largest radix of a pass
+// (2^LOGR), whether the real and imaginary planes are exchanged through LDS
+// together (TWO) and the minimum resident waves per SIMD requested from the
+// register allocator (MINW; 4 with 512-lane tiles lets two workgroups share a
+// CU).  Variant 1 of every length is the default shape.
+struct TileShape {
+    int logv, logr;
+    bool two;
+    int minw;
+};
+
+static const TileShape kShapes64_9[] = {{3, 3, true, 1}};
+static const TileShape kShapes64_mid[] = {{4, 4, true, 1}, {3, 3, true, 1}};
+static const TileShape kShapes64_14[] = {{5, 4, false, 1}, {5, 5, false, 1}, {4, 4, false, 1}};
+static const TileShape kShapes32_14[] = {{5, 4, true, 1}, {5, 5, true, 1}, {4, 4, true, 1}, {5, 4, false, 4}};
+
+static const TileShape *tile_shapes(bool fp32, int log2n, int *n) {
+    const int lognc = log2n - 1;
+#define MGPU_SHAPES(a)                          \
+    do {                                        \
+        *n = (int) (sizeof(a) / sizeof(a[0])); \
+        return a;                               \
+    } while (0)
+    if (lognc == 9) MGPU_SHAPES(kShapes64_9);
+    if (lognc >= 10 && lognc <= 13) MGPU_SHAPES(kShapes64_mid);
+    if (lognc == 14) {
+        if (fp32) MGPU_SHAPES(kShapes32_14);
+        MGPU_SHAPES(kShapes64_14);
+    }
+#undef MGPU_SHAPES
+    *n = 0;
+    return nullptr;
+}
+
+int tile_variant_count(bool fp32, int log2n) {
+    int n = 0;
+    tile_shapes(fp32, log2n, &n);
+    return n;
+}
+
+const char *tile_variant_desc(bool fp32, int log2n, int variant) {
+    static thread_local char buf[160];
+    int n = 0;
+    const TileShape *sh = tile_shapes(fp32, log2n, &n);
+    if (!sh || variant < 1 || variant > n) return "";
+    const TileShape &v = sh[variant - 1];
+    const int nc = 1 << (log2n - 1);
+    std::snprintf(buf, sizeof(buf),
+                  "%s N=%d: %d complex values per lane, %d lanes, radix <= %d, %s LDS exchange, >= %d waves/SIMD",
+                  fp32 ? "fp32" : "fp64", 1 << log2n, 1 << v.logv, nc >> v.logv, 1 << v.logr,
+                  v.two ? "two-plane" : "one-plane", v.minw);
+    return buf;
+}
+
+template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO, int MINW>
 static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *coef, const void *tw) {
     const ToeplitzPlan &T = pb.tp;
     constexpr int NC = 1 << LOGNC, NT = NC / (1 << LOGV);
     constexpr size_t lds = sizeof(R) * (size_t) lds_elems(NC) * (TWO ? 2 : 1);
     static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
-    auto kern = k_tiles<R, LOGNC, LOGV, LOGR, TWO>;
-    static bool attr_set = false;
-    if (!attr_set) {
+    auto kern = k_tiles<R, LOGNC, LOGV, LOGR, TWO, MINW>;
+    static unsigned attr_devices = 0;  // devices on which the LDS limit was raised
+    if (pb.device < 32 && !(attr_devices & (1u << pb.device))) {
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-        attr_set = true;
+        attr_devices |= 1u << pb.device;
     }
     TileArgs a{d_in, d_out, T.d_tiles, coef, tw, T.d_mean, T.d_mean_out, T.lambda, T.L};
     hipLaunchKernelGGL(kern, dim3(T.ntiles), dim3(NT), lds, pb.stream, a);
@@ -898,22 +1009,116 @@ static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *co
 }
 
 template <typename R>
-static int launch_tiles(Problem &pb, const R *d_in, R *d_out) {
+static int launch_tiles(Problem &pb, const R *d_in, R *d_out, int variant) {
     const ToeplitzPlan &T = pb.tp;
     constexpr bool F32 = sizeof(R) == 4;
     const void *coef = F32 ? (const void *) T.d_coef32 : (const void *) T.d_coef64;
     const void *tw = F32 ? (const void *) T.d_tw32 : (const void *) T.d_tw64;
+#define MGPU_TL(LOGNC, LOGV, LOGR, TWO, MINW) \
+    return launch_tiles_cfg<R, LOGNC, LOGV, LOGR, TWO, MINW>(pb, d_in, d_out, coef, tw)
     switch (T.log2n - 1) {  // LOGNC
-        case 9: return launch_tiles_cfg<R, 9, 3, 3, true>(pb, d_in, d_out, coef, tw);
-        case 10: return launch_tiles_cfg<R, 10, 4, 4, true>(pb, d_in, d_out, coef, tw);
-        case 11: return launch_tiles_cfg<R, 11, 4, 4, true>(pb, d_in, d_out, coef, tw);
-        case 12: return launch_tiles_cfg<R, 12, 4, 4, true>(pb, d_in, d_out, coef, tw);
-        case 13: return launch_tiles_cfg<R, 13, 4, 4, true>(pb, d_in, d_out, coef, tw);
+        case 9: MGPU_TL(9, 3, 3, true, 1);
+        case 10:
+            if (variant == 2) MGPU_TL(10, 3, 3, true, 1);
+            MGPU_TL(10, 4, 4, true, 1);
+        case 11:
+            if (variant == 2) MGPU_TL(11, 3, 3, true, 1);
+            MGPU_TL(11, 4, 4, true, 1);
+        case 12:
+            if (variant == 2) MGPU_TL(12, 3, 3, true, 1);
+            MGPU_TL(12, 4, 4, true, 1);
+        case 13:
+            if (variant == 2) MGPU_TL(13, 3, 3, true, 1);
+            MGPU_TL(13, 4, 4, true, 1);
         case 14:
-            if constexpr (F32) return launch_tiles_cfg<R, 14, TILE14_F32_LOGV, TILE14_F32_LOGR, true>(pb, d_in, d_out, coef, tw);
-            else return launch_tiles_cfg<R, 14, TILE14_F64_LOGV, TILE14_F64_LOGR, false>(pb, d_in, d_out, coef, tw);
+            if constexpr (F32) {
+                if (variant == 2) MGPU_TL(14, 5, 5, true, 1);
+                if (variant == 3) MGPU_TL(14, 4, 4, true, 1);
+                if (variant == 4) MGPU_TL(14, 5, 4, false, 4);
+                MGPU_TL(14, 5, 4, true, 1);
+            } else {
+                if (variant == 2) MGPU_TL(14, 5, 5, false, 1);
+                if (variant == 3) MGPU_TL(14, 4, 4, false, 1);
+                MGPU_TL(14, 5, 4, false, 1);
+            }
         default: set_error("unsupported tile length 2^%d", T.log2n); return MGPU_EINVAL;
     }
+#undef MGPU_TL
+}
+
+// Process-wide autotuning results: (device, precision, tile length) -> variant.
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int>, int> g_tune;
+
+// Choose the tile variant of precision R for this plan: the forced one, the
+// cached result, or the fastest of 1 warm-up + 3 timed launches of each
+// variant on the caller's buffers (d_out is overwritten by every launch; the
+// caller launches the winner afterwards).
+template <typename R>
+static int pick_variant(Problem &pb, const R *d_in, R *d_out) {
+    ToeplitzPlan &T = pb.tp;
+    const int slot = sizeof(R) == 4 ? 1 : 0;
+    if (T.variant[slot]) return MGPU_OK;
+    const int nv = tile_variant_count(slot == 1, T.log2n);
+    MGPU_ARG(nv > 0, "unsupported tile length 2^%d", T.log2n);
+    // a forced variant applies to the precision of the PCG iterations (FP32 under
+    // MGPU_MIXED); the FP64 products of a mixed solve (right-hand side, residual
+    // refreshes) are autotuned
+    const int main_slot = pb.opt.precision == MGPU_MIXED ? 1 : 0;
+    if (pb.opt.tile_variant > 0 && slot == main_slot) {
+        MGPU_ARG(pb.opt.tile_variant <= nv, "tile_variant=%d: the %s tile of length 2^%d has %d variant(s)",
+                 pb.opt.tile_variant, slot ? "fp32" : "fp64", T.log2n, nv);
+        T.variant[slot] = pb.opt.tile_variant;
+        return MGPU_OK;
+    }
+    if (nv == 1) {
+        T.variant[slot] = 1;
+        return MGPU_OK;
+    }
+    const auto key = std::make_tuple(pb.device, slot, T.log2n);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            T.variant[slot] = it->second;
+            return MGPU_OK;
+        }
+    }
+    hipEvent_t e0, e1;
+    MGPU_HIP(hipEventCreate(&e0));
+    MGPU_HIP(hipEventCreate(&e1));
+    int best = 1;
+    float best_ms = 0.f;
+    int rc = MGPU_OK;
+    for (int v = 1; v <= nv && rc == MGPU_OK; v++) {
+        rc = launch_tiles<R>(pb, d_in, d_out, v);
+        if (rc != MGPU_OK) break;
+        if (hipEventRecord(e0, pb.stream) != hipSuccess) {
+            set_error("tile autotuning: hipEventRecord failed");
+            rc = MGPU_EHIP;
+            break;
+        }
+        for (int k = 0; k < 3 && rc == MGPU_OK; k++) rc = launch_tiles<R>(pb, d_in, d_out, v);
+        if (rc != MGPU_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, pb.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            set_error("tile autotuning: event timing failed");
+            rc = MGPU_EHIP;
+            break;
+        }
+        if (v == 1 || ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    if (rc != MGPU_OK) return rc;
+    T.variant[slot] = best;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = best;
+    return MGPU_OK;
 }
 
 template <typename R>
@@ -928,7 +1133,8 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
     if (T.offset_mode != MGPU_OFFSET_NONE) {  // MGPU_OFFSET_NONE: means stay 0
         pb.prof.begin(MGPU_KT_MEAN, s);
         if (compute_partials) {
-            k_partials<R><<<(unsigned) T.nchunks, 256, 0, s>>>(d_in, T.m, T.d_start, T.nb, T.d_partials);
+            k_partials<R><<<(unsigned) T.nchunks, 256, 0, s>>>(d_in, T.m, T.d_start, T.d_chunk_block, T.nb,
+                                                               T.d_partials);
             MGPU_LAUNCHED();
         }
         k_block_mean<R><<<T.nb, 256, 0, s>>>(T.d_partials, T.d_start, T.nb, T.d_mean, d_in, T.d_tail,
@@ -938,8 +1144,9 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
         pb.prof.end(s);
     }
     if (T.ntiles) {
+        MGPU_RET(pick_variant<R>(pb, d_in, d_out));
         pb.prof.begin(MGPU_KT_TILES, s);
-        MGPU_RET(launch_tiles<R>(pb, d_in, d_out));
+        MGPU_RET(launch_tiles<R>(pb, d_in, d_out, T.variant[sizeof(R) == 4 ? 1 : 0]));
         pb.prof.end(s);
     }
     if (!T.h_direct_blocks.empty()) {

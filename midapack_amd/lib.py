@@ -34,7 +34,8 @@ EXPORTS = [
     "mappraiser_gpu_problem_create_device", "mappraiser_gpu_problem_destroy",
     "mappraiser_gpu_local_shape", "mappraiser_gpu_matvec", "mappraiser_gpu_trmatvec",
     "mappraiser_gpu_apply_weights", "mappraiser_gpu_precond_build", "mappraiser_gpu_precond_apply",
-    "mappraiser_gpu_pcg", "mappraiser_gpu_solve_device", "mappraiser_gpu_kernel_times", "mappraiser_gpu_pcg_gls",
+    "mappraiser_gpu_pcg", "mappraiser_gpu_solve_device", "mappraiser_gpu_kernel_times",
+    "mappraiser_gpu_tile_variants", "mappraiser_gpu_pcg_batch", "mappraiser_gpu_pcg_gls",
 ]
 
 # kernel classes of mappraiser_gpu_kernel_times (MGPU_KT_*)
@@ -44,17 +45,25 @@ KERNEL_CLASSES = ["P", "mean", "tiles", "Pt", "map", "refresh"]
 class Options(ct.Structure):
     _fields_ = [("precision", ct.c_int), ("refresh_every", ct.c_int), ("tile_log2", ct.c_int),
                 ("poll_every", ct.c_int), ("profile", ct.c_int), ("offset_mode", ct.c_int),
-                ("reserved", ct.c_int * 2)]
+                ("tile_variant", ct.c_int), ("graph", ct.c_int)]
 
 
 class Report(ct.Structure):
     _fields_ = [("iterations", ct.c_int), ("converged", ct.c_int), ("res0", ct.c_double),
                 ("res", ct.c_double), ("n_map", ct.c_int), ("lcount", ct.c_int), ("trash_pix", ct.c_int),
                 ("n_cut", ct.c_int), ("t_setup_ms", ct.c_double), ("t_solve_ms", ct.c_double),
-                ("true_res_rel", ct.c_double), ("refreshes", ct.c_int), ("pad", ct.c_int)]
+                ("true_res_rel", ct.c_double), ("refreshes", ct.c_int), ("tile_variant", ct.c_int)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchItem(ct.Structure):
+    """mgpu_batch_item of include/mappraiser_gpu.h"""
+    _fields_ = [("m", ct.c_int), ("pix", _vp), ("pixweights", _vp), ("signal", _vp), ("noise", _vp),
+                ("nb", ct.c_int), ("local_blocks_sizes", _vp), ("lam", ct.c_int), ("invtt", _vp),
+                ("x", _vp), ("pixels", _vp), ("cond", _vp), ("lhits", _vp), ("x_cap", ct.c_int),
+                ("rep", Report)]
 
 
 class GPUError(RuntimeError):
@@ -92,6 +101,10 @@ def load():
     L.mappraiser_gpu_solve_device.argtypes = [_vp, _vp, _vp, ct.c_double, ct.c_int, _f64p, _vp, _vp, _vp,
                                               ct.POINTER(Report)]
     L.mappraiser_gpu_kernel_times.argtypes = [_vp, _vp, _vp, ct.c_int]
+    L.mappraiser_gpu_tile_variants.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.POINTER(ct.c_int), ct.c_char_p,
+                                               ct.c_int]
+    L.mappraiser_gpu_pcg_batch.argtypes = [ct.c_int, ct.POINTER(BatchItem), ct.c_int, ct.c_int, ct.c_double,
+                                           ct.c_int, ct.POINTER(Options)]
     L.mappraiser_gpu_pcg_gls.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p,
                                          ct.c_int, _i32p, ct.c_int, _f64p, ct.c_double, ct.c_int, _f64p, _f64p,
                                          _i32p, _i32p, ct.POINTER(Options), ct.POINTER(Report)]
@@ -109,14 +122,31 @@ def _check(rc: int, what: str):
 
 
 def options(precision=MGPU_FP64, refresh_every=10, tile_log2=0, poll_every=8, profile=False,
-            offset_mode=None) -> Options:
+            offset_mode=None, tile_variant=0, graph=None) -> Options:
     o = Options()
     load().mappraiser_gpu_options_default(ct.byref(o))
     o.precision, o.refresh_every, o.tile_log2, o.poll_every = precision, refresh_every, tile_log2, poll_every
     o.profile = int(bool(profile))
+    o.tile_variant = int(tile_variant)
+    if graph is not None:
+        o.graph = int(bool(graph))
     if offset_mode is not None:
         o.offset_mode = int(offset_mode)
     return o
+
+
+def tile_variants(precision: int, tile_log2: int) -> list[str]:
+    """Descriptions of the tile kernel variants for a precision and real tile length 2^tile_log2."""
+    L = load()
+    n = ct.c_int(0)
+    _check(L.mappraiser_gpu_tile_variants(int(precision), int(tile_log2), 0, ct.byref(n), None, 0), "tile_variants")
+    out = []
+    for v in range(1, n.value + 1):
+        buf = ct.create_string_buffer(256)
+        _check(L.mappraiser_gpu_tile_variants(int(precision), int(tile_log2), v, ct.byref(n), buf, 256),
+               "tile_variants")
+        out.append(buf.value.decode())
+    return out
 
 
 def init(device: int = 0):
@@ -296,3 +326,53 @@ def pcg_gls(pix, weights, signal, noise, block_sizes, lam, invtt, tol=1e-8, maxi
     return GLSResult(x=x[:n].copy(), lindices=lind[: rep.lcount].copy(), cond=cond[: n // 3].copy(),
                      hits=hits[: n // 3].copy(), res_hist=hist, report=rep.as_dict(), pix_local=p, weights=w,
                      signal=s)
+
+
+@dataclass
+class BatchResult:
+    x: np.ndarray        # (I,Q,U) per kept pixel, pixel order
+    pixels: np.ndarray   # global pixel index of each kept pixel
+    cond: np.ndarray
+    hits: np.ndarray
+    report: dict
+
+
+def pcg_batch(items, grid_npix, tol=1e-8, maxiter=500, opts: Options | None = None) -> list[BatchResult]:
+    """Solve independent problems together (mappraiser_gpu_pcg_batch).  Each item
+    is a dict with pix, weights, signal, noise, block_sizes, lam, invtt (host
+    arrays, the dtypes of pcg_gls)."""
+    L = load()
+    n = len(items)
+    arr = (BatchItem * n)()
+    keep = []
+    outs = []
+    for i, it in enumerate(items):
+        p = _c(it["pix"], np.int32)
+        w = _c(it["weights"], np.float64)
+        s = _c(it["signal"], np.float64)
+        nz = _c(it["noise"], np.float64)
+        bs = _c(it["block_sizes"], np.int32)
+        inv = _c(it["invtt"], np.float64)
+        m = p.size // 3
+        cap = 3 * min(int(grid_npix), m)
+        x = np.zeros(max(cap, 3), np.float64)
+        px = np.zeros(max(cap // 3, 1), np.int32)
+        cond = np.zeros(max(cap // 3, 1), np.float64)
+        hits = np.zeros(max(cap // 3, 1), np.int32)
+        keep += [p, w, s, nz, bs, inv]
+        outs.append((x, px, cond, hits))
+        b = arr[i]
+        b.m, b.pix, b.pixweights, b.signal, b.noise = m, p.ctypes.data, w.ctypes.data, s.ctypes.data, nz.ctypes.data
+        b.nb, b.local_blocks_sizes, b.lam, b.invtt = bs.size, bs.ctypes.data, int(it["lam"]), inv.ctypes.data
+        b.x, b.pixels, b.cond, b.lhits, b.x_cap = x.ctypes.data, px.ctypes.data, cond.ctypes.data, hits.ctypes.data, x.size
+    o = opts or options()
+    _check(L.mappraiser_gpu_pcg_batch(n, arr, int(grid_npix), 0, float(tol), int(maxiter), ct.byref(o)),
+           "pcg_batch")
+    res = []
+    for i in range(n):
+        x, px, cond, hits = outs[i]
+        r = arr[i].rep
+        k = r.n_map // 3
+        res.append(BatchResult(x=x[: 3 * k].copy(), pixels=px[:k].copy(), cond=cond[:k].copy(), hits=hits[:k].copy(),
+                               report=r.as_dict()))
+    return res

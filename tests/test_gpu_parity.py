@@ -247,6 +247,71 @@ def test_dropin_side_effects_match_oracle(gpu, fixture_rank1, tmp_path):
     assert _rel(g.x, r.x) <= 1e-6
 
 
+def _variant_cases():
+    # (precision, real tile length) of the lambda = 1024 and lambda = 8192 workloads
+    from midapack_amd import lib as G
+    out = []
+    for prec, tag in ((G.MGPU_FP64, "fp64"), (G.MGPU_MIXED, "fp32")):
+        for l2 in (12, 15):
+            for v in range(1, len(G.tile_variants(prec, l2)) + 1):
+                out.append((prec, l2, v, f"{tag}-N2^{l2}-v{v}"))
+    return out
+
+
+VARIANT_CASES = _variant_cases()
+
+
+@pytest.mark.parametrize("case", [c for c in VARIANT_CASES if c[0] == 0], ids=lambda c: c[3])
+def test_ninv_tile_variants_match_closed_form(gpu, case):
+    # every FP64 tile kernel variant gives the closed-form product (interior and edge tiles)
+    _, l2, v, name = case
+    print(f"case: {name}", flush=True)
+    lam = 1024 if l2 == 12 else 8192
+    blocks = np.array([131072, 50001, 3 * lam], np.int32)
+    m = int(blocks.sum())
+    rng = np.random.default_rng(l2 * 10 + v)
+    invtt = np.concatenate([W.invtt_block(int(n), lam, 1.0, 0.2, 1.5) for n in blocks])
+    pix = np.repeat(3 * (np.arange(m) % 97), 3).astype(np.int32) + np.tile(np.arange(3, dtype=np.int32), m)
+    w = np.tile([1.0, 0.3, -0.2], m)
+    x = rng.standard_normal(m) + 0.5
+    opts = gpu.options(tile_log2=l2, tile_variant=v, offset_mode=gpu.MGPU_OFFSET_SYMMETRIC)
+    with gpu.Problem(pix, w, blocks, lam, invtt, opts=opts) as pb:
+        out = pb.apply_weights(x)
+    with O.ninv_mode(O.NINV_DIRECT), O.offset_mode(O.OFFSET_SYMMETRIC):
+        ref = O.apply_weights(x, blocks, lam, invtt)
+    assert _rel(out, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("case", [c for c in VARIANT_CASES if c[0] == 1], ids=lambda c: c[3])
+def test_mixed_tile_variants_map_error(gpu, case):
+    # every FP32 tile kernel variant: mixed-precision map within 1e-6 of the FP64 one
+    _, l2, v, name = case
+    print(f"case: {name}", flush=True)
+    lam = 1024 if l2 == 12 else 8192
+    wl = W.make_workload("variant", 4, 1 << 17, 128, 128, lam, 7, block_len=1 << 16)
+    g64 = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                      maxiter=400, opts=gpu.options(tile_log2=l2, offset_mode=gpu.MGPU_OFFSET_SYMMETRIC))
+    gm = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                     maxiter=400, opts=gpu.options(precision=gpu.MGPU_MIXED, tile_log2=l2, tile_variant=v,
+                                                   offset_mode=gpu.MGPU_OFFSET_SYMMETRIC))
+    err = _rel(gm.x, g64.x)
+    print(f"{name}: {gm.report['iterations']} iterations (FP64 {g64.report['iterations']}), error {err:.2e}",
+          flush=True)
+    assert g64.report["converged"] and gm.report["converged"]
+    assert gm.report["tile_variant"] == v
+    assert err <= 1e-6
+
+
+def test_tile_autotune_picks_a_variant(gpu):
+    wl = W.make_workload("autotune", 2, 1 << 17, 64, 64, 8192, 3, block_len=1 << 17)
+    print("case: autotuned tile variant, lambda 8192", flush=True)
+    for prec in (gpu.MGPU_FP64, gpu.MGPU_MIXED):
+        g = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                        maxiter=50, opts=gpu.options(precision=prec, offset_mode=gpu.MGPU_OFFSET_SYMMETRIC))
+        nv = len(gpu.tile_variants(prec, 15))
+        assert 1 <= g.report["tile_variant"] <= nv
+
+
 def test_mixed_precision_map_error(gpu):
     wl = W.config_c1()
     print("case: C1 mixed precision", flush=True)
