@@ -59,6 +59,10 @@ def main():
     ap.add_argument("--offset", choices=["symmetric", "reference", "none"], default="symmetric",
                     help="per-block offset handling of N^-1 for the timed solves")
     ap.add_argument("--no-reference-probe", action="store_true")
+    ap.add_argument("--c4-real", type=int, default=32, help="batched Monte Carlo realizations (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=12, help="requests per serving rate (0: skip)")
+    ap.add_argument("--serve-rates", default="1,2,5,10", help="Poisson arrival rates (req/s)")
+    ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded requests cycled by the load")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs one rank's share on one device "
@@ -190,7 +194,124 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args, reps[-1]["iterations"])
+    del d_pix, d_w, d_sig0, d_sig, d_noise, d_inv, wl
+    torch.cuda.empty_cache()
+    if args.c4_real > 0:
+        out["batch_c4"] = c4_section(G, W, torch, args)
+    if args.serve_requests > 0:
+        out["serve_c5"] = serve_section(G, W, args)
     print(json.dumps(out), flush=True)
+
+
+def c4_section(G, W, torch, args):
+    """Batched Monte Carlo (C4): independent realizations solved as segmented
+    batches (one per lambda and tile length), inputs resident in HBM.  Compared
+    with the plain FP64 path solving the realizations one after the other."""
+    t = time.time()
+    reals = W.config_c4(nreal=args.c4_real)
+    total = int(sum(r.m for r in reals))
+    log(f"C4: generated {len(reals)} realizations, {total} samples in {time.time() - t:.1f}s")
+    dev = torch.device("cuda:0")
+    items = [dict(pix=torch.from_numpy(r.pix).to(dev), weights=torch.from_numpy(r.weights).to(dev),
+                  signal=torch.from_numpy(r.signal).to(dev), noise=torch.from_numpy(r.noise).to(dev),
+                  invtt=torch.from_numpy(r.invtt).to(dev), block_sizes=r.block_sizes, lam=r.lam) for r in reals]
+    grid = reals[0].nx * reals[0].ny
+    del reals
+    torch.cuda.synchronize()
+    sym = G.MGPU_OFFSET_SYMMETRIC
+    res = {}
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    o64 = G.options(precision=G.MGPU_FP64, offset_mode=sym, graph=1)
+    omx = G.options(precision=G.MGPU_MIXED, offset_mode=sym, graph=1)
+    solo, t_solo = timed(lambda: [G.pcg_batch([it], grid, tol=args.tol, maxiter=args.maxiter, opts=o64)[0]
+                                  for it in items])
+    log(f"C4: FP64 one realization at a time {t_solo:.2f}s")
+    b64, t64 = timed(lambda: G.pcg_batch(items, grid, tol=args.tol, maxiter=args.maxiter, opts=o64))
+    log(f"C4: FP64 batched {t64:.2f}s")
+    bmx, tmx = timed(lambda: G.pcg_batch(items, grid, tol=args.tol, maxiter=args.maxiter, opts=omx))
+    log(f"C4: mixed batched {tmx:.2f}s")
+    same = all(np.array_equal(a.x, b.x) for a, b in zip(solo, b64))
+    err = max(float(np.max(np.abs(a.x - b.x)) / np.max(np.abs(a.x))) for a, b in zip(b64, bmx))
+    it64 = [r.report["iterations"] for r in b64]
+    itmx = [r.report["iterations"] for r in bmx]
+    res.update(realizations=len(items), samples=total,
+               lambdas={str(l): int(sum(1 for it in items if it["lam"] == l)) for l in (1024, 4096, 8192)},
+               fp64_sequential_s=round(t_solo, 3), fp64_batched_s=round(t64, 3), mixed_batched_s=round(tmx, 3),
+               maps_per_s_mixed_batched=round(len(items) / tmx, 3),
+               speedup_mixed_batched_vs_fp64_sequential=round(t_solo / tmx, 3),
+               speedup_fp64_batched_vs_sequential=round(t_solo / t64, 3),
+               batched_fp64_bit_identical_to_solo=bool(same), max_map_rel_err_mixed_vs_fp64=err,
+               all_converged=bool(all(r.report["converged"] for r in b64 + bmx)),
+               iterations_fp64=[min(it64), int(np.median(it64)), max(it64)],
+               iterations_mixed=[min(itmx), int(np.median(itmx)), max(itmx)])
+    del items
+    torch.cuda.empty_cache()
+    return res
+
+
+def serve_section(G, W, args):
+    """Serving (C5): Poisson arrivals of seeded requests (4-128 det x 2^16-2^20
+    samples, 256^2-1024^2 grids, lambda 1024-8192) into the request loop,
+    mixed precision; latency from submission to the map on the host."""
+    import threading
+
+    t = time.time()
+    pool = [W.c5_request(5, i) for i in range(args.serve_pool)]
+    reqs = [dict(pix=w.pix, weights=w.weights, signal=w.signal, noise=w.noise, block_sizes=w.block_sizes,
+                 lam=w.lam, invtt=w.invtt) for w in pool]
+    grids = [w.nx * w.ny for w in pool]
+    log(f"serve: generated {len(pool)} requests ({sum(w.m for w in pool)} samples) in {time.time() - t:.1f}s")
+    del pool
+    sym = G.MGPU_OFFSET_SYMMETRIC
+    # FP64 reference maps of the pool, one solo solve each
+    ref = [G.pcg_batch([r], g, tol=args.tol, maxiter=args.maxiter,
+                       opts=G.options(precision=G.MGPU_FP64, offset_mode=sym))[0] for r, g in zip(reqs, grids)]
+    omx = G.options(precision=G.MGPU_MIXED, offset_mode=sym, graph=1)
+    out = {"requests_per_rate": args.serve_requests, "pool": len(reqs), "precision": "mixed", "rates": []}
+    rng = np.random.default_rng(55)
+    for rate in [float(v) for v in args.serve_rates.split(",")]:
+        n = args.serve_requests
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+        tickets = [None] * n
+        with G.Server(hbm_budget_gb=0, max_slots=4, max_batch=8, opts=omx) as srv:
+            t0 = time.perf_counter()
+
+            def feed():
+                for i in range(n):
+                    dt = arrivals[i] - (time.perf_counter() - t0)
+                    if dt > 0:
+                        time.sleep(dt)
+                    j = i % len(reqs)
+                    tickets[i] = (j, srv.submit(reqs[j], grids[j], tol=args.tol, maxiter=args.maxiter))
+
+            th = threading.Thread(target=feed)
+            th.start()
+            lat, errs = [], []
+            for i in range(n):
+                while tickets[i] is None:
+                    time.sleep(0.001)
+                j, tk = tickets[i]
+                r, l_ms, _ = srv.wait(tk)
+                lat.append(l_ms)
+                a = ref[j].x
+                errs.append(float(np.max(np.abs(r.x - a)) / np.max(np.abs(a))))
+            th.join()
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        ent = {"rate": rate, "p50_ms": round(float(np.percentile(lat, 50)), 1),
+               "p99_ms": round(float(np.percentile(lat, 99)), 1), "maps_per_s": round(n / wall, 3),
+               "max_map_rel_err_vs_fp64": max(errs), "batched_groups": st["groups"],
+               "peak_concurrent_solves": st["peak_running"], "peak_hbm_gb": round(st["peak_hbm_gb"], 2)}
+        log(f"serve rate {rate}: p50 {ent['p50_ms']} ms, p99 {ent['p99_ms']} ms, {ent['maps_per_s']} maps/s")
+        out["rates"].append(ent)
+    return out
 
 
 def cpu_baseline(wl, args, gpu_iters):

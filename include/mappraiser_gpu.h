@@ -178,13 +178,13 @@ int mappraiser_gpu_tile_variants(int precision, int tile_log2, int variant, int 
  * One item is one independent GLS problem (a Monte Carlo noise realization, a
  * detector subset, a serving request) with its own pointing, signal, noise,
  * Toeplitz blocks (their sizes must sum to m) and lambda.  Items of equal
- * lambda are solved together as one segmented problem: every kernel of a PCG
- * iteration is launched once for all of them, each item keeps its own PCG
- * scalars and stop test, and converged items are frozen while the others go
- * on.  With the same tile length (opt->tile_log2) and tile variant, each
+ * lambda and equal Toeplitz tile length (the one each would use alone) are
+ * solved together as one segmented problem: every kernel of a PCG iteration
+ * is launched once for all of them, each item keeps its own PCG scalars and
+ * stop test, and converged items are frozen while the others go on.  Each
  * item's map, iteration count and residuals are bit-identical to its solo
- * solve with the same options.  Groups of different lambda run concurrently
- * on their own streams.  No reference counterpart: the reference solves one
+ * solve with the same options (tile variants are autotuned once per process).
+ * Groups run concurrently on their own streams.  No reference counterpart: the reference solves one
  * problem per MLmap call (mappraiser.c:34-232). */
 typedef struct mgpu_batch_item {
     /* inputs: host memory, or device memory when inputs_on_device != 0 */
@@ -208,6 +208,30 @@ typedef struct mgpu_batch_item {
 
 int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int grid_npix, int inputs_on_device, double tol,
                              int maxiter, const mgpu_options *opt);
+
+/* ---- serving: request loop with HBM-budget admission and continuous batching ----
+ * Requests are batch items (host arrays, owned by the caller until
+ * server_wait returns for their ticket).  A dispatcher admits the oldest
+ * queued request as soon as a slot is free and its estimated device footprint
+ * fits what the running solves leave of the HBM budget, together with every
+ * later queued request of the same solve key (lambda, tile length, grid,
+ * tol, maxiter) that fits, up to max_batch; each admitted group runs as one
+ * mappraiser_gpu_pcg_batch on its own stream and host thread, so solves
+ * admitted at different times overlap on the device.  Results are
+ * bit-identical to the requests' solo solves.  hbm_budget_gb <= 0 uses 80% of
+ * the free device memory.  No reference counterpart. */
+typedef struct mgpu_server mgpu_server;
+
+int mappraiser_gpu_server_create(mgpu_server **out, double hbm_budget_gb, int max_slots, int max_batch,
+                                 const mgpu_options *opt);
+int mappraiser_gpu_server_submit(mgpu_server *s, mgpu_batch_item *item, int grid_npix, double tol, int maxiter,
+                                 long long *ticket);
+/* blocks until the request is solved; latency = done - submitted, queue = admitted - submitted */
+int mappraiser_gpu_server_wait(mgpu_server *s, long long ticket, double *latency_ms, double *queue_ms);
+int mappraiser_gpu_server_stats(mgpu_server *s, long long *admitted, long long *groups, double *peak_gb,
+                                int *peak_running);
+/* finishes every queued and running request, then frees the server */
+int mappraiser_gpu_server_destroy(mgpu_server *s);
 
 /* Drop-in for PCG_GLS_true(outpath, ref, &A, &Nm1, x, signal, noise, cond,
  * lhits, tol, maxiter, precond=0, Z_2lvl) at mappraiser.c:211 for one rank,

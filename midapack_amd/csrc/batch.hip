@@ -1,6 +1,7 @@
 // batch.hip -- many independent GLS problems solved as one segmented problem.
 //
-// Items of equal lambda are concatenated into one problem:
+// Items of equal lambda and equal tile length (the one each would use alone)
+// are concatenated into one problem:
 //   * samples: item r occupies [off_r, off_r + m_r) of the TOD, off_r a multiple
 //     of kChunk, followed by padding samples up to the next multiple.  Padding
 //     samples are flagged (-3,-2,-1), carry weights (1,0,0) and zero signal and
@@ -60,6 +61,7 @@ struct EventPair {
 
 struct Group {
     int lambda = 0;
+    int tile_log2 = 0;       // tile length every item of the group uses alone
     std::vector<int> items;  // indices into the caller's array
     int rc = MGPU_OK;
     std::string err;
@@ -84,8 +86,13 @@ int solve_group(Group &gr, mgpu_batch_item *items, int grid_npix, bool on_dev, d
     MGPU_ARG(3 * (int64_t) grid_npix * R < (int64_t) INT32_MAX - 3,
              "%d items x grid_npix %d exceed the int32 pixel index range: split the batch", R, grid_npix);
 
+    // the tile length each item would use alone (same block-length profile class)
+    mgpu_options gopt;
+    mappraiser_gpu_options_default(&gopt);
+    if (opt) gopt = *opt;
+    gopt.tile_log2 = gr.tile_log2;
     std::unique_ptr<Problem> pb;
-    MGPU_RET(new_problem(pb, opt));
+    MGPU_RET(new_problem(pb, &gopt));
     Problem &P = *pb;
     hipStream_t s = P.stream;
     EventPair ev;
@@ -205,19 +212,23 @@ extern "C" int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int 
     MGPU_ARG(nitems > 0 && items, "empty batch");
     MGPU_ARG(grid_npix > 0, "grid_npix=%d must be positive", grid_npix);
     MGPU_ARG(maxiter >= 1, "maxiter=%d must be >= 1", maxiter);
-    std::map<int, Group> groups;
+    std::map<std::pair<int, int>, Group> groups;
     for (int i = 0; i < nitems; i++) {
         const mgpu_batch_item &it = items[i];
         MGPU_ARG(it.pix && it.pixweights && it.signal && it.noise && it.invtt, "item %d: NULL input", i);
         MGPU_RET(check_common(it.m, 3, it.nb, it.local_blocks_sizes, it.lambda));
-        int64_t tot = 0;
+        int64_t tot = 0, nmax = 0;
         for (int b = 0; b < it.nb; b++) {
             MGPU_ARG(it.local_blocks_sizes[b] > 0, "item %d: block %d has size %d", i, b, it.local_blocks_sizes[b]);
             tot += it.local_blocks_sizes[b];
+            nmax = std::max<int64_t>(nmax, it.local_blocks_sizes[b]);
         }
         MGPU_ARG(tot == it.m, "item %d: block sizes sum to %lld, not m = %d", i, (long long) tot, it.m);
-        Group &g = groups[it.lambda];
+        // items are grouped by lambda and by the tile length they would use alone
+        const int l2 = toeplitz_tile_log2(it.lambda, nmax, opt ? opt->tile_log2 : 0);
+        Group &g = groups[std::make_pair(it.lambda, l2)];
         g.lambda = it.lambda;
+        g.tile_log2 = l2;
         g.items.push_back(i);
     }
     std::vector<Group *> gl;
@@ -236,7 +247,7 @@ extern "C" int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int 
     }
     for (Group *g : gl)
         if (g->rc != MGPU_OK) {
-            set_error("lambda=%d group: %s", g->lambda, g->err.c_str());
+            set_error("lambda=%d, tile 2^%d group: %s", g->lambda, g->tile_log2, g->err.c_str());
             return g->rc;
         }
     return MGPU_OK;

@@ -348,6 +348,8 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
 int toeplitz_apply(Problem &pb, const double *d_in, double *d_out, bool compute_partials);
 int toeplitz_apply_f32(Problem &pb, const float *d_in, float *d_out, bool compute_partials);
 int tile_variant_count(bool fp32, int log2n);
+// real tile length 2^k the plan of a lambda / longest block would use (0: no tiles)
+int toeplitz_tile_log2(int lambda, int64_t nmax, int forced);
 const char *tile_variant_desc(bool fp32, int log2n, int variant);
 
 int precond_build(Problem &pb, double *d_signal);

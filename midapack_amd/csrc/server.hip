@@ -1,0 +1,254 @@
+// server.hip -- map-making request loop with HBM-budget admission and
+// continuous batching.
+//
+// Requests (mgpu_batch_item, host arrays owned by the caller until wait
+// returns) enter a FIFO queue.  A dispatcher thread admits the queue head as
+// soon as a slot is free and its estimated device footprint fits the HBM
+// budget that running solves leave; with it, it admits every later queued
+// request that shares its solve key (lambda, Toeplitz tile length, grid,
+// tolerance, iteration cap) and still fits, up to max_batch.  An admitted
+// group runs as one segmented batch solve (batch.hip) on its own stream and
+// host thread, so solves admitted at different times overlap on the device;
+// when it finishes, its memory and slot return to the pool and the queue is
+// scanned again.  Each result is bit-identical to the request's solo solve:
+// batching changes neither the kernels nor the summation order of an item.
+#include "mgpu_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <list>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace mgpu {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    long long ticket = 0;
+    mgpu_batch_item *item = nullptr;
+    double tol = 1e-8;
+    int maxiter = 500;
+    int grid = 0;
+    int lambda = 0, tile_log2 = 0;
+    int64_t bytes = 0;
+    Clock::time_point t_submit, t_start, t_done;
+    bool done = false;
+    int status = MGPU_OK;
+    std::string err;
+};
+
+struct Worker {
+    std::thread th;
+    bool finished = false;
+};
+
+}  // namespace
+
+struct Server {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request *> pending;
+    std::unordered_map<long long, std::unique_ptr<Request>> reqs;
+    std::list<Worker> workers;
+    int64_t budget = 0, used = 0;
+    int max_slots = 4, max_batch = 16, running = 0;
+    long long next_ticket = 1;
+    bool stop = false;
+    mgpu_options opt{};
+    std::thread dispatcher;
+    // statistics
+    long long admitted = 0, groups = 0;
+    int64_t peak_used = 0;
+    int peak_running = 0;
+
+    void run_dispatcher();
+    void run_group(std::vector<Request *> grp, Worker *self);
+};
+
+// Device bytes of one solve: inputs staged on the device (pixels, weights,
+// signal, noise), pointing tables (q, Q/U weights, sliced ELL with its sort
+// keys during the build), TOD work vectors in both precisions, and the map
+// work vectors and 3x3 blocks per pixel of the grid.
+static int64_t request_bytes(const mgpu_batch_item &it, int grid, bool mixed) {
+    const int64_t per_sample = 12 + 24 + 8 + 8 + 4 + 16 + 24 + 16 + 12 + (mixed ? 8 + 8 + 8 : 0);
+    const int64_t per_pixel = 8 * 3 * 9 + 8 * 9 + 8 + 4 + 4 * 4;
+    return (int64_t) it.m * per_sample + (int64_t) std::min<int64_t>(grid, it.m) * per_pixel + (64 << 20);
+}
+
+void Server::run_group(std::vector<Request *> grp, Worker *self) {
+    std::vector<mgpu_batch_item> items(grp.size());
+    for (size_t i = 0; i < grp.size(); i++) items[i] = *grp[i]->item;
+    const Clock::time_point t0 = Clock::now();
+    for (Request *r : grp) r->t_start = t0;
+    int rc = mappraiser_gpu_pcg_batch((int) items.size(), items.data(), grp[0]->grid, 0, grp[0]->tol,
+                                      grp[0]->maxiter, &opt);
+    std::string err = rc == MGPU_OK ? std::string() : std::string(mappraiser_gpu_last_error());
+    const Clock::time_point t1 = Clock::now();
+    std::lock_guard<std::mutex> g(mu);
+    for (size_t i = 0; i < grp.size(); i++) {
+        Request *r = grp[i];
+        r->item->rep = items[i].rep;
+        r->status = rc;
+        r->err = err;
+        r->t_done = t1;
+        r->done = true;
+        used -= r->bytes;
+    }
+    running--;
+    self->finished = true;
+    cv.notify_all();
+}
+
+void Server::run_dispatcher() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        // reap finished worker threads
+        for (auto it = workers.begin(); it != workers.end();) {
+            if (it->finished) {
+                it->th.join();
+                it = workers.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        auto fits = [&](const Request *r) { return running == 0 || used + r->bytes <= budget; };
+        if (stop && pending.empty() && running == 0) break;
+        if (pending.empty() || running >= max_slots || !fits(pending.front())) {
+            cv.wait(lk);
+            continue;
+        }
+        // admit the head and every later request with the same solve key that fits
+        std::vector<Request *> grp;
+        Request *head = pending.front();
+        pending.pop_front();
+        grp.push_back(head);
+        used += head->bytes;
+        for (auto it = pending.begin(); it != pending.end() && (int) grp.size() < max_batch;) {
+            Request *r = *it;
+            const bool same = r->lambda == head->lambda && r->tile_log2 == head->tile_log2 &&
+                              r->grid == head->grid && r->tol == head->tol && r->maxiter == head->maxiter;
+            if (same && used + r->bytes <= budget) {
+                grp.push_back(r);
+                used += r->bytes;
+                it = pending.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        running++;
+        admitted += (long long) grp.size();
+        groups++;
+        peak_used = std::max(peak_used, used);
+        peak_running = std::max(peak_running, running);
+        workers.emplace_back();
+        Worker *w = &workers.back();
+        w->th = std::thread(&Server::run_group, this, grp, w);
+    }
+    for (auto &w : workers) w.th.join();
+    workers.clear();
+}
+
+}  // namespace mgpu
+
+using namespace mgpu;
+
+extern "C" {
+
+int mappraiser_gpu_server_create(mgpu_server **out, double hbm_budget_gb, int max_slots, int max_batch,
+                                 const mgpu_options *opt) {
+    MGPU_ARG(out, "NULL argument");
+    MGPU_ARG(max_slots >= 1 && max_batch >= 1, "max_slots=%d / max_batch=%d must be >= 1", max_slots, max_batch);
+    std::unique_ptr<Server> s(new Server());
+    mappraiser_gpu_options_default(&s->opt);
+    if (opt) s->opt = *opt;
+    s->opt.profile = 0;
+    if (hbm_budget_gb > 0) {
+        s->budget = (int64_t) (hbm_budget_gb * 1e9);
+    } else {
+        size_t free_b = 0, total_b = 0;
+        MGPU_HIP(hipMemGetInfo(&free_b, &total_b));
+        s->budget = (int64_t) (0.8 * (double) free_b);
+    }
+    s->max_slots = max_slots;
+    s->max_batch = max_batch;
+    Server *raw = s.get();
+    s->dispatcher = std::thread(&Server::run_dispatcher, raw);
+    *out = reinterpret_cast<mgpu_server *>(s.release());
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_server_submit(mgpu_server *h, mgpu_batch_item *item, int grid_npix, double tol, int maxiter,
+                                 long long *ticket) {
+    MGPU_ARG(h && item && ticket, "NULL argument");
+    MGPU_ARG(grid_npix > 0 && maxiter >= 1, "grid_npix=%d / maxiter=%d", grid_npix, maxiter);
+    MGPU_ARG(item->local_blocks_sizes && item->nb > 0 && item->m > 0, "invalid request");
+    Server &s = *reinterpret_cast<Server *>(h);
+    int64_t nmax = 0;
+    for (int b = 0; b < item->nb; b++) nmax = std::max<int64_t>(nmax, item->local_blocks_sizes[b]);
+    std::unique_ptr<Request> r(new Request());
+    r->item = item;
+    r->tol = tol;
+    r->maxiter = maxiter;
+    r->grid = grid_npix;
+    r->lambda = item->lambda;
+    r->tile_log2 = toeplitz_tile_log2(item->lambda, nmax, s.opt.tile_log2);
+    r->bytes = request_bytes(*item, grid_npix, s.opt.precision == MGPU_MIXED);
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> g(s.mu);
+    MGPU_ARG(!s.stop, "server is shutting down");
+    r->ticket = s.next_ticket++;
+    *ticket = r->ticket;
+    s.pending.push_back(r.get());
+    s.reqs[r->ticket] = std::move(r);
+    s.cv.notify_all();
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_server_wait(mgpu_server *h, long long ticket, double *latency_ms, double *queue_ms) {
+    MGPU_ARG(h, "NULL handle");
+    Server &s = *reinterpret_cast<Server *>(h);
+    std::unique_lock<std::mutex> lk(s.mu);
+    auto it = s.reqs.find(ticket);
+    MGPU_ARG(it != s.reqs.end(), "unknown ticket %lld", ticket);
+    Request *r = it->second.get();
+    s.cv.wait(lk, [&] { return r->done; });
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (queue_ms) *queue_ms = std::chrono::duration<double, std::milli>(r->t_start - r->t_submit).count();
+    const int rc = r->status;
+    const std::string err = r->err;
+    s.reqs.erase(it);
+    if (rc != MGPU_OK) set_error("%s", err.c_str());
+    return rc;
+}
+
+int mappraiser_gpu_server_stats(mgpu_server *h, long long *admitted, long long *groups, double *peak_gb,
+                                int *peak_running) {
+    MGPU_ARG(h, "NULL handle");
+    Server &s = *reinterpret_cast<Server *>(h);
+    std::lock_guard<std::mutex> g(s.mu);
+    if (admitted) *admitted = s.admitted;
+    if (groups) *groups = s.groups;
+    if (peak_gb) *peak_gb = (double) s.peak_used / 1e9;
+    if (peak_running) *peak_running = s.peak_running;
+    return MGPU_OK;
+}
+
+int mappraiser_gpu_server_destroy(mgpu_server *h) {
+    if (!h) return MGPU_OK;
+    Server *s = reinterpret_cast<Server *>(h);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    s->dispatcher.join();
+    delete s;
+    return MGPU_OK;
+}
+
+}  // extern "C"

@@ -716,6 +716,12 @@ static int max_log2n(bool fp32) { return fp32 ? 16 : 15; }
 static int min_log2n() { return 10; }
 
 // tile length choice: minimise N (log2 N + c) / L over the sizes that fit LDS
+static int choose_log2n(int lambda, int64_t nmax, bool fp32, int forced);
+
+int toeplitz_tile_log2(int lambda, int64_t nmax, int forced) {
+    return lambda > 1 ? choose_log2n(lambda, nmax, false, forced) : 0;
+}
+
 static int choose_log2n(int lambda, int64_t nmax, bool fp32, int forced) {
     if (forced > 0) return forced;
     int best = -1;

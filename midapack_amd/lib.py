@@ -35,7 +35,9 @@ EXPORTS = [
     "mappraiser_gpu_local_shape", "mappraiser_gpu_matvec", "mappraiser_gpu_trmatvec",
     "mappraiser_gpu_apply_weights", "mappraiser_gpu_precond_build", "mappraiser_gpu_precond_apply",
     "mappraiser_gpu_pcg", "mappraiser_gpu_solve_device", "mappraiser_gpu_kernel_times",
-    "mappraiser_gpu_tile_variants", "mappraiser_gpu_pcg_batch", "mappraiser_gpu_pcg_gls",
+    "mappraiser_gpu_tile_variants", "mappraiser_gpu_pcg_batch", "mappraiser_gpu_server_create",
+    "mappraiser_gpu_server_submit", "mappraiser_gpu_server_wait", "mappraiser_gpu_server_stats",
+    "mappraiser_gpu_server_destroy", "mappraiser_gpu_pcg_gls",
 ]
 
 # kernel classes of mappraiser_gpu_kernel_times (MGPU_KT_*)
@@ -105,6 +107,14 @@ def load():
                                                ct.c_int]
     L.mappraiser_gpu_pcg_batch.argtypes = [ct.c_int, ct.POINTER(BatchItem), ct.c_int, ct.c_int, ct.c_double,
                                            ct.c_int, ct.POINTER(Options)]
+    L.mappraiser_gpu_server_create.argtypes = [ct.POINTER(_vp), ct.c_double, ct.c_int, ct.c_int,
+                                               ct.POINTER(Options)]
+    L.mappraiser_gpu_server_submit.argtypes = [_vp, ct.POINTER(BatchItem), ct.c_int, ct.c_double, ct.c_int,
+                                               ct.POINTER(ct.c_longlong)]
+    L.mappraiser_gpu_server_wait.argtypes = [_vp, ct.c_longlong, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]
+    L.mappraiser_gpu_server_stats.argtypes = [_vp, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong),
+                                              ct.POINTER(ct.c_double), ct.POINTER(ct.c_int)]
+    L.mappraiser_gpu_server_destroy.argtypes = [_vp]
     L.mappraiser_gpu_pcg_gls.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p,
                                          ct.c_int, _i32p, ct.c_int, _f64p, ct.c_double, ct.c_int, _f64p, _f64p,
                                          _i32p, _i32p, ct.POINTER(Options), ct.POINTER(Report)]
@@ -339,40 +349,102 @@ class BatchResult:
 
 def pcg_batch(items, grid_npix, tol=1e-8, maxiter=500, opts: Options | None = None) -> list[BatchResult]:
     """Solve independent problems together (mappraiser_gpu_pcg_batch).  Each item
-    is a dict with pix, weights, signal, noise, block_sizes, lam, invtt (host
-    arrays, the dtypes of pcg_gls)."""
+    is a dict with pix, weights, signal, noise, block_sizes, lam, invtt: numpy
+    arrays with the dtypes of pcg_gls, or (all items alike) device tensors with
+    a data_ptr() (int32 pix, float64 others; block_sizes stays a host array)."""
     L = load()
     n = len(items)
     arr = (BatchItem * n)()
     keep = []
     outs = []
+    on_dev = hasattr(items[0]["pix"], "data_ptr")
     for i, it in enumerate(items):
-        p = _c(it["pix"], np.int32)
-        w = _c(it["weights"], np.float64)
-        s = _c(it["signal"], np.float64)
-        nz = _c(it["noise"], np.float64)
-        bs = _c(it["block_sizes"], np.int32)
-        inv = _c(it["invtt"], np.float64)
-        m = p.size // 3
-        cap = 3 * min(int(grid_npix), m)
-        x = np.zeros(max(cap, 3), np.float64)
-        px = np.zeros(max(cap // 3, 1), np.int32)
-        cond = np.zeros(max(cap // 3, 1), np.float64)
-        hits = np.zeros(max(cap // 3, 1), np.int32)
-        keep += [p, w, s, nz, bs, inv]
-        outs.append((x, px, cond, hits))
-        b = arr[i]
-        b.m, b.pix, b.pixweights, b.signal, b.noise = m, p.ctypes.data, w.ctypes.data, s.ctypes.data, nz.ctypes.data
-        b.nb, b.local_blocks_sizes, b.lam, b.invtt = bs.size, bs.ctypes.data, int(it["lam"]), inv.ctypes.data
-        b.x, b.pixels, b.cond, b.lhits, b.x_cap = x.ctypes.data, px.ctypes.data, cond.ctypes.data, hits.ctypes.data, x.size
+        k, o = _fill_item(arr[i], it, grid_npix)
+        keep.append(k)
+        outs.append(o)
     o = opts or options()
-    _check(L.mappraiser_gpu_pcg_batch(n, arr, int(grid_npix), 0, float(tol), int(maxiter), ct.byref(o)),
+    _check(L.mappraiser_gpu_pcg_batch(n, arr, int(grid_npix), int(on_dev), float(tol), int(maxiter), ct.byref(o)),
            "pcg_batch")
-    res = []
-    for i in range(n):
-        x, px, cond, hits = outs[i]
-        r = arr[i].rep
-        k = r.n_map // 3
-        res.append(BatchResult(x=x[: 3 * k].copy(), pixels=px[:k].copy(), cond=cond[:k].copy(), hits=hits[:k].copy(),
-                               report=r.as_dict()))
-    return res
+    return [_item_result(arr[i], outs[i]) for i in range(n)]
+
+
+def _ptr(a, dt):
+    """(pointer, element count, object to keep alive) of a host array or a device tensor"""
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr(), a.numel(), a
+    h = _c(a, dt)
+    return h.ctypes.data, h.size, h
+
+
+def _fill_item(b: BatchItem, it: dict, grid_npix: int):
+    """Point a BatchItem at one problem's arrays; returns (inputs to keep alive, outputs)."""
+    p, npix3, kp = _ptr(it["pix"], np.int32)
+    w, _, kw = _ptr(it["weights"], np.float64)
+    s, _, ks = _ptr(it["signal"], np.float64)
+    nz, _, kn = _ptr(it["noise"], np.float64)
+    inv, _, ki = _ptr(it["invtt"], np.float64)
+    bs = _c(it["block_sizes"], np.int32)
+    m = npix3 // 3
+    cap = 3 * min(int(grid_npix), m)
+    x = np.zeros(max(cap, 3), np.float64)
+    px = np.zeros(max(cap // 3, 1), np.int32)
+    cond = np.zeros(max(cap // 3, 1), np.float64)
+    hits = np.zeros(max(cap // 3, 1), np.int32)
+    b.m, b.pix, b.pixweights, b.signal, b.noise = m, p, w, s, nz
+    b.nb, b.local_blocks_sizes, b.lam, b.invtt = bs.size, bs.ctypes.data, int(it["lam"]), inv
+    b.x, b.pixels, b.cond, b.lhits, b.x_cap = x.ctypes.data, px.ctypes.data, cond.ctypes.data, hits.ctypes.data, x.size
+    return (kp, kw, ks, kn, bs, ki), (x, px, cond, hits)
+
+
+def _item_result(b: BatchItem, outs) -> BatchResult:
+    x, px, cond, hits = outs
+    r = b.rep
+    k = r.n_map // 3
+    return BatchResult(x=x[: 3 * k].copy(), pixels=px[:k].copy(), cond=cond[:k].copy(), hits=hits[:k].copy(),
+                       report=r.as_dict())
+
+
+class Server:
+    """Request loop of mappraiser_gpu_server_*: submit() returns a ticket,
+    wait(ticket) returns (BatchResult, latency_ms, queue_ms)."""
+
+    def __init__(self, hbm_budget_gb=0.0, max_slots=4, max_batch=16, opts: Options | None = None):
+        L = load()
+        self._h = _vp()
+        self._opts = opts or options()
+        _check(L.mappraiser_gpu_server_create(ct.byref(self._h), float(hbm_budget_gb), int(max_slots),
+                                              int(max_batch), ct.byref(self._opts)), "server_create")
+        self._live = {}
+
+    def submit(self, item: dict, grid_npix: int, tol=1e-8, maxiter=500) -> int:
+        b = BatchItem()
+        keep, outs = _fill_item(b, item, grid_npix)
+        t = ct.c_longlong(0)
+        _check(load().mappraiser_gpu_server_submit(self._h, ct.byref(b), int(grid_npix), float(tol), int(maxiter),
+                                                   ct.byref(t)), "server_submit")
+        self._live[t.value] = (b, keep, outs)
+        return t.value
+
+    def wait(self, ticket: int):
+        lat, q = ct.c_double(0), ct.c_double(0)
+        rc = load().mappraiser_gpu_server_wait(self._h, int(ticket), ct.byref(lat), ct.byref(q))
+        b, keep, outs = self._live.pop(ticket)
+        _check(rc, "server_wait")
+        return _item_result(b, outs), lat.value, q.value
+
+    def stats(self) -> dict:
+        a, g, pk, pr = ct.c_longlong(0), ct.c_longlong(0), ct.c_double(0), ct.c_int(0)
+        _check(load().mappraiser_gpu_server_stats(self._h, ct.byref(a), ct.byref(g), ct.byref(pk), ct.byref(pr)),
+               "server_stats")
+        return {"admitted": a.value, "groups": g.value, "peak_hbm_gb": pk.value, "peak_running": pr.value}
+
+    def close(self):
+        if self._h:
+            load().mappraiser_gpu_server_destroy(self._h)
+            self._h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -82,3 +82,37 @@ def test_graph_replay_matches_eager_bitwise(gpu, precision):
     assert out[0].report["iterations"] == out[1].report["iterations"]
     assert out[0].report["refreshes"] == out[1].report["refreshes"]
     assert np.array_equal(out[0].x, out[1].x)
+
+
+def test_server_results_match_solo_bitwise(gpu):
+    """Requests submitted together are admitted as slots and memory free up,
+    same-key requests are batched; every result equals its solo solve."""
+    print("case: server, 6 requests (2 lambdas), 2 slots, batches of <= 3", flush=True)
+    wls = [W.make_workload(f"s{i}", 2 + i % 3, 20000 + 3000 * i, 32, 32, 128 if i % 2 else 256, 40 + i)
+           for i in range(6)]
+    opts = gpu.options(offset_mode=gpu.MGPU_OFFSET_SYMMETRIC)
+    with gpu.Server(hbm_budget_gb=0, max_slots=2, max_batch=3, opts=opts) as srv:
+        tickets = [srv.submit(_item(w), 32 * 32, tol=1e-8, maxiter=300) for w in wls]
+        got = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    print(f"server stats: {st}", flush=True)
+    assert st["admitted"] == 6 and st["groups"] < 6 and st["peak_running"] <= 2
+    for w, (r, lat, q) in zip(wls, got):
+        g = gpu.pcg_gls(w.pix, w.weights, w.signal, w.noise, w.block_sizes, w.lam, w.invtt, tol=1e-8,
+                        maxiter=300, opts=opts)
+        assert lat >= q >= 0.0
+        assert r.report["iterations"] == g.report["iterations"] and r.report["converged"]
+        np.testing.assert_array_equal(r.pixels, _solo_pixels(g))
+        assert np.array_equal(r.x, g.x), f"{w.name}: max diff {np.max(np.abs(r.x - g.x))}"
+
+
+def test_server_budget_serializes_solves(gpu):
+    # a budget smaller than any request admits one solve at a time (never zero)
+    print("case: server with a 1 MB budget", flush=True)
+    wls = [W.make_workload(f"t{i}", 2, 16384, 16, 16, 64, 60 + i) for i in range(3)]
+    with gpu.Server(hbm_budget_gb=1e-3, max_slots=4, max_batch=4) as srv:
+        tickets = [srv.submit(_item(w), 256, tol=1e-8, maxiter=200) for w in wls]
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["peak_running"] == 1 and st["groups"] == 3
+    assert all(r.report["converged"] for r, _, _ in res)
