@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=12, help="requests per serving rate (0: skip)")
     ap.add_argument("--serve-rates", default="1,2,5,10", help="Poisson arrival rates (req/s)")
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded requests cycled by the load")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy 1 GiB device to device once before the solves (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs one rank's share on one device "
@@ -86,6 +88,13 @@ def main():
     d_noise = torch.from_numpy(wl.noise).to(dev)
     d_inv = torch.from_numpy(wl.invtt).to(dev)
     torch.cuda.synchronize()
+
+    if args.copy_calibration:
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     offset = {"symmetric": G.MGPU_OFFSET_SYMMETRIC, "reference": G.MGPU_OFFSET_REFERENCE,
               "none": G.MGPU_OFFSET_NONE}[args.offset]
@@ -135,7 +144,7 @@ def main():
     dominant = max((k for k in ("P", "tiles", "Pt") if k in kernels), key=lambda k: kernels[k]["total_ms_per_map"])
     dk = kernels[dominant]
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": dk["GBps"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": dk["frac"], "traffic": None}
+                "unit": "GB/s", "frac": dk["frac"], "traffic": measured_traffic(dominant, args)}
 
     out = {
         "metric": "time-to-map at rel. residual 1e-8 (s)",
@@ -208,7 +217,7 @@ def c4_section(G, W, torch, args):
     batches (one per lambda and tile length), inputs resident in HBM.  Compared
     with the plain FP64 path solving the realizations one after the other."""
     t = time.time()
-    reals = W.config_c4(nreal=args.c4_real)
+    reals = W.config_c4(nreal=args.c4_real, progress=lambda name: log(f"C4: generated {name}"))
     total = int(sum(r.m for r in reals))
     log(f"C4: generated {len(reals)} realizations, {total} samples in {time.time() - t:.1f}s")
     dev = torch.device("cuda:0")
@@ -230,8 +239,14 @@ def c4_section(G, W, torch, args):
 
     o64 = G.options(precision=G.MGPU_FP64, offset_mode=sym, graph=1)
     omx = G.options(precision=G.MGPU_MIXED, offset_mode=sym, graph=1)
-    solo, t_solo = timed(lambda: [G.pcg_batch([it], grid, tol=args.tol, maxiter=args.maxiter, opts=o64)[0]
-                                  for it in items])
+    def one_by_one():
+        out = []
+        for i, it in enumerate(items):
+            out.append(G.pcg_batch([it], grid, tol=args.tol, maxiter=args.maxiter, opts=o64)[0])
+            log(f"C4: FP64 alone r{i}: {out[-1].report['iterations']} iterations")
+        return out
+
+    solo, t_solo = timed(one_by_one)
     log(f"C4: FP64 one realization at a time {t_solo:.2f}s")
     b64, t64 = timed(lambda: G.pcg_batch(items, grid, tol=args.tol, maxiter=args.maxiter, opts=o64))
     log(f"C4: FP64 batched {t64:.2f}s")
@@ -263,7 +278,10 @@ def serve_section(G, W, args):
     import threading
 
     t = time.time()
-    pool = [W.c5_request(5, i) for i in range(args.serve_pool)]
+    pool = []
+    for i in range(args.serve_pool):
+        pool.append(W.c5_request(5, i))
+        log(f"serve: generated {pool[-1].name}")
     reqs = [dict(pix=w.pix, weights=w.weights, signal=w.signal, noise=w.noise, block_sizes=w.block_sizes,
                  lam=w.lam, invtt=w.invtt) for w in pool]
     grids = [w.nx * w.ny for w in pool]
@@ -271,8 +289,11 @@ def serve_section(G, W, args):
     del pool
     sym = G.MGPU_OFFSET_SYMMETRIC
     # FP64 reference maps of the pool, one solo solve each
-    ref = [G.pcg_batch([r], g, tol=args.tol, maxiter=args.maxiter,
-                       opts=G.options(precision=G.MGPU_FP64, offset_mode=sym))[0] for r, g in zip(reqs, grids)]
+    ref = []
+    for r, g in zip(reqs, grids):
+        ref.append(G.pcg_batch([r], g, tol=args.tol, maxiter=args.maxiter,
+                               opts=G.options(precision=G.MGPU_FP64, offset_mode=sym))[0])
+        log(f"serve: FP64 reference map {len(ref)}: {ref[-1].report['iterations']} iterations")
     omx = G.options(precision=G.MGPU_MIXED, offset_mode=sym, graph=1)
     out = {"requests_per_rate": args.serve_requests, "pool": len(reqs), "precision": "mixed", "rates": []}
     rng = np.random.default_rng(55)
@@ -300,6 +321,7 @@ def serve_section(G, W, args):
                 j, tk = tickets[i]
                 r, l_ms, _ = srv.wait(tk)
                 lat.append(l_ms)
+                log(f"serve rate {rate}: request {i} done in {l_ms:.0f} ms")
                 a = ref[j].x
                 errs.append(float(np.max(np.abs(r.x - a)) / np.max(np.abs(a))))
             th.join()
@@ -312,6 +334,19 @@ def serve_section(G, W, args):
         log(f"serve rate {rate}: p50 {ent['p50_ms']} ms, p99 {ent['p99_ms']} ms, {ent['maps_per_s']} maps/s")
         out["rates"].append(ent)
     return out
+
+
+def measured_traffic(kernel_class, args):
+    """HBM bytes per launch of a kernel class on this workload, from the
+    calibrated FETCH_SIZE / WRITE_SIZE counter runs summarised in
+    profiles/traffic_c2.json (None when absent or for another workload)."""
+    fn = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if args.ndet != 64 or not os.path.exists(fn):
+        return None
+    with open(fn) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel_class)
+    return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
 
 
 def cpu_baseline(wl, args, gpu_iters):

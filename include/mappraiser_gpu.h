@@ -174,6 +174,23 @@ int mappraiser_gpu_kernel_times(mgpu_problem *pb, double *ms, long long *counts,
  * No reference counterpart (the reference's FFT is FFTW's, toeplitz.c:418-423). */
 int mappraiser_gpu_tile_variants(int precision, int tile_log2, int variant, int *count, char *desc, int desc_len);
 
+/* ---- noise-model builder ----
+ * compute_invtt / noise2invtt (TOAST_interface/utils.py:435-601) on the
+ * device: for every Toeplitz block of the noise timestream, a one-segment
+ * Welch periodogram (Hann window, linear detrend), a bounded fit of the
+ * log-PSD model a + log10(1 + ((f + fmin)/fknee)^alpha) (the reference's
+ * p0 and bounds, utils.py:530-538), the inverse PSD of the fit, its inverse
+ * FFT over the block length, and the apodisation window's first lambda
+ * points (utils.py:562-577).  noise: [sum of sizes]; invtt: [nb*lambda]; both
+ * host memory, or device memory when on_device != 0.  fit_params: host
+ * [nb*4] (a, alpha, fknee, fmin) or NULL.  The reference fits with scipy's
+ * trust-region-reflective least squares, this build with a bounded
+ * Levenberg-Marquardt iteration: the minimiser agrees to the fit tolerance. */
+#define MGPU_APOD_CHEBWIN 0   /* ifftshift(chebwin(2 lambda, 150 dB)) (the reference default) */
+#define MGPU_APOD_GAUSSIAN 1  /* general Gaussian, p = 1, sigma = lambda / 3 */
+int mappraiser_gpu_noise_invtt(const double *noise, int nb, const int *local_blocks_sizes, int lambda, double fsamp,
+                               int apod_window, int on_device, double *invtt, double *fit_params);
+
 /* ---- batched solves ----
  * One item is one independent GLS problem (a Monte Carlo noise realization, a
  * detector subset, a serving request) with its own pointing, signal, noise,

@@ -286,6 +286,31 @@ int mappraiser_gpu_kernel_times(mgpu_problem *h, double *ms, long long *counts, 
     return MGPU_OK;
 }
 
+int mappraiser_gpu_noise_invtt(const double *noise, int nb, const int *local_blocks_sizes, int lambda, double fsamp,
+                               int apod_window, int on_device, double *invtt, double *fit_params) {
+    g_err[0] = 0;
+    MGPU_ARG(noise && local_blocks_sizes && invtt && nb > 0, "NULL argument");
+    std::unique_ptr<Problem> pb;
+    MGPU_RET(new_problem(pb, nullptr));
+    int64_t m = 0;
+    for (int b = 0; b < nb; b++) m += local_blocks_sizes[b];
+    const double *d_noise = noise;
+    double *d_inv = invtt;
+    if (!on_device) {
+        double *dn, *di;
+        MGPU_RET(upload(*pb, &dn, noise, (size_t) m));
+        MGPU_RET(pb->arena.alloc(&di, (size_t) nb * lambda));
+        d_noise = dn;
+        d_inv = di;
+    }
+    MGPU_RET(noise_invtt(*pb, d_noise, nb, local_blocks_sizes, lambda, fsamp, apod_window, d_inv, fit_params));
+    if (!on_device)
+        MGPU_HIP(hipMemcpyAsync(invtt, d_inv, sizeof(double) * (size_t) nb * lambda, hipMemcpyDeviceToHost,
+                                pb->stream));
+    MGPU_HIP(hipStreamSynchronize(pb->stream));
+    return MGPU_OK;
+}
+
 int mappraiser_gpu_tile_variants(int precision, int tile_log2, int variant, int *count, char *desc, int desc_len) {
     MGPU_ARG(count, "NULL count");
     MGPU_ARG(precision == MGPU_FP64 || precision == MGPU_MIXED, "unknown precision %d", precision);

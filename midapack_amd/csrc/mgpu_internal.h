@@ -367,6 +367,9 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             std::vector<SegReport> *segs = nullptr);
 
 int ensure_work(Problem &pb);
+// invtt rows [nb*lambda] (device) from the noise timestream (device), see mappraiser_gpu_noise_invtt
+int noise_invtt(Problem &pb, const double *d_noise, int nb, const int *block_sizes, int lambda, double fsamp,
+                int window_kind, double *d_invtt, double *h_params);
 // a problem on the current device with its own stream and the given options
 int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt);
 // common argument checks of the problem constructors

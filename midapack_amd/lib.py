@@ -22,6 +22,8 @@ MGPU_MIXED = 1
 MGPU_OFFSET_REFERENCE = 0
 MGPU_OFFSET_SYMMETRIC = 1
 MGPU_OFFSET_NONE = 2
+MGPU_APOD_CHEBWIN = 0
+MGPU_APOD_GAUSSIAN = 1
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -37,7 +39,7 @@ EXPORTS = [
     "mappraiser_gpu_pcg", "mappraiser_gpu_solve_device", "mappraiser_gpu_kernel_times",
     "mappraiser_gpu_tile_variants", "mappraiser_gpu_pcg_batch", "mappraiser_gpu_server_create",
     "mappraiser_gpu_server_submit", "mappraiser_gpu_server_wait", "mappraiser_gpu_server_stats",
-    "mappraiser_gpu_server_destroy", "mappraiser_gpu_pcg_gls",
+    "mappraiser_gpu_server_destroy", "mappraiser_gpu_noise_invtt", "mappraiser_gpu_pcg_gls",
 ]
 
 # kernel classes of mappraiser_gpu_kernel_times (MGPU_KT_*)
@@ -115,6 +117,8 @@ def load():
     L.mappraiser_gpu_server_stats.argtypes = [_vp, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong),
                                               ct.POINTER(ct.c_double), ct.POINTER(ct.c_int)]
     L.mappraiser_gpu_server_destroy.argtypes = [_vp]
+    L.mappraiser_gpu_noise_invtt.argtypes = [_vp, ct.c_int, _i32p, ct.c_int, ct.c_double, ct.c_int, ct.c_int, _vp,
+                                             _vp]
     L.mappraiser_gpu_pcg_gls.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p,
                                          ct.c_int, _i32p, ct.c_int, _f64p, ct.c_double, ct.c_int, _f64p, _f64p,
                                          _i32p, _i32p, ct.POINTER(Options), ct.POINTER(Report)]
@@ -448,3 +452,14 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def noise_invtt(noise, block_sizes, lam, fsamp, apod_window=MGPU_APOD_CHEBWIN):
+    """compute_invtt on the device: returns (invtt [nb*lam], fit params [nb, 4] = (a, alpha, fknee, fmin))."""
+    nz = _c(noise, np.float64)
+    bs = _c(block_sizes, np.int32)
+    out = np.zeros(bs.size * int(lam), np.float64)
+    par = np.zeros((bs.size, 4), np.float64)
+    _check(load().mappraiser_gpu_noise_invtt(nz.ctypes.data, bs.size, bs, int(lam), float(fsamp), int(apod_window), 0,
+                                             out.ctypes.data, par.ctypes.data), "noise_invtt")
+    return out, par

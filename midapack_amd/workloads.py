@@ -62,61 +62,36 @@ def raster_pixels(nsamp, nx, ny, ox, oy):
     return (row * nx + col).astype(np.int64)
 
 
-_GPU_FFT_MIN = 1 << 16  # transforms at least this long use torch's device FFT when a GPU is present
+def _fast_len(n):
+    """Smallest 5-smooth length >= n: the FFT-derived fields of a length with a
+    large prime factor are computed on this length (an FFT of a prime length
+    costs ~10x more), then truncated."""
+    from scipy.fft import next_fast_len
 
-
-def _device_fft():
-    """torch (with a visible GPU) for long transforms, else None (numpy).  The
-    seeded draws are numpy's either way; FFT-derived fields agree between the
-    two backends to rounding and are reproducible for a given backend."""
-    global _TORCH
-    try:
-        return _TORCH
-    except NameError:
-        pass
-    _TORCH = None
-    try:
-        import torch
-        if torch.cuda.is_available():
-            _TORCH = torch
-    except ImportError:
-        pass
-    return _TORCH
-
-
-def _rfft_filter(x, amp, n):
-    """irfft(rfft(x) * amp, n) on the fastest available backend."""
-    th = _device_fft() if n >= _GPU_FFT_MIN else None
-    if th is None:
-        return np.fft.irfft(np.fft.rfft(x) * amp, n=n)
-    xt = th.from_numpy(x).to("cuda")
-    at = th.from_numpy(amp).to("cuda")
-    return th.fft.irfft(th.fft.rfft(xt) * at, n=n).cpu().numpy()
-
-
-def _irfft(spec, n):
-    th = _device_fft() if n >= _GPU_FFT_MIN else None
-    if th is None:
-        return np.fft.irfft(spec, n=n)
-    return th.fft.irfft(th.from_numpy(np.ascontiguousarray(spec)).to("cuda"), n=n).cpu().numpy()
+    return int(next_fast_len(int(n), real=True))
 
 
 def one_over_f_noise(rng, nsamp, sigma, fknee, alpha, fsamp=FSAMP):
-    w = rng.standard_normal(nsamp)
-    f = np.fft.rfftfreq(nsamp, 1.0 / fsamp)
+    """sigma * irfft(rfft(w) sqrt(1 + (fknee/f)^alpha)), w ~ N(0,1), on the fast
+    length >= nsamp, truncated to nsamp."""
+    n = _fast_len(nsamp)
+    w = rng.standard_normal(n)
+    f = np.fft.rfftfreq(n, 1.0 / fsamp)
     amp = np.zeros_like(f)
     amp[1:] = np.sqrt(1.0 + (fknee / f[1:]) ** alpha)
-    return sigma * _rfft_filter(w, amp, nsamp)
+    return sigma * np.fft.irfft(np.fft.rfft(w) * amp, n=n)[:nsamp]
 
 
 def invtt_block(nb_samp, lam, sigma, fknee, alpha, fsamp=FSAMP):
-    """Analytic noise2invtt (utils.py:550-580): first row of the inverse-noise band."""
+    """Analytic noise2invtt (utils.py:550-580): first row of the inverse-noise band
+    (the inverse PSD sampled on the fast length >= the block length)."""
     if lam > nb_samp // 2:
         raise ValueError("Bandwidth cannot be larger than timestream (utils.py:561).")
-    f = np.fft.rfftfreq(nb_samp, 1.0 / fsamp)
+    n = _fast_len(nb_samp)  # = nb_samp for the power-of-two blocks of C1/C2
+    f = np.fft.rfftfreq(n, 1.0 / fsamp)
     psdm1 = np.zeros_like(f)
     psdm1[1:] = 1.0 / (sigma**2 * (1.0 + (fknee / f[1:]) ** alpha))
-    inv_tt = _irfft(psdm1, nb_samp)
+    inv_tt = np.fft.irfft(psdm1, n=n)
     if lam == 1:
         return inv_tt[:1].copy()
     return np.multiply(_chebwin_half(lam), inv_tt[:lam], dtype=np.float64)
@@ -219,20 +194,28 @@ def config_small(seed=7, ndet=2, nsamp=4096, nx=16, ny=16, lam=64, block_len=Non
                          block_len=block_len, flag_frac=flag_frac, det_lengths=det_lengths)
 
 
-def config_c4(seed=4, nreal=32, nx=512, ny=512):
+def config_c4(seed=4, nreal=32, nx=512, ny=512, progress=None):
     """Batched Monte Carlo: nreal independent realizations on a shared 512^2
     grid, each with U{16..64} detectors, per-detector lengths floor(2^U(16,20))
     (one Toeplitz block per detector), lambda from {1024, 4096, 8192},
     fknee ~ U[0.05, 0.5] Hz, alpha ~ U[1, 2]."""
-    out = []
-    for r in range(nreal):
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(r):
         rr = _rng(seed, 100, r)
         ndet = int(rr.integers(16, 65))
         lens = np.floor(2.0 ** rr.uniform(16.0, 20.0, ndet)).astype(np.int64)
         lam = int(rr.choice([1024, 4096, 8192]))
-        out.append(make_workload(f"C4 r{r}: {ndet} det, {int(lens.sum())} samples, lambda={lam}", ndet, 0, nx, ny, lam,
-                                 seed * 1000 + r, det_lengths=lens, fknee_range=(0.05, 0.5), alpha_range=(1.0, 2.0)))
-    return out
+        wl = make_workload(f"C4 r{r}: {ndet} det, {int(lens.sum())} samples, lambda={lam}", ndet, 0, nx, ny, lam,
+                           seed * 1000 + r, det_lengths=lens, fknee_range=(0.05, 0.5), alpha_range=(1.0, 2.0))
+        if progress:
+            progress(wl.name)
+        return wl
+
+    # realizations are independent and numpy releases the GIL in its array kernels
+    with ThreadPoolExecutor(max(1, min(8, os.cpu_count() or 1))) as ex:
+        return list(ex.map(one, range(nreal)))
 
 
 def c5_request(seed, idx, max_det=128):

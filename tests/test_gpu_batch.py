@@ -110,8 +110,9 @@ def test_server_budget_serializes_solves(gpu):
     # a budget smaller than any request admits one solve at a time (never zero)
     print("case: server with a 1 MB budget", flush=True)
     wls = [W.make_workload(f"t{i}", 2, 16384, 16, 16, 64, 60 + i) for i in range(3)]
-    with gpu.Server(hbm_budget_gb=1e-3, max_slots=4, max_batch=4) as srv:
-        tickets = [srv.submit(_item(w), 256, tol=1e-8, maxiter=200) for w in wls]
+    opts = gpu.options(offset_mode=gpu.MGPU_OFFSET_SYMMETRIC)
+    with gpu.Server(hbm_budget_gb=1e-3, max_slots=4, max_batch=4, opts=opts) as srv:
+        tickets = [srv.submit(_item(w), 256, tol=1e-8, maxiter=300) for w in wls]
         res = [srv.wait(t) for t in tickets]
         st = srv.stats()
     assert st["peak_running"] == 1 and st["groups"] == 3
