@@ -294,7 +294,7 @@ def serve_section(G, W, args):
         ref.append(G.pcg_batch([r], g, tol=args.tol, maxiter=args.maxiter,
                                opts=G.options(precision=G.MGPU_FP64, offset_mode=sym))[0])
         log(f"serve: FP64 reference map {len(ref)}: {ref[-1].report['iterations']} iterations")
-    omx = G.options(precision=G.MGPU_MIXED, offset_mode=sym, graph=1)
+    omx = G.options(precision=G.MGPU_MIXED, offset_mode=sym)
     out = {"requests_per_rate": args.serve_requests, "pool": len(reqs), "precision": "mixed", "rates": []}
     rng = np.random.default_rng(55)
     for rate in [float(v) for v in args.serve_rates.split(",")]:

@@ -201,7 +201,7 @@ int mappraiser_gpu_noise_invtt(const double *noise, int nb, const int *local_blo
  * stop test, and converged items are frozen while the others go on.  Each
  * item's map, iteration count and residuals are bit-identical to its solo
  * solve with the same options (tile variants are autotuned once per process).
- * Groups run concurrently on their own streams.  No reference counterpart: the reference solves one
+ * Groups are solved one after the other.  No reference counterpart: the reference solves one
  * problem per MLmap call (mappraiser.c:34-232). */
 typedef struct mgpu_batch_item {
     /* inputs: host memory, or device memory when inputs_on_device != 0 */
@@ -234,7 +234,8 @@ int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int grid_npix, 
  * later queued request of the same solve key (lambda, tile length, grid,
  * tol, maxiter) that fits, up to max_batch; each admitted group runs as one
  * mappraiser_gpu_pcg_batch on its own stream and host thread, so solves
- * admitted at different times overlap on the device.  Results are
+ * admitted at different times overlap on the device (opt->graph is ignored:
+ * no graph is captured while other threads allocate).  Results are
  * bit-identical to the requests' solo solves.  hbm_budget_gb <= 0 uses 80% of
  * the free device memory.  No reference counterpart. */
 typedef struct mgpu_server mgpu_server;

@@ -13,13 +13,13 @@
 //     contiguous range, a map segment of the PCG (pcg.hip).
 // P, N^-1 and P^T are block diagonal over the items, the preconditioner and
 // the cut are per pixel and the PCG scalars are per segment, so each item
-// follows exactly the arithmetic of its solo solve.
+// follows exactly the arithmetic of its solo solve.  Groups (different lambda
+// or tile length) are solved one after the other.
 #include "mgpu_internal.h"
 
 #include <algorithm>
 #include <map>
 #include <string>
-#include <thread>
 
 namespace mgpu {
 
@@ -237,14 +237,9 @@ extern "C" int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int 
         g->rc = solve_group(*g, items, grid_npix, inputs_on_device != 0, tol, maxiter, opt);
         if (g->rc != MGPU_OK) g->err = mappraiser_gpu_last_error();
     };
-    if (gl.size() == 1) {
-        run(gl[0]);
-    } else {
-        // one host thread per lambda group: their streams overlap on the device
-        std::vector<std::thread> th;
-        for (Group *g : gl) th.emplace_back(run, g);
-        for (auto &t : th) t.join();
-    }
+    // groups one after the other: each fills the device on its own, and no
+    // graph capture ever overlaps allocations made by another host thread
+    for (Group *g : gl) run(g);
     for (Group *g : gl)
         if (g->rc != MGPU_OK) {
             set_error("lambda=%d, tile 2^%d group: %s", g->lambda, g->tile_log2, g->err.c_str());

@@ -167,6 +167,8 @@ int mappraiser_gpu_server_create(mgpu_server **out, double hbm_budget_gb, int ma
     mappraiser_gpu_options_default(&s->opt);
     if (opt) s->opt = *opt;
     s->opt.profile = 0;
+    // solves of concurrent workers allocate while others run: no graph capture here
+    s->opt.graph = 0;
     if (hbm_budget_gb > 0) {
         s->budget = (int64_t) (hbm_budget_gb * 1e9);
     } else {
