@@ -201,15 +201,52 @@ def main():
                                          "final_rel_residual": float(hr[-1])}
         log(f"reference offsets: converged={rr.converged}, min residual {np.nanmin(hr):.2e}")
 
+    # the same solve from host arrays (the drop-in call: host -> HBM transfers included)
+    t0 = time.perf_counter()
+    gh = G.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=args.tol,
+                   maxiter=args.maxiter, opts=G.options(offset_mode=offset))
+    out["time_to_map_from_host_s"] = round(time.perf_counter() - t0, 4)
+    log(f"from host arrays: {out['time_to_map_from_host_s']} s ({gh.report['iterations']} iterations)")
+
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args, reps[-1]["iterations"])
-    del d_pix, d_w, d_sig0, d_sig, d_noise, d_inv, wl
+        out["c1_full_solve"] = c1_section(G, W, args)
+    del d_pix, d_w, d_sig0, d_sig, d_noise, d_inv, wl, gh
     torch.cuda.empty_cache()
     if args.c4_real > 0:
         out["batch_c4"] = c4_section(G, W, torch, args)
     if args.serve_requests > 0:
         out["serve_c5"] = serve_section(G, W, args)
     print(json.dumps(out), flush=True)
+
+
+def c1_section(G, W, args):
+    """C1 (the reference's CPU-runnable configuration) solved whole on both
+    sides: the GPU drop-in from host arrays and the CPU oracle (1 core), same
+    offset mode and tolerance -- no extrapolation."""
+    from oracle import oracle as O
+
+    wl = W.config_c1()
+    mode = {"symmetric": O.OFFSET_SYMMETRIC, "reference": O.OFFSET_REFERENCE, "none": O.OFFSET_NONE}[args.offset]
+    gmode = {"symmetric": G.MGPU_OFFSET_SYMMETRIC, "reference": G.MGPU_OFFSET_REFERENCE,
+             "none": G.MGPU_OFFSET_NONE}[args.offset]
+    o = G.options(offset_mode=gmode)
+    G.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=args.tol,
+              maxiter=args.maxiter, opts=o)  # warm-up
+    t0 = time.perf_counter()
+    g = G.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=args.tol,
+                  maxiter=args.maxiter, opts=o)
+    tg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with O.offset_mode(mode):
+        r = O.gls_solve(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=args.tol,
+                        maxiter=args.maxiter)
+    tc = time.perf_counter() - t0
+    err = float(np.max(np.abs(g.x - r.x)) / np.max(np.abs(r.x)))
+    log(f"C1: GPU {tg:.3f}s ({g.report['iterations']} it), CPU oracle {tc:.2f}s ({r.iterations} it), map diff {err:.1e}")
+    return {"workload": wl.name, "gpu_s": round(tg, 4), "cpu_oracle_s": round(tc, 3), "cpu_cores": 1,
+            "gpu_iterations": g.report["iterations"], "cpu_iterations": int(r.iterations),
+            "map_rel_diff": err, "speedup": round(tc / tg, 1)}
 
 
 def c4_section(G, W, torch, args):

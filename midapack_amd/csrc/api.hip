@@ -7,7 +7,9 @@
 // precond.c:485-488, 577).
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 
 #include "mgpu_internal.h"
 
@@ -21,6 +23,67 @@ void set_error(const char *fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
+}
+
+// ---------------------------------------------------------------------------
+// device block cache
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_cache_mu;
+std::map<int, std::multimap<size_t, void *>> g_cache;  // device -> (class bytes, block)
+}  // namespace
+
+static size_t size_class(size_t bytes) {
+    const size_t mib = (size_t) 1 << 20;
+    if (bytes <= mib) return (bytes + 511) / 512 * 512;
+    return (bytes + 2 * mib - 1) / (2 * mib) * (2 * mib);
+}
+
+int dev_alloc(void **p, size_t bytes, int device, size_t *cls) {
+    const size_t c = size_class(bytes);
+    *cls = c;
+    {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        auto &fl = g_cache[device];
+        auto it = fl.find(c);
+        if (it != fl.end()) {
+            *p = it->second;
+            fl.erase(it);
+            return MGPU_OK;
+        }
+    }
+    hipError_t e = hipMalloc(p, c);
+    if (e != hipSuccess) {
+        (void) hipGetLastError();
+        dev_cache_trim(device);
+        e = hipMalloc(p, c);
+    }
+    if (e != hipSuccess) {
+        (void) hipGetLastError();
+        *p = nullptr;
+        set_error("hipMalloc of %zu bytes failed: %s", c, hipGetErrorString(e));
+        return MGPU_ENOMEM;
+    }
+    return MGPU_OK;
+}
+
+void dev_free(void *p, size_t cls, int device) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    g_cache[device].emplace(cls, p);
+}
+
+void dev_cache_trim(int device) {
+    std::vector<void *> blocks;
+    {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        for (auto &kv : g_cache)
+            if (device < 0 || kv.first == device) {
+                for (auto &b : kv.second) blocks.push_back(b.second);
+                kv.second.clear();
+            }
+    }
+    for (void *b : blocks) (void) hipFree(b);
 }
 
 Problem::~Problem() {
@@ -42,8 +105,10 @@ static int upload(Problem &pb, T **d, const T *h, size_t n) {
 int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt) {
     pb.reset(new Problem());
     pb->device = g_device;
+    pb->arena.device = g_device;
     MGPU_HIP(hipSetDevice(g_device));
     MGPU_HIP(hipStreamCreateWithFlags(&pb->stream, hipStreamNonBlocking));
+    pb->arena.stream = pb->stream;
     mappraiser_gpu_options_default(&pb->opt);
     if (opt) pb->opt = *opt;
     pb->prof.on = pb->opt.profile != 0;
@@ -77,7 +142,10 @@ int mappraiser_gpu_init(int device) {
     return MGPU_OK;
 }
 
-int mappraiser_gpu_finalize(void) { return MGPU_OK; }
+int mappraiser_gpu_finalize(void) {
+    dev_cache_trim(-1);
+    return MGPU_OK;
+}
 
 const char *mappraiser_gpu_last_error(void) { return g_err; }
 

@@ -57,33 +57,53 @@ void set_error(const char *fmt, ...);
 #define MGPU_LAUNCHED() MGPU_HIP(hipGetLastError())
 
 // ---------------------------------------------------------------------------
-// device buffers (owned by a problem; freed in its destructor)
+// device buffers
 // ---------------------------------------------------------------------------
+// Process-wide cache of device blocks: hipFree synchronises the device and
+// hipMalloc of large blocks costs milliseconds, so blocks released by a
+// problem are kept per device (size classes of 2 MiB above 1 MiB) and handed
+// to later allocations of the same class.  On an allocation failure the
+// cache of the device is emptied and the allocation retried.
+// mappraiser_gpu_finalize() empties every cache.
+int dev_alloc(void **p, size_t bytes, int device, size_t *cls);
+void dev_free(void *p, size_t cls, int device);
+void dev_cache_trim(int device);  // device < 0: every device
+
+// buffers owned by a problem; returned to the cache in its destructor (after
+// the problem's stream has drained) or by release(), which first waits for
+// the stream so that no pending kernel of this problem still uses the block
+// when another problem receives it
 struct Arena {
-    std::vector<void *> ptrs;
+    struct Blk {
+        void *p;
+        size_t cls;
+    };
+    std::vector<Blk> blks;
+    int device = 0;
+    hipStream_t stream = nullptr;
     template <typename T>
     int alloc(T **p, size_t n) {
         *p = nullptr;
         if (n == 0) n = 1;
-        hipError_t e = hipMalloc((void **) p, n * sizeof(T));
-        if (e != hipSuccess) {
-            (void) hipGetLastError();
-            set_error("hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
-            return MGPU_ENOMEM;
-        }
-        ptrs.push_back((void *) *p);
+        void *q = nullptr;
+        size_t cls = 0;
+        const int rc = dev_alloc(&q, n * sizeof(T), device, &cls);
+        if (rc != MGPU_OK) return rc;
+        blks.push_back({q, cls});
+        *p = (T *) q;
         return MGPU_OK;
     }
     void release(void *p) {
-        for (auto &q : ptrs)
-            if (q == p) {
-                (void) hipFree(q);
-                q = nullptr;
+        for (auto &b : blks)
+            if (b.p == p) {
+                if (stream) (void) hipStreamSynchronize(stream);
+                dev_free(b.p, b.cls, device);
+                b.p = nullptr;
             }
     }
     ~Arena() {
-        for (void *p : ptrs)
-            if (p) (void) hipFree(p);
+        for (auto &b : blks)
+            if (b.p) dev_free(b.p, b.cls, device);
     }
 };
 

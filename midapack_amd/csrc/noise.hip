@@ -145,6 +145,7 @@ int dft_real(Problem &pb, const double *d_x, const int64_t *d_xoff, const int *d
         // a.p holds the spectrum (pointers swapped in place)
         MGPU_HIP(hipMemcpy2DAsync(d_X, sizeof(double2) * nout_max, a.p, sizeof(double2) * M,
                                   sizeof(double2) * std::min<int64_t>(nout_max, M), batch, hipMemcpyDeviceToDevice, s));
+        MGPU_HIP(hipStreamSynchronize(s));  // before the work buffers go back to the block cache
         return MGPU_OK;
     }
     MGPU_RET(pb.arena.alloc(&c.p, (size_t) total));
@@ -157,6 +158,7 @@ int dft_real(Problem &pb, const double *d_x, const int64_t *d_xoff, const int *d
     MGPU_RET(fft(a.p, b.p, +1));
     k_blue_post<<<grid_for(nout_max * batch, 256), 256, 0, s>>>(a.p, d_nlen, logm, batch, nout_max, d_X);
     MGPU_LAUNCHED();
+    MGPU_HIP(hipStreamSynchronize(s));  // before the work buffers go back to the block cache
     return MGPU_OK;
 }
 

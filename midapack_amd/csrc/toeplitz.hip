@@ -820,9 +820,15 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
         std::unordered_map<uint64_t, std::vector<int>> by_hash;
         for (int b = 0; b < nb; b++) {
             const double *Tb = inv + (int64_t) b * lambda;
+            // hash of <= 256 evenly spaced values (the candidates are confirmed by memcmp)
             uint64_t h = 1469598103934665603ull;
-            const unsigned char *bytes = reinterpret_cast<const unsigned char *>(Tb);
-            for (size_t i = 0; i < sizeof(double) * (size_t) lambda; i++) h = (h ^ bytes[i]) * 1099511628211ull;
+            const int step = lambda > 256 ? lambda / 256 : 1;
+            for (int i = 0; i < lambda; i += step) {
+                uint64_t w;
+                std::memcpy(&w, Tb + i, sizeof(w));
+                h = (h ^ w) * 1099511628211ull;
+                h ^= h >> 29;
+            }
             int fid = -1;
             auto &cand = by_hash[h];
             for (int f : cand)
