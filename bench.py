@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=12, help="requests per serving rate (0: skip)")
     ap.add_argument("--serve-rates", default="1,2,5,10", help="Poisson arrival rates (req/s)")
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded requests cycled by the load")
+    ap.add_argument("--no-noise-model", action="store_true", help="skip the invtt-from-noise section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device to device once before the solves (byte-counter calibration)")
     args = ap.parse_args()
@@ -211,6 +212,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args, reps[-1]["iterations"])
         out["c1_full_solve"] = c1_section(G, W, args)
+    if not args.no_noise_model:
+        out["noise_model_c2"] = noise_section(G, wl, args)
     del d_pix, d_w, d_sig0, d_sig, d_noise, d_inv, wl, gh
     torch.cuda.empty_cache()
     if args.c4_real > 0:
@@ -218,6 +221,29 @@ def main():
     if args.serve_requests > 0:
         out["serve_c5"] = serve_section(G, W, args)
     print(json.dumps(out), flush=True)
+
+
+def noise_section(G, wl, args):
+    """invtt of every C2 block built from the noise timestream on the device
+    (noise2invtt, utils.py:500-601), from host arrays, against the oracle
+    (scipy periodogram + curve_fit, 1 core) on the first 4 blocks."""
+    from oracle import noise as N
+
+    nb = int(wl.block_sizes.size)
+    G.noise_invtt(wl.noise[: int(wl.block_sizes[0])], wl.block_sizes[:1], wl.lam, 10.0)  # warm-up
+    t0 = time.perf_counter()
+    inv, par = G.noise_invtt(wl.noise, wl.block_sizes, wl.lam, 10.0)
+    tg = time.perf_counter() - t0
+    k = 4
+    n_k = int(np.sum(wl.block_sizes[:k]))
+    t0 = time.perf_counter()
+    ref, rpar = N.compute_invtt(wl.noise[:n_k], wl.block_sizes[:k], wl.lam, 10.0)
+    tc = (time.perf_counter() - t0) * nb / k
+    err = float(np.max(np.abs(inv[: k * wl.lam] - ref)) / np.max(np.abs(ref)))
+    log(f"noise model: GPU {tg:.3f}s for {nb} blocks, CPU oracle {tc:.1f}s (extrapolated), row error {err:.1e}")
+    return {"blocks": nb, "block_len": int(wl.block_sizes[0]), "lambda": wl.lam, "gpu_s_from_host": round(tg, 4),
+            "cpu_oracle_s": round(tc, 2), "cpu_sample": f"first {k} blocks timed, scaled to {nb}",
+            "max_row_rel_err_vs_oracle": err}
 
 
 def c1_section(G, W, args):
