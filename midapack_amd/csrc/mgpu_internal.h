@@ -177,6 +177,13 @@ struct ToeplitzPlan {
     int offset_mode = MGPU_OFFSET_REFERENCE;
     int *d_chunk_block = nullptr;    // nchunks: first block overlapping chunk c (partial-sum epilogues)
     int variant[2] = {0, 0};         // tile kernel variant used for FP64 [0] and FP32 [1] (0 = not chosen)
+    // FP32 products (MGPU_MIXED) with a longer tile than the FP64 one, when that
+    // pays off (less overlap per useful sample); log2n32 == 0: FP32 uses the FP64 tiles
+    int log2n32 = 0, L32 = 0, ntiles32 = 0;
+    bool fp32_long = true;           // FP32 uses the long tiles (set by the autotuner)
+    TileDesc *d_tiles32 = nullptr;
+    float *d_coef32L = nullptr;      // nfilters * (N32/2) (a_k, b_k)
+    float *d_tw32L = nullptr;
     double *d_mean_out = nullptr;    // nb
     double *d_tail = nullptr;        // nfilters * lambda: tail_f[j] = sum_{k>j} T_f[k]
     int *d_block_filter = nullptr;   // nb

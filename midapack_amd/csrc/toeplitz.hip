@@ -805,8 +805,8 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
 
     const bool fp32 = pb.opt.precision == MGPU_MIXED;
     // FP64 tiles use the FP64 limit even in mixed mode (the FP64 residual refresh needs them)
-    int l2 = choose_log2n(lambda, nmax, false, pb.opt.tile_log2);
-    (void) fp32;
+    // a forced length longer than the FP64 limit applies to the FP32 tiles only
+    int l2 = choose_log2n(lambda, nmax, false, std::min(pb.opt.tile_log2, max_log2n(false)));
     T.h_short_blocks.clear();
     std::vector<int> direct_blocks;
     if (l2 < 0 || l2 < min_log2n() || l2 > max_log2n(false)) {
@@ -920,6 +920,43 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
         MGPU_RET(pb.arena.alloc(&T.d_tw32, (size_t) NC * 2));
         k_twiddles<<<grid_for(NC, 256), 256, 0, s>>>(NC, T.d_tw64, T.d_tw32);
         MGPU_LAUNCHED();
+        // FP32 products of a mixed solve: a longer tile when it costs fewer FFT
+        // operations per useful sample (FP32 halves the registers and LDS a tile needs)
+        const int l2f = fp32 ? choose_log2n(lambda, nmax, true, pb.opt.tile_log2) : -1;
+        if (l2f > l2 && l2f <= max_log2n(true)) {
+            const int Nf = 1 << l2f, NCf = Nf / 2;
+            T.log2n32 = l2f;
+            T.L32 = Nf - 2 * (lambda - 1);
+            std::vector<TileDesc> tf;
+            for (int b = 0; b < nb; b++) {
+                const int64_t n = block_sizes[b];
+                if (n < lambda - 1) continue;
+                const int64_t nt = (n + T.L32 - 1) / T.L32;
+                for (int64_t k = 0; k < nt; k++)
+                    tf.push_back(TileDesc{T.h_start[b], T.h_start[b + 1], T.h_start[b] + k * T.L32, b,
+                                          T.h_block_filter[b]});
+            }
+            T.ntiles32 = (int) tf.size();
+            MGPU_RET(pb.arena.alloc(&T.d_tiles32, tf.size()));
+            MGPU_HIP(hipMemcpyAsync(T.d_tiles32, tf.data(), sizeof(TileDesc) * tf.size(), hipMemcpyHostToDevice, s));
+            double2 *d_bf;
+            double *d_c64;
+            double *d_t64;
+            MGPU_RET(pb.arena.alloc(&d_bf, (size_t) T.nfilters * Nf));
+            MGPU_RET(pb.arena.alloc(&d_c64, (size_t) T.nfilters * NCf * 2));
+            MGPU_RET(pb.arena.alloc(&d_t64, (size_t) NCf * 2));
+            MGPU_RET(pb.arena.alloc(&T.d_coef32L, (size_t) T.nfilters * NCf * 2));
+            MGPU_RET(pb.arena.alloc(&T.d_tw32L, (size_t) NCf * 2));
+            k_circ_fft<<<T.nfilters, 1024, 0, s>>>(d_inv, d_src, lambda, l2f, d_bf);
+            MGPU_LAUNCHED();
+            k_coef<<<grid_for((int64_t) T.nfilters * NCf, 256), 256, 0, s>>>(d_bf, l2f, T.nfilters, d_c64,
+                                                                               T.d_coef32L);
+            MGPU_LAUNCHED();
+            k_twiddles<<<grid_for(NCf, 256), 256, 0, s>>>(NCf, d_t64, T.d_tw32L);
+            MGPU_LAUNCHED();
+            MGPU_HIP(hipStreamSynchronize(s));
+            for (void *p : {(void *) d_bf, (void *) d_c64, (void *) d_t64}) pb.arena.release(p);
+        }
         MGPU_HIP(hipStreamSynchronize(s));
         pb.arena.release(d_buf);
         pb.arena.release(d_src);
@@ -963,6 +1000,7 @@ static const TileShape kShapes64_9[] = {{3, 3, true, 1}};
 static const TileShape kShapes64_mid[] = {{4, 4, true, 1}, {3, 3, true, 1}};
 static const TileShape kShapes64_14[] = {{5, 4, false, 1}, {5, 5, false, 1}, {4, 4, false, 1}};
 static const TileShape kShapes32_14[] = {{5, 4, true, 1}, {5, 5, true, 1}, {4, 4, true, 1}, {5, 4, false, 4}};
+static const TileShape kShapes32_15[] = {{5, 4, false, 1}, {6, 5, false, 1}};
 
 static const TileShape *tile_shapes(bool fp32, int log2n, int *n) {
     const int lognc = log2n - 1;
@@ -977,6 +1015,7 @@ static const TileShape *tile_shapes(bool fp32, int log2n, int *n) {
         if (fp32) MGPU_SHAPES(kShapes32_14);
         MGPU_SHAPES(kShapes64_14);
     }
+    if (lognc == 15 && fp32) MGPU_SHAPES(kShapes32_15);
 #undef MGPU_SHAPES
     *n = 0;
     return nullptr;
@@ -1002,9 +1041,24 @@ const char *tile_variant_desc(bool fp32, int log2n, int variant) {
     return buf;
 }
 
+// the tiles a precision's products use: the FP64 set, or the FP32 long-tile set
+struct TileUse {
+    int log2n, L, ntiles;
+    const TileDesc *tiles;
+    const void *coef, *tw;
+};
+template <typename R>
+static TileUse tile_use(const ToeplitzPlan &T) {
+    if (sizeof(R) == 4 && T.log2n32 > 0 && T.fp32_long)
+        return {T.log2n32, T.L32, T.ntiles32, T.d_tiles32, T.d_coef32L, T.d_tw32L};
+    if (sizeof(R) == 4) return {T.log2n, T.L, T.ntiles, T.d_tiles, T.d_coef32, T.d_tw32};
+    return {T.log2n, T.L, T.ntiles, T.d_tiles, T.d_coef64, T.d_tw64};
+}
+
 template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO, int MINW>
 static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *coef, const void *tw) {
     const ToeplitzPlan &T = pb.tp;
+    const TileUse u = tile_use<R>(T);
     constexpr int NC = 1 << LOGNC, NT = NC / (1 << LOGV);
     constexpr size_t lds = sizeof(R) * (size_t) lds_elems(NC) * (TWO ? 2 : 1);
     static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
@@ -1014,8 +1068,8 @@ static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *co
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
         attr_devices |= 1u << pb.device;
     }
-    TileArgs a{d_in, d_out, T.d_tiles, coef, tw, T.d_mean, T.d_mean_out, T.lambda, T.L};
-    hipLaunchKernelGGL(kern, dim3(T.ntiles), dim3(NT), lds, pb.stream, a);
+    TileArgs a{d_in, d_out, u.tiles, coef, tw, T.d_mean, T.d_mean_out, T.lambda, u.L};
+    hipLaunchKernelGGL(kern, dim3(u.ntiles), dim3(NT), lds, pb.stream, a);
     MGPU_LAUNCHED();
     return MGPU_OK;
 }
@@ -1024,11 +1078,11 @@ template <typename R>
 static int launch_tiles(Problem &pb, const R *d_in, R *d_out, int variant) {
     const ToeplitzPlan &T = pb.tp;
     constexpr bool F32 = sizeof(R) == 4;
-    const void *coef = F32 ? (const void *) T.d_coef32 : (const void *) T.d_coef64;
-    const void *tw = F32 ? (const void *) T.d_tw32 : (const void *) T.d_tw64;
+    const TileUse u = tile_use<R>(T);
+    const void *coef = u.coef, *tw = u.tw;
 #define MGPU_TL(LOGNC, LOGV, LOGR, TWO, MINW) \
     return launch_tiles_cfg<R, LOGNC, LOGV, LOGR, TWO, MINW>(pb, d_in, d_out, coef, tw)
-    switch (T.log2n - 1) {  // LOGNC
+    switch (u.log2n - 1) {  // LOGNC
         case 9: MGPU_TL(9, 3, 3, true, 1);
         case 10:
             if (variant == 2) MGPU_TL(10, 3, 3, true, 1);
@@ -1053,7 +1107,14 @@ static int launch_tiles(Problem &pb, const R *d_in, R *d_out, int variant) {
                 if (variant == 3) MGPU_TL(14, 4, 4, false, 1);
                 MGPU_TL(14, 5, 4, false, 1);
             }
-        default: set_error("unsupported tile length 2^%d", T.log2n); return MGPU_EINVAL;
+        case 15:
+            if constexpr (F32) {
+                if (variant == 2) MGPU_TL(15, 6, 5, false, 1);
+                MGPU_TL(15, 5, 4, false, 1);
+            }
+            set_error("unsupported FP64 tile length 2^16");
+            return MGPU_EINVAL;
+        default: set_error("unsupported tile length 2^%d", u.log2n); return MGPU_EINVAL;
     }
 #undef MGPU_TL
 }
@@ -1071,28 +1132,33 @@ static int pick_variant(Problem &pb, const R *d_in, R *d_out) {
     ToeplitzPlan &T = pb.tp;
     const int slot = sizeof(R) == 4 ? 1 : 0;
     if (T.variant[slot]) return MGPU_OK;
-    const int nv = tile_variant_count(slot == 1, T.log2n);
-    MGPU_ARG(nv > 0, "unsupported tile length 2^%d", T.log2n);
+    const int log2n = tile_use<R>(T).log2n;
+    const int nv = tile_variant_count(slot == 1, log2n);
+    MGPU_ARG(nv > 0, "unsupported tile length 2^%d", log2n);
     // a forced variant applies to the precision of the PCG iterations (FP32 under
     // MGPU_MIXED); the FP64 products of a mixed solve (right-hand side, residual
     // refreshes) are autotuned
     const int main_slot = pb.opt.precision == MGPU_MIXED ? 1 : 0;
     if (pb.opt.tile_variant > 0 && slot == main_slot) {
         MGPU_ARG(pb.opt.tile_variant <= nv, "tile_variant=%d: the %s tile of length 2^%d has %d variant(s)",
-                 pb.opt.tile_variant, slot ? "fp32" : "fp64", T.log2n, nv);
+                 pb.opt.tile_variant, slot ? "fp32" : "fp64", log2n, nv);
         T.variant[slot] = pb.opt.tile_variant;
         return MGPU_OK;
     }
-    if (nv == 1) {
+    // FP32 with a long-tile set: the candidates are every variant of both lengths
+    const bool two_len = slot == 1 && T.log2n32 > 0;
+    if (nv == 1 && !two_len) {
         T.variant[slot] = 1;
         return MGPU_OK;
     }
-    const auto key = std::make_tuple(pb.device, slot, T.log2n);
+    const int nv_short = two_len ? tile_variant_count(true, T.log2n) : 0;
+    const auto key = std::make_tuple(pb.device, slot, log2n * 100 + (two_len ? T.log2n : 0));
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tune.find(key);
         if (it != g_tune.end()) {
-            T.variant[slot] = it->second;
+            T.variant[slot] = it->second % 1000;
+            if (two_len) T.fp32_long = it->second < 1000;
             return MGPU_OK;
         }
     }
@@ -1100,9 +1166,13 @@ static int pick_variant(Problem &pb, const R *d_in, R *d_out) {
     MGPU_HIP(hipEventCreate(&e0));
     MGPU_HIP(hipEventCreate(&e1));
     int best = 1;
+    bool best_long = true;
     float best_ms = 0.f;
     int rc = MGPU_OK;
-    for (int v = 1; v <= nv && rc == MGPU_OK; v++) {
+    for (int c = 1; c <= nv + nv_short && rc == MGPU_OK; c++) {
+        const bool lng = c <= nv;
+        const int v = lng ? c : c - nv;
+        if (two_len) T.fp32_long = lng;
         rc = launch_tiles<R>(pb, d_in, d_out, v);
         if (rc != MGPU_OK) break;
         if (hipEventRecord(e0, pb.stream) != hipSuccess) {
@@ -1111,6 +1181,7 @@ static int pick_variant(Problem &pb, const R *d_in, R *d_out) {
             break;
         }
         for (int k = 0; k < 3 && rc == MGPU_OK; k++) rc = launch_tiles<R>(pb, d_in, d_out, v);
+        // per useful output: the two lengths have different tile counts but the same outputs
         if (rc != MGPU_OK) break;
         float ms = 0.f;
         if (hipEventRecord(e1, pb.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -1119,17 +1190,19 @@ static int pick_variant(Problem &pb, const R *d_in, R *d_out) {
             rc = MGPU_EHIP;
             break;
         }
-        if (v == 1 || ms < best_ms) {
+        if (c == 1 || ms < best_ms) {
             best_ms = ms;
             best = v;
+            best_long = lng;
         }
     }
     (void) hipEventDestroy(e0);
     (void) hipEventDestroy(e1);
     if (rc != MGPU_OK) return rc;
     T.variant[slot] = best;
+    if (two_len) T.fp32_long = best_long;
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[key] = best;
+    g_tune[key] = best + (two_len && !best_long ? 1000 : 0);
     return MGPU_OK;
 }
 

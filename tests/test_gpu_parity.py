@@ -252,7 +252,7 @@ def _variant_cases():
     from midapack_amd import lib as G
     out = []
     for prec, tag in ((G.MGPU_FP64, "fp64"), (G.MGPU_MIXED, "fp32")):
-        for l2 in (12, 15):
+        for l2 in ((12, 15) if prec == G.MGPU_FP64 else (12, 15, 16)):
             for v in range(1, len(G.tile_variants(prec, l2)) + 1):
                 out.append((prec, l2, v, f"{tag}-N2^{l2}-v{v}"))
     return out
@@ -288,7 +288,8 @@ def test_mixed_tile_variants_map_error(gpu, case):
     _, l2, v, name = case
     print(f"case: {name}", flush=True)
     lam = 1024 if l2 == 12 else 8192
-    wl = W.make_workload("variant", 4, 1 << 17, 128, 128, lam, 7, block_len=1 << 16)
+    # 2^16-sample tiles need blocks longer than one tile's useful part
+    wl = W.make_workload("variant", 4, 1 << 17, 128, 128, lam, 7, block_len=(1 << 17) if l2 == 16 else (1 << 16))
     g64 = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
                       maxiter=400, opts=gpu.options(tile_log2=l2, offset_mode=gpu.MGPU_OFFSET_SYMMETRIC))
     gm = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
