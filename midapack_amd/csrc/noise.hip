@@ -306,6 +306,19 @@ __global__ __launch_bounds__(256) void k_fit(const double *__restrict__ y, const
                 }
             for (int i = 0; i < 4; i++) g[i] = -sh[10 + i];
             for (int i = 0; i < 4; i++) A[i][i] += mu * (A[i][i] > 0.0 ? A[i][i] : 1.0);
+            // active set: a parameter on a bound whose descent direction points
+            // outside stays fixed (its row and column leave the system)
+            for (int i = 0; i < 4; i++) {
+                const bool act = (p[i] <= lo[i] && g[i] < 0.0) || (p[i] >= hi[i] && g[i] > 0.0);
+                if (act) {
+                    for (int j = 0; j < 4; j++) {
+                        A[i][j] = 0.0;
+                        A[j][i] = 0.0;
+                    }
+                    A[i][i] = 1.0;
+                    g[i] = 0.0;
+                }
+            }
             // Gaussian elimination with partial pivoting (4x4)
             for (int c = 0; c < 4; c++) {
                 int pr = c;
