@@ -341,10 +341,7 @@ def serve_section(G, W, args):
     import threading
 
     t = time.time()
-    pool = []
-    for i in range(args.serve_pool):
-        pool.append(W.c5_request(5, i))
-        log(f"serve: generated {pool[-1].name}")
+    pool = W.c5_pool(5, args.serve_pool, progress=lambda name: log(f"serve: generated {name}"))
     reqs = [dict(pix=w.pix, weights=w.weights, signal=w.signal, noise=w.noise, block_sizes=w.block_sizes,
                  lam=w.lam, invtt=w.invtt) for w in pool]
     grids = [w.nx * w.ny for w in pool]

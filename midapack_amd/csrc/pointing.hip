@@ -510,37 +510,6 @@ __device__ __forceinline__ double wg_sum_256(double v, double *red) {
     return s;  // valid in thread 0
 }
 
-// y for the PER samples of this lane from the map window xw (LDS or global);
-// q relative to the window's first pixel
-template <typename TX, typename TW, typename TW2, bool W0, int PER>
-__device__ __forceinline__ void p_rows(const TX *__restrict__ xw, int qbase, const int *qq, const TW2 *wv,
-                                       const TW *__restrict__ w0, int64_t c0, int64_t m, TX *__restrict__ y,
-                                       TX *yv) {
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const int64_t t = c0 + threadIdx.x + (int64_t) i * kPBlock;
-        TX v = (TX) 0;
-        if (qq[i] >= 0) {
-            const TX *xp = xw + 3 * (qq[i] - qbase);
-            const TX x0 = xp[0], x1 = xp[1], x2 = xp[2];
-            TX acc = (TX) 0;
-            acc += (W0 ? (TX) w0[t] * x0 : x0);
-            acc += (TX) wv[i].x * x1;
-            acc += (TX) wv[i].y * x2;
-            v = acc;
-        }
-        if (t < m) y[t] = v;
-        yv[i] = v;
-    }
-}
-
-// Map window staging: the pixels a chunk of a scan touches are usually a
-// narrow range (a raster row, a few rows); when that range spans fewer than
-// kStagePix pixels the workgroup copies it into LDS with coalesced loads and
-// gathers from LDS, otherwise it gathers from global memory.  Same arithmetic
-// either way.
-constexpr int kStagePix = 1024;
-
 template <typename TX, typename TW, typename TW2, bool W0, bool PARTIALS>
 __global__ __launch_bounds__(kPBlock) void k_matvec(const int *__restrict__ q, const TW *__restrict__ w0,
                                                     const TW2 *__restrict__ w12, const TX *__restrict__ x,
@@ -548,8 +517,6 @@ __global__ __launch_bounds__(kPBlock) void k_matvec(const int *__restrict__ q, c
                                                     const int *__restrict__ chunk_block, int nb,
                                                     double *__restrict__ partials) {
     __shared__ double red[kPBlock / 64];
-    __shared__ int rlo[kPBlock / 64], rhi[kPBlock / 64];
-    __shared__ TX xs[3 * kStagePix];
     constexpr int PER = kChunk / kPBlock;
     const int64_t c = blockIdx.x;
     const int64_t c0 = c * kChunk;
@@ -565,37 +532,22 @@ __global__ __launch_bounds__(kPBlock) void k_matvec(const int *__restrict__ q, c
             wv[i] = w12[t];
         }
     }
-    // pixel window of the chunk
-    int lo = INT32_MAX, hi = -1;
-#pragma unroll
-    for (int i = 0; i < PER; i++)
-        if (qq[i] >= 0) {
-            lo = min(lo, qq[i]);
-            hi = max(hi, qq[i]);
-        }
-    for (int o = 32; o > 0; o >>= 1) {
-        lo = min(lo, __shfl_xor(lo, o));
-        hi = max(hi, __shfl_xor(hi, o));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        rlo[threadIdx.x >> 6] = lo;
-        rhi[threadIdx.x >> 6] = hi;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int w = 0; w < kPBlock / 64; w++) {
-        lo = min(lo, rlo[w]);
-        hi = max(hi, rhi[w]);
-    }
     TX yv[PER];
-    if (hi >= lo && hi - lo < kStagePix) {
-        const int n3 = 3 * (hi - lo + 1);
-        const TX *__restrict__ xw = x + 3 * (int64_t) lo;
-        for (int j = threadIdx.x; j < n3; j += kPBlock) xs[j] = xw[j];
-        __syncthreads();
-        p_rows<TX, TW, TW2, W0, PER>(xs, lo, qq, wv, w0, c0, m, y, yv);
-    } else {
-        p_rows<TX, TW, TW2, W0, PER>(x, 0, qq, wv, w0, c0, m, y, yv);
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int64_t t = c0 + threadIdx.x + (int64_t) i * kPBlock;
+        TX v = (TX) 0;
+        if (qq[i] >= 0) {
+            const TX *xp = x + 3 * (int64_t) qq[i];
+            const TX x0 = xp[0], x1 = xp[1], x2 = xp[2];
+            TX acc = (TX) 0;
+            acc += (W0 ? (TX) w0[t] * x0 : x0);
+            acc += (TX) wv[i].x * x1;
+            acc += (TX) wv[i].y * x2;
+            v = acc;
+        }
+        if (t < m) y[t] = v;
+        yv[i] = v;
     }
     if (PARTIALS) {
         const int64_t c1 = min(c0 + (int64_t) kChunk, m);

@@ -229,3 +229,18 @@ def c5_request(seed, idx, max_det=128):
     lam = int(rr.choice([1024, 4096, 8192]))
     return make_workload(f"C5 req{idx}: {ndet}x{nsamp} on {side}^2, lambda={lam}", ndet, nsamp, side, side, lam,
                          seed * 100000 + idx, fknee_range=(0.05, 0.5), alpha_range=(1.0, 2.0))
+
+
+def c5_pool(seed, n, max_det=128, progress=None):
+    """Requests 0..n-1 of c5_request, generated on a thread pool."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
+        wl = c5_request(seed, i, max_det)
+        if progress:
+            progress(wl.name)
+        return wl
+
+    with ThreadPoolExecutor(max(1, min(8, os.cpu_count() or 1))) as ex:
+        return list(ex.map(one, range(n)))
