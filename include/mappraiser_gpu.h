@@ -55,6 +55,12 @@ typedef struct mgpu_options {
     int graph;          /* 1: capture each block of PCG iterations once into a HIP graph and
                            replay it (same kernels, bit-identical results; ignored when
                            profile = 1) */
+    int precond;        /* PCG_GLS_true's precond (pcg_true.c:52, build_precond precond.c:1310-1398):
+                           0 block Jacobi, 1 two-level a priori (build_Z), 2 two-level
+                           a posteriori (Lanczos_eig); two-level needs lambda > 1 and one
+                           problem per solve (not the batch / server entry points) */
+    int z_2lvl;         /* Z_2lvl: deflation vectors of the two-level preconditioners
+                           (0 -> 1, the one-rank communicator size, pcg_true.c:50; at most 256) */
 } mgpu_options;
 
 /* Per-block offset handling in N^-1 (block b: C = I - 1 1^T / n_b, T = Toeplitz band):
@@ -92,6 +98,7 @@ typedef struct mgpu_report {
     double true_res_rel;/* ||b - A x||/||b - A x0|| recomputed in FP64 at exit (MGPU_MIXED), else -1 */
     int    refreshes;   /* FP64 residual recomputations performed (MGPU_MIXED) */
     int    tile_variant;  /* tile kernel variant used by the solve's Toeplitz product (0: none) */
+    double t_2lvl_ms;   /* device time building the two-level space Z, A Z, pinv(E) (precond 1, 2) */
 } mgpu_report;
 
 /* ---- library ---- */

@@ -212,6 +212,9 @@ extern "C" int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int 
     MGPU_ARG(nitems > 0 && items, "empty batch");
     MGPU_ARG(grid_npix > 0, "grid_npix=%d must be positive", grid_npix);
     MGPU_ARG(maxiter >= 1, "maxiter=%d must be >= 1", maxiter);
+    MGPU_ARG(!opt || opt->precond == 0,
+             "precond=%d: batches use block Jacobi (two-level preconditioners solve one problem at a time)",
+             opt ? opt->precond : 0);
     std::map<std::pair<int, int>, Group> groups;
     for (int i = 0; i < nitems; i++) {
         const mgpu_batch_item &it = items[i];

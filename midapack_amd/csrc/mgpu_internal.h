@@ -321,6 +321,23 @@ struct Segments {
     int *d_state = nullptr;   // nseg: PCG state (0 running, 1 converged, 2 stopped)
 };
 
+// ---------------------------------------------------------------------------
+// Two-level preconditioner (build_precond precond = 1, 2): deflation space Z,
+// its image A Z and pinv(E), E[i][j] = <Z_j, A Z_i>
+// ---------------------------------------------------------------------------
+constexpr int kMaxZn = 256;
+struct TwoLevel {
+    int zn = 0;                 // columns of Z (0: block Jacobi only)
+    int ld = 0;                 // column stride (3 * npix of the solve)
+    double *d_Z = nullptr;      // [zn * ld], column j at j * ld
+    double *d_AZ = nullptr;     // [zn * ld]
+    double *d_Em1 = nullptr;    // [zn * zn] row-major
+    double *d_qtx = nullptr;    // [zn] Em1 Z^T g of the last apply
+    double *d_part = nullptr;   // [zn * nchunk] partials of Z^T g
+    int nchunk = 0;
+    double build_ms = 0.0;      // device time of the last build
+};
+
 struct Problem {
     Profiler prof;
     int device = 0;
@@ -332,6 +349,7 @@ struct Problem {
     ToeplitzPlan tp;
     Precond pc;
     Segments sg;
+    TwoLevel tl;
     // working TOD / map vectors (allocated lazily)
     double *d_tod = nullptr, *d_tod2 = nullptr;   // [m] FP64
     float *d_todf = nullptr, *d_tod2f = nullptr;  // [m] FP32 (mixed)
@@ -380,6 +398,17 @@ int toeplitz_tile_log2(int lambda, int64_t nmax, int forced);
 const char *tile_variant_desc(bool fp32, int log2n, int variant);
 
 int precond_build(Problem &pb, double *d_signal);
+// A v = P^T N^-1 P v in FP64 (map -> map), optional fused <v, Av> partials
+int apply_A64(Problem &pb, const double *d_v, double *d_out, double *d_dot);
+// two-level preconditioner: build Z, A Z, pinv(E) for opt.precond (1 a priori,
+// 2 a posteriori from signal + noise); release frees it
+int twolvl_build(Problem &pb, const double *d_signal, const double *d_noise);
+void twolvl_release(Problem &pb);
+// Cg = BJ (g - AZ Qtx) + Z Qtx, Qtx = Em1 Z^T g, with the PCG dot partials:
+// init: h = Cg, part = [<Cg,g> | <g,g>]; else part = [<Cg,g> | <Cg,g-gp> | <g,g>]
+// (one entry per map workgroup of each dot, the layout of the block-Jacobi kernels)
+int twolvl_apply(Problem &pb, const double *d_g, const double *d_gp, double *d_Cg, double *d_h, double *d_part,
+                 bool init, const int *d_state, const int *d_flag);
 int launch_precond_apply(Problem &pb, const double *d_g, double *d_Cg);
 
 // per-segment PCG outcome (batched problems)

@@ -28,7 +28,7 @@
 // MGPU_MIXED: P, N^-1 and P^T run in FP32 on FP32 copies of the pointing
 // weights and TOD; map vectors, dot products and the 3x3 blocks stay FP64,
 // and every `refresh_every` iterations g is recomputed in FP64 as b - A x.
-#include "mgpu_internal.h"
+#include "pcg_device.h"
 
 #include <algorithm>
 #include <cmath>
@@ -36,70 +36,6 @@
 
 namespace mgpu {
 
-// scalar slots of one segment in d_scal (segment r at r * S_NSCAL)
-enum {
-    S_G2PIX = 0,
-    S_RES,
-    S_RES0,
-    S_TOL2,
-    S_RO,
-    S_GAMMA,
-    S_COEFF,
-    S_POLAK,
-    S_G2PIXP,
-    S_K,        // iteration counter (as double)
-    S_TOL,
-    S_STOPK,    // stop_after (0 = off)
-    S_MAXK,
-    S_TRUE_R2,  // ||b - A x||^2 of the returned x (mixed precision)
-    S_TRUE_B2,  // ||b||^2
-    S_NSCAL = 16
-};
-
-// device flags: [0] every segment stopped, [1] segment 0 converged, [2] largest
-// iteration count reached, [3] number of stopped segments; [4..7] never set
-enum { F_STOP = 0, F_CONV0, F_KMAX, F_NDONE, F_NEVER = 4, F_NFLAGS = 8 };
-
-constexpr int kMB = kMapWG;  // map-domain workgroup size (pixels)
-
-__device__ __forceinline__ bool stopped(const int *flag) { return *(volatile const int *) flag != 0; }
-
-__device__ __forceinline__ double block_sum(double v, double *red) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int) (blockDim.x >> 6); i++) s += red[i];
-    return s;
-}
-
-// fixed-order reduction of nparts partials (single workgroup of 256)
-__device__ __forceinline__ double reduce_parts(const double *__restrict__ p, int nparts, double *red) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += 256) s += p[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    return red[0];
-}
-
-// pixel handled by this thread of a map-domain workgroup, its segment and the segment end
-struct SegPix {
-    int r, q, qend;
-};
-__device__ __forceinline__ SegPix seg_pixel(const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
-                                            const int *__restrict__ seg_q) {
-    SegPix p;
-    p.r = wg_seg[blockIdx.x];
-    p.q = wg_q0[blockIdx.x] + (int) threadIdx.x;
-    p.qend = seg_q[p.r + 1];
-    return p;
-}
 
 // a segment is finished: count it, and raise the global stop flag with the last one
 __device__ __forceinline__ void segment_done(int *flag, int nseg) {
@@ -239,6 +175,24 @@ __global__ __launch_bounds__(kMB) void k_update(double *__restrict__ x, const do
         part[blockIdx.x] = a;
         part[gridDim.x + blockIdx.x] = b;
         part[2 * gridDim.x + blockIdx.x] = c;
+    }
+}
+
+// x += ro h; g = gp - ro q (the two-level preconditioner applies M and the dots afterwards)
+__global__ __launch_bounds__(kMB) void k_update_xg(double *__restrict__ x, const double *__restrict__ h,
+                                                   double *__restrict__ g, const double *__restrict__ gp,
+                                                   const double *__restrict__ qv, const int *__restrict__ wg_seg,
+                                                   const int *__restrict__ wg_q0, const int *__restrict__ seg_q,
+                                                   const double *__restrict__ S, const int *__restrict__ state,
+                                                   const int *__restrict__ flag) {
+    if (stopped(flag)) return;
+    const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
+    if (state[sp.r] || sp.q >= sp.qend) return;
+    const double ro = S[(int64_t) sp.r * S_NSCAL + S_RO];
+    for (int r = 0; r < 3; r++) {
+        const int i = 3 * sp.q + r;
+        x[i] = x[i] + ro * h[i];
+        g[i] = gp[i] - ro * qv[i];
     }
 }
 
@@ -505,7 +459,7 @@ int ensure_work(Problem &pb) {
 }
 
 // A v = P^T N^-1 P v in FP64 (map -> map), fused <v, Av> partials optional
-static int apply_A64(Problem &pb, const double *d_v, double *d_out, double *d_dot) {
+int apply_A64(Problem &pb, const double *d_v, double *d_out, double *d_dot) {
     pb.prof.begin(MGPU_KT_P, pb.stream);
     MGPU_RET(launch_matvec(pb, d_v, pb.d_tod, true));
     pb.prof.end(pb.stream);
@@ -548,6 +502,15 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     MGPU_RET(ensure_work(pb));
     const bool mixed = pb.opt.precision == MGPU_MIXED;
     if (mixed) MGPU_RET(pointing_make_fp32(pb));
+    // two-level preconditioner: Z, A Z and pinv(E) (build_precond, precond.c:1342-1398), timed apart
+    const bool two = pb.opt.precond != 0;
+    if (two) {
+        const bool keep = pb.prof.on;
+        pb.prof.on = false;
+        const int rc = twolvl_build(pb, d_signal, d_noise);
+        pb.prof.on = keep;
+        MGPU_RET(rc);
+    }
     hipStream_t s = pb.stream;
     const Segments &G = pb.sg;
     const int nseg = G.nseg, nwg = G.nwg;
@@ -585,8 +548,12 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, true));
     MGPU_RET(launch_trmatvec(pb, pb.d_tod2, gA, nullptr, nullptr, nullptr));
     MGPU_HIP(hipMemcpyAsync(bvec, gA, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    k_init_dots<<<nwg, kMB, 0, s>>>(pb.pc.d_inv, gA, Cg, h, wseg, wq0, sq, part);
-    MGPU_LAUNCHED();
+    if (two) {
+        MGPU_RET(twolvl_apply(pb, gA, nullptr, Cg, h, part, true, nullptr, nullptr));
+    } else {
+        k_init_dots<<<nwg, kMB, 0, s>>>(pb.pc.d_inv, gA, Cg, h, wseg, wq0, sq, part);
+        MGPU_LAUNCHED();
+    }
     k_init_scalars<<<nseg, 256, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag);
     MGPU_LAUNCHED();
 
@@ -617,8 +584,14 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         pb.prof.begin(MGPU_KT_MAP, s);
         k_coeff<<<nseg, 256, 0, s>>>(part, wg0, S, state, flag);
         MGPU_LAUNCHED();
-        k_update<<<nwg, kMB, 0, s>>>(x, h, g, gp, qv, pb.pc.d_inv, Cg, wseg, wq0, sq, S, state, part, flag);
-        MGPU_LAUNCHED();
+        if (two) {
+            k_update_xg<<<nwg, kMB, 0, s>>>(x, h, g, gp, qv, wseg, wq0, sq, S, state, flag);
+            MGPU_LAUNCHED();
+            MGPU_RET(twolvl_apply(pb, g, gp, Cg, nullptr, part, false, state, flag));
+        } else {
+            k_update<<<nwg, kMB, 0, s>>>(x, h, g, gp, qv, pb.pc.d_inv, Cg, wseg, wq0, sq, S, state, part, flag);
+            MGPU_LAUNCHED();
+        }
         pb.prof.end(s);
         if (refresh && launched % refresh == 0) {
             // g = b - A x in FP64, then Cg = M g and the three dots on the refreshed residual
@@ -628,8 +601,12 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             MGPU_RET(apply_A64(pb, x, Ax, nullptr));
             k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, g, wseg, wq0, sq, state, flag);
             MGPU_LAUNCHED();
-            k_refresh_three<<<nwg, kMB, 0, s>>>(pb.pc.d_inv, g, gp, Cg, wseg, wq0, sq, state, part, flag);
-            MGPU_LAUNCHED();
+            if (two) {
+                MGPU_RET(twolvl_apply(pb, g, gp, Cg, nullptr, part, false, state, flag));
+            } else {
+                k_refresh_three<<<nwg, kMB, 0, s>>>(pb.pc.d_inv, g, gp, Cg, wseg, wq0, sq, state, part, flag);
+                MGPU_LAUNCHED();
+            }
             pb.prof.on = keep;
             pb.prof.end(s);
         }
@@ -723,6 +700,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         rep->refreshes = refreshes;
         rep->tile_variant = pb.tp.variant[mixed ? 1 : 0];
         rep->true_res_rel = true_rel[0];
+        rep->t_2lvl_ms = two ? pb.tl.build_ms : 0.0;
     }
     if (segs) {
         segs->assign((size_t) nseg, SegReport{});

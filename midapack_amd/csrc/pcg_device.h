@@ -1,0 +1,75 @@
+// pcg_device.h -- device-side helpers shared by the PCG kernels (pcg.hip)
+// and the two-level preconditioner (twolvl.hip): per-segment scalar slots,
+// stop flags, workgroup reductions in a fixed order and the map-domain
+// workgroup -> (segment, pixel) mapping.
+#pragma once
+#include "mgpu_internal.h"
+
+namespace mgpu {
+
+// scalar slots of one segment in d_scal (segment r at r * S_NSCAL)
+enum {
+    S_G2PIX = 0,
+    S_RES,
+    S_RES0,
+    S_TOL2,
+    S_RO,
+    S_GAMMA,
+    S_COEFF,
+    S_POLAK,
+    S_G2PIXP,
+    S_K,        // iteration counter (as double)
+    S_TOL,
+    S_STOPK,    // stop_after (0 = off)
+    S_MAXK,
+    S_TRUE_R2,  // ||b - A x||^2 of the returned x (mixed precision)
+    S_TRUE_B2,  // ||b||^2
+    S_NSCAL = 16
+};
+
+// device flags: [0] every segment stopped, [1] segment 0 converged, [2] largest
+// iteration count reached, [3] number of stopped segments; [4..7] never set
+enum { F_STOP = 0, F_CONV0, F_KMAX, F_NDONE, F_NEVER = 4, F_NFLAGS = 8 };
+
+constexpr int kMB = kMapWG;  // map-domain workgroup size (pixels)
+
+__device__ __forceinline__ bool stopped(const int *flag) { return *(volatile const int *) flag != 0; }
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int) (blockDim.x >> 6); i++) s += red[i];
+    return s;
+}
+
+// fixed-order reduction of nparts partials (single workgroup of 256)
+__device__ __forceinline__ double reduce_parts(const double *__restrict__ p, int nparts, double *red) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) s += p[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// pixel handled by this thread of a map-domain workgroup, its segment and the segment end
+struct SegPix {
+    int r, q, qend;
+};
+__device__ __forceinline__ SegPix seg_pixel(const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
+                                            const int *__restrict__ seg_q) {
+    SegPix p;
+    p.r = wg_seg[blockIdx.x];
+    p.q = wg_q0[blockIdx.x] + (int) threadIdx.x;
+    p.qend = seg_q[p.r + 1];
+    return p;
+}
+
+}  // namespace mgpu

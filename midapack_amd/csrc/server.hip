@@ -163,6 +163,7 @@ int mappraiser_gpu_server_create(mgpu_server **out, double hbm_budget_gb, int ma
                                  const mgpu_options *opt) {
     MGPU_ARG(out, "NULL argument");
     MGPU_ARG(max_slots >= 1 && max_batch >= 1, "max_slots=%d / max_batch=%d must be >= 1", max_slots, max_batch);
+    MGPU_ARG(!opt || opt->precond == 0, "precond=%d: the request loop solves with block Jacobi", opt->precond);
     std::unique_ptr<Server> s(new Server());
     mappraiser_gpu_options_default(&s->opt);
     if (opt) s->opt = *opt;

@@ -49,14 +49,15 @@ KERNEL_CLASSES = ["P", "mean", "tiles", "Pt", "map", "refresh"]
 class Options(ct.Structure):
     _fields_ = [("precision", ct.c_int), ("refresh_every", ct.c_int), ("tile_log2", ct.c_int),
                 ("poll_every", ct.c_int), ("profile", ct.c_int), ("offset_mode", ct.c_int),
-                ("tile_variant", ct.c_int), ("graph", ct.c_int)]
+                ("tile_variant", ct.c_int), ("graph", ct.c_int), ("precond", ct.c_int), ("z_2lvl", ct.c_int)]
 
 
 class Report(ct.Structure):
     _fields_ = [("iterations", ct.c_int), ("converged", ct.c_int), ("res0", ct.c_double),
                 ("res", ct.c_double), ("n_map", ct.c_int), ("lcount", ct.c_int), ("trash_pix", ct.c_int),
                 ("n_cut", ct.c_int), ("t_setup_ms", ct.c_double), ("t_solve_ms", ct.c_double),
-                ("true_res_rel", ct.c_double), ("refreshes", ct.c_int), ("tile_variant", ct.c_int)]
+                ("true_res_rel", ct.c_double), ("refreshes", ct.c_int), ("tile_variant", ct.c_int),
+                ("t_2lvl_ms", ct.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -136,12 +137,15 @@ def _check(rc: int, what: str):
 
 
 def options(precision=MGPU_FP64, refresh_every=10, tile_log2=0, poll_every=8, profile=False,
-            offset_mode=None, tile_variant=0, graph=None) -> Options:
+            offset_mode=None, tile_variant=0, graph=None, precond=0, z_2lvl=0) -> Options:
+    """mgpu_options; precond / z_2lvl are MLmap's precond and Z_2lvl (0 block
+    Jacobi, 1 two-level a priori, 2 two-level a posteriori)."""
     o = Options()
     load().mappraiser_gpu_options_default(ct.byref(o))
     o.precision, o.refresh_every, o.tile_log2, o.poll_every = precision, refresh_every, tile_log2, poll_every
     o.profile = int(bool(profile))
     o.tile_variant = int(tile_variant)
+    o.precond, o.z_2lvl = int(precond), int(z_2lvl)
     if graph is not None:
         o.graph = int(bool(graph))
     if offset_mode is not None:

@@ -938,6 +938,373 @@ void orc_getlocalW(int m, int nnz, const int *indices, const double *values, int
 }
 
 /* ========================================================================
+ * Small dense kernels of the two-level preconditioners
+ * ====================================================================== */
+
+/* Pseudo-inverse of the row-major n x n matrix a: the minimum-norm solution X
+ * of min ||a X - I|| with singular values s_j <= rcond * s_max treated as zero,
+ * which is what inverse_svd_bis obtains from LAPACKE_dgelss against the
+ * identity (precond.c:777-796, rcond = eps = 1e-15, precond.c:25).  The SVD is
+ * the one-sided Jacobi method (Hestenes 1958; Demmel & Veselic, SIAM J. Matrix
+ * Anal. Appl. 13 (1992)): rotate column pairs of W = a until they are
+ * orthogonal, so that W = U S and a = U S V^T with V the accumulated rotations;
+ * then pinv(a) = sum_j v_j w_j^T / s_j^2 over the kept j. */
+int orc_pinv(int n, const double *a, double rcond, double *out) {
+    if (n <= 0) return ORC_EINVAL;
+    double *W = (double *) malloc((size_t) n * n * sizeof(double));
+    double *V = (double *) calloc((size_t) n * n, sizeof(double));
+    if (!W || !V) {
+        free(W);
+        free(V);
+        return ORC_ENOMEM;
+    }
+    memcpy(W, a, (size_t) n * n * sizeof(double));
+    for (int i = 0; i < n; i++) V[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 80; sweep++) {
+        int rotated = 0;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double al = 0, be = 0, ga = 0;
+                for (int i = 0; i < n; i++) {
+                    al += W[i * n + p] * W[i * n + p];
+                    be += W[i * n + q] * W[i * n + q];
+                    ga += W[i * n + p] * W[i * n + q];
+                }
+                if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+                rotated = 1;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+                for (int i = 0; i < n; i++) {
+                    const double wp = W[i * n + p], wq = W[i * n + q];
+                    W[i * n + p] = c * wp - sn * wq;
+                    W[i * n + q] = sn * wp + c * wq;
+                    const double vp = V[i * n + p], vq = V[i * n + q];
+                    V[i * n + p] = c * vp - sn * vq;
+                    V[i * n + q] = sn * vp + c * vq;
+                }
+            }
+        if (!rotated) break;
+    }
+    double smax = 0;
+    double *s2 = (double *) malloc((size_t) n * sizeof(double));
+    for (int j = 0; j < n; j++) {
+        double t = 0;
+        for (int i = 0; i < n; i++) t += W[i * n + j] * W[i * n + j];
+        s2[j] = t;
+        if (sqrt(t) > smax) smax = sqrt(t);
+    }
+    for (int i = 0; i < n * n; i++) out[i] = 0.0;
+    for (int j = 0; j < n; j++) {
+        if (!(sqrt(s2[j]) > rcond * smax)) continue;
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < n; k++) out[i * n + k] += V[i * n + j] * W[k * n + j] / s2[j];
+    }
+    free(s2);
+    free(W);
+    free(V);
+    return ORC_OK;
+}
+
+/* Eigen-decomposition of the symmetric row-major n x n matrix a by cyclic
+ * Jacobi rotations (Golub & Van Loan, Matrix Computations, 4th ed., 8.5):
+ * eigenvalues ascending in w, eigenvector j in column j of the row-major v --
+ * the layout LAPACKE_dsyev(LAPACK_ROW_MAJOR, 'V', ...) leaves in its matrix
+ * argument (precond.c:1240). */
+int orc_symeig(int n, const double *a, double *w, double *v) {
+    if (n <= 0) return ORC_EINVAL;
+    double *A = (double *) malloc((size_t) n * n * sizeof(double));
+    double *Vt = (double *) calloc((size_t) n * n, sizeof(double));
+    int *ord = (int *) malloc((size_t) n * sizeof(int));
+    if (!A || !Vt || !ord) {
+        free(A);
+        free(Vt);
+        free(ord);
+        return ORC_ENOMEM;
+    }
+    memcpy(A, a, (size_t) n * n * sizeof(double));
+    for (int i = 0; i < n; i++) Vt[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                tot += A[i * n + j] * A[i * n + j];
+                if (i != j) off += A[i * n + j] * A[i * n + j];
+            }
+        if (off == 0.0 || off <= 1e-32 * tot) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A[p * n + q];
+                if (apq == 0.0) continue;
+                const double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; k++) { /* A <- A J (columns p, q) */
+                    const double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = c * akp - s * akq;
+                    A[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) { /* A <- J^T A (rows p, q) */
+                    const double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = c * apk - s * aqk;
+                    A[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) { /* V <- V J */
+                    const double vkp = Vt[k * n + p], vkq = Vt[k * n + q];
+                    Vt[k * n + p] = c * vkp - s * vkq;
+                    Vt[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) ord[i] = i;
+    for (int i = 1; i < n; i++) { /* insertion sort by eigenvalue */
+        const int o = ord[i];
+        int j = i - 1;
+        while (j >= 0 && A[ord[j] * n + ord[j]] > A[o * n + o]) {
+            ord[j + 1] = ord[j];
+            j--;
+        }
+        ord[j + 1] = o;
+    }
+    for (int j = 0; j < n; j++) {
+        w[j] = A[ord[j] * n + ord[j]];
+        for (int i = 0; i < n; i++) v[i * n + j] = Vt[i * n + ord[j]];
+    }
+    free(A);
+    free(Vt);
+    free(ord);
+    return ORC_OK;
+}
+
+/* ========================================================================
+ * Two-level preconditioners (build_precond precond = 1, 2, precond.c:1342-1398)
+ * ====================================================================== */
+
+typedef struct {
+    int Zn, n;
+    double *Z, *AZ; /* column j at [j * n] */
+    double *Em1;    /* Zn x Zn row-major */
+    double *w, *Qtx, *Qg, *AQg;
+} orc_twolvl;
+
+typedef struct {
+    int m, nnz, trash, lcount, nb, lambda;
+    const int *pix, *block_sizes;
+    const double *w, *invtt;
+} orc_op;
+
+/* A v = P^T stbmmProd(P v), the product of build_AZ (precond.c:1017-1022) and
+ * Lanczos_eig (precond.c:1151-1153, 1205-1207): stbmmProd, not apply_weights,
+ * so lambda == 1 also removes the offsets there. */
+static int op_apply(const orc_op *o, const double *v, double *tod, double *out) {
+    orc_matvec(o->m, o->nnz, o->pix, o->w, o->trash, v, tod);
+    int rc = orc_stbmm(o->nb, o->block_sizes, o->lambda, o->invtt, o->m, tod, 1);
+    if (rc < 0) return rc;
+    orc_trmatvec(o->m, o->nnz, o->pix, o->w, o->trash, o->lcount, tod, out);
+    return ORC_OK;
+}
+
+/* build_Z on one rank (precond.c:798-953): group = Zn (Zn > 1) or 1, chunk g =
+ * samples [g*(m/group), (g+1)*(m/group)); count[g*lcount + i] = occurrences of
+ * local index i in chunk g; column g at local index i (step nnz) =
+ * count[g*group + i] / tcount[i] -- the reference's own index (precond.c:874),
+ * kept as written -- then the trash entries are dropped (precond.c:943-945). */
+static int build_Z(const orc_op *o, int Zn, double *Z, int n) {
+    const int lcount = o->lcount, nnz = o->nnz;
+    const int group = Zn > 1 ? Zn : 1;
+    int *count = (int *) calloc((size_t) group * lcount, sizeof(int));
+    int *tcount = (int *) calloc((size_t) (lcount > 0 ? lcount : 1), sizeof(int));
+    double *col = (double *) malloc((size_t) (lcount > 0 ? lcount : 1) * sizeof(double));
+    if (!count || !tcount || !col) {
+        free(count);
+        free(tcount);
+        free(col);
+        return ORC_ENOMEM;
+    }
+    const int64_t per = (int64_t) (o->m / group) * nnz;
+    for (int g = 0; g < group; g++)
+        for (int64_t i = 0; i < per; i++) count[(int64_t) g * lcount + o->pix[g * per + i]]++;
+    for (int g = 0; g < group; g++)
+        for (int i = 0; i < lcount; i++) tcount[i] += count[(int64_t) g * lcount + i];
+    for (int g = 0; g < Zn; g++) {
+        for (int i = 0; i < lcount; i++) col[i] = 0.0;
+        for (int i = 0; i < lcount; i += nnz)
+            col[i] = (double) ((double) count[(int64_t) g * group + i] / (double) tcount[i]);
+        for (int i = 0; i < n; i++) Z[(int64_t) g * n + i] = col[i + nnz * o->trash];
+    }
+    free(count);
+    free(tcount);
+    free(col);
+    return ORC_OK;
+}
+
+static double dot_seq(int n, const double *a, const double *b);
+
+/* Lanczos_eig (precond.c:1073-1306) from x = 0: K steps on A from
+ * w = P^T N^-1 (b + noise), the Ritz vectors of the K x K tridiagonal T and
+ * their images under A.  Kept as written: the first step subtracts no
+ * beta * v_prev term (vold starts at 0, precond.c:1214-1218). */
+static int lanczos(const orc_op *o, const double *b, const double *noise, int K, int n, double *Z, double *AZ) {
+    const double eps = 1.0e-15;
+    const int m = o->m;
+    const int K1 = K + 1;
+    double *tod = (double *) malloc((size_t) (m > 0 ? m : 1) * sizeof(double));
+    double *w = (double *) calloc((size_t) n, sizeof(double));
+    double *v = (double *) calloc((size_t) n, sizeof(double));
+    double *vold = (double *) calloc((size_t) n, sizeof(double));
+    double *Av = (double *) calloc((size_t) n, sizeof(double));
+    double *V = (double *) calloc((size_t) n * K1, sizeof(double));     /* column k at [k*n] */
+    double *AmulV = (double *) calloc((size_t) n * K1, sizeof(double)); /* column k at [k*n] */
+    double *T = (double *) calloc((size_t) K1 * K1, sizeof(double));
+    double *Tk = (double *) calloc((size_t) K * K, sizeof(double));
+    double *ev = (double *) calloc((size_t) K * K, sizeof(double));
+    double *lam = (double *) calloc((size_t) K, sizeof(double));
+    int rc = ORC_ENOMEM;
+    if (!tod || !w || !v || !vold || !Av || !V || !AmulV || !T || !Tk || !ev || !lam) goto out;
+    for (int i = 0; i < m; i++) tod[i] = b[i] + noise[i] - 0.0; /* P x with x = 0 */
+    rc = orc_stbmm(o->nb, o->block_sizes, o->lambda, o->invtt, m, tod, 1);
+    if (rc < 0) goto out;
+    orc_trmatvec(m, o->nnz, o->pix, o->w, o->trash, o->lcount, tod, w);
+    double beta = sqrt(dot_seq(n, w, w));
+    if (beta > eps)
+        for (int i = 0; i < n; i++) {
+            v[i] = w[i] / beta;
+            V[i] = v[i];
+        }
+    if ((rc = op_apply(o, v, tod, Av)) < 0) goto out;
+    memcpy(w, Av, (size_t) n * sizeof(double));
+    double alpha = dot_seq(n, v, Av);
+    for (int i = 0; i < n; i++) {
+        AmulV[i] = w[i];
+        w[i] = w[i] - (alpha * v[i]);
+    }
+    for (int it = 0; it < K; it++) {
+        beta = sqrt(dot_seq(n, w, w));
+        if (beta > eps)
+            for (int j = 0; j < n; j++) {
+                v[j] = w[j] / beta;
+                V[(int64_t) (it + 1) * n + j] = v[j];
+            }
+        T[it * K1 + it] = alpha;
+        T[it * K1 + it + 1] = beta;
+        T[(it + 1) * K1 + it] = beta;
+        if ((rc = op_apply(o, v, tod, Av)) < 0) goto out;
+        memcpy(w, Av, (size_t) n * sizeof(double));
+        alpha = dot_seq(n, v, Av);
+        for (int j = 0; j < n; j++) {
+            AmulV[(int64_t) (it + 1) * n + j] = w[j];
+            w[j] = w[j] - (alpha * v[j]) - (beta * vold[j]);
+            vold[j] = v[j];
+        }
+    }
+    for (int i = 0; i < K; i++)
+        for (int j = 0; j < K; j++) Tk[i * K + j] = T[i * K1 + j];
+    if ((rc = orc_symeig(K, Tk, lam, ev)) < 0) goto out;
+    /* Ritz vectors: Z = V ev, AZ = AmulV ev over the first K columns (precond.c:1254-1282) */
+    for (int j = 0; j < K; j++)
+        for (int i = 0; i < n; i++) {
+            double z = 0.0, az = 0.0;
+            for (int k = 0; k < K; k++) {
+                z += V[(int64_t) k * n + i] * ev[k * K + j];
+                az += AmulV[(int64_t) k * n + i] * ev[k * K + j];
+            }
+            Z[(int64_t) j * n + i] = z;
+            AZ[(int64_t) j * n + i] = az;
+        }
+    rc = ORC_OK;
+out:
+    free(tod);
+    free(w);
+    free(v);
+    free(vold);
+    free(Av);
+    free(V);
+    free(AmulV);
+    free(T);
+    free(Tk);
+    free(ev);
+    free(lam);
+    return rc;
+}
+
+static void twolvl_free(orc_twolvl *t) {
+    free(t->Z);
+    free(t->AZ);
+    free(t->Em1);
+    free(t->w);
+    free(t->Qtx);
+    free(t->Qg);
+    free(t->AQg);
+    memset(t, 0, sizeof(*t));
+}
+
+/* build_precond for precond = 1 (build_Z + build_AZ) or 2 (Lanczos_eig), then
+ * build_Em1 (precond.c:956-1003): E[i*Zn+j] = <Z_j, AZ_i>, Em1 = pinv(E). */
+static int twolvl_build(orc_twolvl *t, const orc_op *o, int precond, int Zn, int n, const double *b,
+                        const double *noise) {
+    memset(t, 0, sizeof(*t));
+    t->Zn = Zn;
+    t->n = n;
+    const size_t nz = (size_t) Zn * (n > 0 ? n : 1);
+    t->Z = (double *) calloc(nz, sizeof(double));
+    t->AZ = (double *) calloc(nz, sizeof(double));
+    t->Em1 = (double *) calloc((size_t) Zn * Zn, sizeof(double));
+    t->w = (double *) calloc((size_t) Zn, sizeof(double));
+    t->Qtx = (double *) calloc((size_t) Zn, sizeof(double));
+    t->Qg = (double *) calloc((size_t) (n > 0 ? n : 1), sizeof(double));
+    t->AQg = (double *) calloc((size_t) (n > 0 ? n : 1), sizeof(double));
+    double *tod = (double *) malloc((size_t) (o->m > 0 ? o->m : 1) * sizeof(double));
+    double *E = (double *) calloc((size_t) Zn * Zn, sizeof(double));
+    int rc = ORC_ENOMEM;
+    if (!t->Z || !t->AZ || !t->Em1 || !t->w || !t->Qtx || !t->Qg || !t->AQg || !tod || !E) goto out;
+    if (precond == 1) {
+        if ((rc = build_Z(o, Zn, t->Z, n)) < 0) goto out;
+        for (int i = 0; i < Zn; i++)
+            if ((rc = op_apply(o, t->Z + (int64_t) i * n, tod, t->AZ + (int64_t) i * n)) < 0) goto out;
+    } else {
+        if ((rc = lanczos(o, b, noise, Zn, n, t->Z, t->AZ)) < 0) goto out;
+    }
+    for (int i = 0; i < Zn; i++)
+        for (int j = 0; j < Zn; j++)
+            for (int k = 0; k < n; k++)
+                E[i * Zn + j] += (double) (t->Z[(int64_t) j * n + k] * t->AZ[(int64_t) i * n + k] * 1.0);
+    rc = orc_pinv(Zn, E, 1.0e-15, t->Em1);
+out:
+    free(tod);
+    free(E);
+    if (rc < 0) twolvl_free(t);
+    return rc;
+}
+
+/* apply_precond, two-level branch (precond.c:1412-1422 with build_Qtx 1032-1055
+ * and mul_ZQtx 1057-1071): Cg = BJ (g - AZ Qtx) + Z Qtx, Qtx = Em1 Z^T g. */
+static void twolvl_apply(orc_twolvl *t, int nnz, const double *inv, const double *g, double *Cg) {
+    const int n = t->n, Zn = t->Zn;
+    for (int j = 0; j < Zn; j++) {
+        t->w[j] = 0.0;
+        for (int k = 0; k < n; k++) t->w[j] += t->Z[(int64_t) j * n + k] * g[k] * 1.0;
+    }
+    for (int i = 0; i < Zn; i++) {
+        t->Qtx[i] = 0.0;
+        for (int j = 0; j < Zn; j++) t->Qtx[i] += t->Em1[i * Zn + j] * t->w[j];
+    }
+    for (int k = 0; k < n; k++) t->Qg[k] = 0.0;
+    for (int j = 0; j < Zn; j++)
+        for (int k = 0; k < n; k++) t->Qg[k] += t->Z[(int64_t) j * n + k] * t->Qtx[j];
+    for (int k = 0; k < n; k++) t->AQg[k] = 0.0;
+    for (int j = 0; j < Zn; j++)
+        for (int k = 0; k < n; k++) t->AQg[k] += t->AZ[(int64_t) j * n + k] * t->Qtx[j];
+    for (int i = 0; i < n; i++) t->AQg[i] = g[i] - t->AQg[i];
+    for (int i = 0; i < n; i++) {
+        const int q = i / nnz;
+        double y = 0.0;
+        for (int j = 0; j < nnz; j++) y += inv[(int64_t) i * nnz + j] * t->AQg[q * nnz + j];
+        Cg[i] = y;
+    }
+    for (int i = 0; i < n; i++) Cg[i] += t->Qg[i];
+}
+
+/* ========================================================================
  * Whole solve: MLmap setup + build_precond + PCG_GLS_true
  * ====================================================================== */
 
@@ -963,6 +1330,19 @@ int orc_gls_solve(int m, int nnz, int *pix, double *pixweights, double *signal,
                   const double *invtt, double tol, int maxiter, int stop_after,
                   double *x_out, int *lindices_out, double *cond_out, int *lhits_out,
                   double *res_hist, orc_report *rep) {
+    return orc_gls_solve2(m, nnz, pix, pixweights, signal, noise, nb, block_sizes, lambda, invtt, tol, maxiter,
+                          stop_after, 0, 0, x_out, lindices_out, cond_out, lhits_out, res_hist, NULL, NULL, NULL,
+                          rep);
+}
+
+int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
+                   const double *noise, int nb, const int *block_sizes, int lambda,
+                   const double *invtt, double tol, int maxiter, int stop_after, int precond, int Z_2lvl,
+                   double *x_out, int *lindices_out, double *cond_out, int *lhits_out,
+                   double *res_hist, double *z_out, double *az_out, double *em1_out, orc_report *rep) {
+    if (precond < 0 || precond > 2 || Z_2lvl < 0) return ORC_EINVAL;
+    /* PCG_GLS_true: Z_2lvl == 0 means the communicator size (pcg_true.c:50) */
+    const int Zn = Z_2lvl == 0 ? 1 : Z_2lvl;
     const int64_t mn = (int64_t) m * nnz;
     int lcount = 0, trash = 0, rc;
     int *lind = (int *) malloc((size_t) (mn > 0 ? mn : 1) * sizeof(int));
@@ -1036,6 +1416,26 @@ int orc_gls_solve(int m, int nnz, int *pix, double *pixweights, double *signal,
     if (rc) return rc;
     n = lcount - nnz * trash;
 
+    /* two-level part of build_precond (precond.c:1342-1398) */
+    orc_twolvl tl;
+    memset(&tl, 0, sizeof(tl));
+    if (precond != 0) {
+        if (Zn > n) return ORC_EINVAL;
+        orc_op op = {m, nnz, trash, lcount, nb, lambda, pix, block_sizes, pixweights, invtt};
+        rc = twolvl_build(&tl, &op, precond, Zn, n, signal, noise);
+        if (rc < 0) return rc;
+        if (z_out) memcpy(z_out, tl.Z, (size_t) Zn * n * sizeof(double));
+        if (az_out) memcpy(az_out, tl.AZ, (size_t) Zn * n * sizeof(double));
+        if (em1_out) memcpy(em1_out, tl.Em1, (size_t) Zn * Zn * sizeof(double));
+    }
+#define PRECOND_APPLY(G, CG) \
+    do { \
+        if (precond) \
+            twolvl_apply(&tl, nnz, Winv, (G), (CG)); \
+        else \
+            bj_apply(n, nnz, Winv, (G), (CG)); \
+    } while (0)
+
     /* PCG_GLS_true (pcg_true.c:20-241) */
     double *x = x_out;
     for (int i = 0; i < n; i++) x[i] = 0.0;
@@ -1051,7 +1451,7 @@ int orc_gls_solve(int m, int nnz, int *pix, double *pixweights, double *signal,
     rc = orc_apply_weights(nb, block_sizes, lambda, invtt, m, Ah, 1);
     if (rc < 0) return rc;
     orc_trmatvec(m, nnz, pix, pixweights, trash, lcount, Ah, g);
-    bj_apply(n, nnz, Winv, g, Cg);
+    PRECOND_APPLY(g, Cg);
     for (int j = 0; j < n; j++) h[j] = Cg[j];
     double g2pix = dot_seq(n, Cg, g);
     double res = dot_seq(n, g, g);
@@ -1074,7 +1474,7 @@ int orc_gls_solve(int m, int nnz, int *pix, double *pixweights, double *signal,
         double ro = g2pix / coeff;
         for (int j = 0; j < n; j++) x[j] = x[j] + ro * h[j];
         for (int j = 0; j < n; j++) g[j] = gp[j] - ro * AtNm1Ah[j];
-        bj_apply(n, nnz, Winv, g, Cg);
+        PRECOND_APPLY(g, Cg);
         double g2pixp = g2pix;
         g2pix = dot_seq(n, Cg, g);
         double g2pix_polak = 0.0;
@@ -1114,5 +1514,7 @@ int orc_gls_solve(int m, int nnz, int *pix, double *pixweights, double *signal,
     free(gp);
     free(AtNm1Ah);
     free(Ah);
+    twolvl_free(&tl);
+#undef PRECOND_APPLY
     return ORC_OK;
 }

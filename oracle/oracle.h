@@ -132,6 +132,25 @@ int orc_gls_solve(int m, int nnz, int *pix, double *pixweights, double *signal,
                   double *x_out, int *lindices_out, double *cond_out, int *lhits_out,
                   double *res_hist, orc_report *rep);
 
+/* orc_gls_solve with the preconditioner choice of PCG_GLS_true (pcg_true.c:52):
+ * precond 0 = block Jacobi, 1 = two-level a priori (build_Z, precond.c:798-953),
+ * 2 = two-level a posteriori (Lanczos_eig, precond.c:1073-1306); Z_2lvl = number
+ * of deflation vectors (0 -> 1, the one-rank communicator size).  When not
+ * NULL, z_out / az_out receive the Zn columns of Z and A Z (column j at j*n_map)
+ * and em1_out the Zn x Zn row-major pseudo-inverse of E (precond.c:956-1003). */
+int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
+                   const double *noise, int nb, const int *block_sizes, int lambda,
+                   const double *invtt, double tol, int maxiter, int stop_after, int precond, int Z_2lvl,
+                   double *x_out, int *lindices_out, double *cond_out, int *lhits_out,
+                   double *res_hist, double *z_out, double *az_out, double *em1_out, orc_report *rep);
+
+/* pinv(a) with singular values <= rcond * s_max dropped (inverse_svd_bis,
+ * precond.c:777-796), one-sided Jacobi SVD; row-major n x n. */
+int orc_pinv(int n, const double *a, double rcond, double *out);
+/* symmetric eigen-decomposition (LAPACKE_dsyev row-major 'V', precond.c:1240):
+ * ascending eigenvalues w, eigenvector j in column j of row-major v. */
+int orc_symeig(int n, const double *a, double *w, double *v);
+
 #ifdef __cplusplus
 }
 #endif

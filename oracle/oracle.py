@@ -69,6 +69,12 @@ def lib():
         L.orc_gls_solve.argtypes = [ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p, ct.c_int, _i32p, ct.c_int,
                                     _f64p, ct.c_double, ct.c_int, ct.c_int, _f64p, _i32p, _f64p, _i32p, _f64p,
                                     ct.POINTER(OracleReport)]
+        L.orc_gls_solve2.argtypes = [ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p, ct.c_int, _i32p, ct.c_int,
+                                     _f64p, ct.c_double, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _f64p, _i32p,
+                                     _f64p, _i32p, _f64p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                     ct.POINTER(OracleReport)]
+        L.orc_pinv.argtypes = [ct.c_int, _f64p, ct.c_double, _f64p]
+        L.orc_symeig.argtypes = [ct.c_int, _f64p, _f64p, _f64p]
         _LIB = L
     return _LIB
 
@@ -290,9 +296,35 @@ class SolveResult:
     signal: np.ndarray
 
 
+def pinv(a, rcond=1e-15):
+    """inverse_svd_bis (precond.c:777-796): pseudo-inverse, singular values
+    <= rcond * s_max dropped."""
+    a = _c(a, np.float64)
+    n = a.shape[0]
+    out = np.zeros((n, n), np.float64)
+    if lib().orc_pinv(n, a.ravel().copy(), float(rcond), out.ravel()):
+        raise RuntimeError("orc_pinv failed")
+    return out
+
+
+def symeig(a):
+    """LAPACKE_dsyev row-major 'V' (precond.c:1240): ascending eigenvalues and
+    the eigenvectors as columns."""
+    a = _c(a, np.float64)
+    n = a.shape[0]
+    w = np.zeros(n, np.float64)
+    v = np.zeros((n, n), np.float64)
+    if lib().orc_symeig(n, a.ravel().copy(), w, v.ravel()):
+        raise RuntimeError("orc_symeig failed")
+    return w, v
+
+
 def gls_solve(pix, weights, signal, noise, block_sizes, lam, invtt, nnz=3, tol=1e-8, maxiter=500,
-              stop_after=0):
-    """MLmap setup + PCG_GLS_true on one rank (copies; inputs are not modified)."""
+              stop_after=0, precond=0, z_2lvl=0, deflation=False):
+    """MLmap setup + PCG_GLS_true on one rank (copies; inputs are not modified).
+    precond / z_2lvl as in MLmap (0 block Jacobi, 1 two-level a priori, 2
+    two-level a posteriori).  deflation=True also returns Z, AZ (n_map x Zn)
+    and Em1 (Zn x Zn) as result.Z, result.AZ, result.Em1."""
     idx = _c(pix, np.int32).copy()
     w = _c(weights, np.float64).copy()
     s = _c(signal, np.float64).copy()
@@ -306,13 +338,27 @@ def gls_solve(pix, weights, signal, noise, block_sizes, lam, invtt, nnz=3, tol=1
     hits = np.zeros(max(m, 1) + cap // nnz, np.int32)
     hist = np.zeros(max(maxiter, 1), np.float64)
     rep = OracleReport()
-    rc = lib().orc_gls_solve(m, nnz, idx, w, s, nz, bs.size, bs, int(lam), _c(invtt, np.float64), float(tol),
-                             int(maxiter), int(stop_after), x, lind, cond, hits, hist, ct.byref(rep))
+    zn = z_2lvl if z_2lvl > 0 else 1
+    Zb = AZb = Eb = None
+    if deflation and precond:
+        Zb = np.zeros(zn * cap, np.float64)
+        AZb = np.zeros(zn * cap, np.float64)
+        Eb = np.zeros(zn * zn, np.float64)
+    ptr = lambda a: None if a is None else a.ctypes.data_as(ct.c_void_p)
+    rc = lib().orc_gls_solve2(m, nnz, idx, w, s, nz, bs.size, bs, int(lam), _c(invtt, np.float64), float(tol),
+                              int(maxiter), int(stop_after), int(precond), int(z_2lvl), x, lind, cond, hits, hist,
+                              ptr(Zb), ptr(AZb), ptr(Eb), ct.byref(rep))
     if rc:
-        raise RuntimeError(f"orc_gls_solve rc={rc}")
+        raise RuntimeError(f"orc_gls_solve2 rc={rc}")
     n = rep.n_map
     nh = min(rep.iterations + 1, maxiter) if maxiter > 0 else 0
-    return SolveResult(x=x[:n].copy(), lindices=lind[: rep.lcount].copy(), cond=cond[: n // nnz].copy(),
+    res = SolveResult(x=x[:n].copy(), lindices=lind[: rep.lcount].copy(), cond=cond[: n // nnz].copy(),
                        hits=hits[: n // nnz].copy(), res_hist=hist[:nh].copy(), iterations=rep.iterations,
                        converged=bool(rep.converged), res0=rep.res0, res=rep.res, n_cut=rep.n_cut,
                        trash_pix=rep.trash_pix, pix_local=idx, weights=w, signal=s)
+    if Zb is not None:
+        n = rep.n_map
+        res.Z = Zb[: zn * n].reshape(zn, n).T.copy()
+        res.AZ = AZb[: zn * n].reshape(zn, n).T.copy()
+        res.Em1 = Eb.reshape(zn, zn).copy()
+    return res
