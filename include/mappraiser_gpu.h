@@ -274,6 +274,57 @@ int mappraiser_gpu_pcg_gls(const char *outpath, const char *ref, int m, int nnz,
                            double tol, int maxiter, double *x, double *cond, int *lhits,
                            int *lindices, const mgpu_options *opt, mgpu_report *rep);
 
+/* ---- MLmap and its outputs ---- */
+/* data type codes of mappraiser_gpu_write_map (cfitsio's TDOUBLE / TINT values,
+ * so the reference's write_map calls carry over unchanged) */
+#define MGPU_TDOUBLE 82
+#define MGPU_TINT 31
+
+/* x2map_pol (mappraiser.c:836-853): scatter the local map x[xsize] (I,Q,U per
+ * pixel, global indices lstid = 3p+k) into full maps mapI/mapQ/mapU, Cond and
+ * hits (12 nside^2 entries each).  lstid[i]/3 must index those maps. */
+int mappraiser_gpu_x2map_pol(double *mapI, double *mapQ, double *mapU, double *Cond, int *hits,
+                             const double *x, const int *lstid, const double *cond, const int *lhits,
+                             int xsize);
+
+/* write_map (iofiles.c:19-53): a HEALPix map of 12 nside^2 values (type
+ * MGPU_TDOUBLE or MGPU_TINT) as a FITS binary table with one 8-byte float
+ * column "SIGNAL", ORDERING NESTED if nest else RING, COORDSYS from coordsys[0].
+ * Written without cfitsio; byte-identical to the reference's files apart from
+ * the DATE value.  An existing file is overwritten. */
+int mappraiser_gpu_write_map(const void *signal, int type, long nside, const char *filename, char nest,
+                             const char *coordsys);
+
+/* MLmap (mappraiser.c:34-380, declared in map.h:10-13) for ONE rank on one GPU:
+ * the same arguments, so the reference's ctypes wrapper
+ * (mappraiser_wrapper/__init__.py:58-84) binds it unchanged.  comm is not read
+ * (call it from a one-rank communicator; MPICH's int and Open MPI's pointer
+ * handles both arrive in this first integer argument); m = data_size_proc[0].
+ * solver must be 0 (PCG); precond / Z_2lvl select block Jacobi or the
+ * two-level preconditioners.  The solve is mappraiser_gpu_pcg_gls with default
+ * options (FP64, reference offsets; the environment variables
+ * MGPU_PRECISION=mixed and MGPU_OFFSET_MODE=symmetric|none override them),
+ * with the same side effects on pix, pixweights and signal and the residual
+ * file <outpath>/pcg_residuals_<ref>.dat; then x2map_pol and the five FITS
+ * maps <outpath>/{mapI,mapQ,mapU,Cond,Hits}_<ref>.fits (old files removed
+ * first, mappraiser.c:297-355).  void like the reference: failures are
+ * printed to stderr and left in mappraiser_gpu_last_error (no exit). */
+void MLmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int Z_2lvl, int pointing_commflag,
+           double tol, int maxiter, int enlFac, int ortho_alg, int bs_red, int nside, void *data_size_proc,
+           int nb_blocks_loc, void *local_blocks_sizes, int Nnz, void *pix, void *pixweights, void *signal,
+           double *noise, int lambda, double *invtt);
+
+/* MTmap (mappraiser.c:387 ff., map.h:15-18): exported with the reference's
+ * signature so its ctypes wrapper loads, but template map-making
+ * (PCG_GLS_templates) is not provided: it prints an error, sets
+ * mappraiser_gpu_last_error and returns. */
+void MTmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int Z_2lvl, int pointing_commflag,
+           int npoly, int nhwp, double delta_t, int ground, int n_sss_bins, double tol, int maxiter, int enlFac,
+           int ortho_alg, int bs_red, int nside, int **sweeptstamps, int *nsweeps, double **az, double *az_min,
+           double *az_max, double **hwp_angle, int nces, void *data_size_proc, int nb_blocks_loc,
+           void *local_blocks_sizes, int Nnz, void *pix, void *pixweights, void *signal, double *noise,
+           double sampling_freq, double *invtt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmappraiser_gpu.so")
-SOURCES = ["api.hip", "pointing.hip", "toeplitz.hip", "precond.hip", "pcg.hip", "batch.hip", "server.hip", "noise.hip", "twolvl.hip"]
+SOURCES = ["api.hip", "pointing.hip", "toeplitz.hip", "precond.hip", "pcg.hip", "batch.hip", "server.hip", "noise.hip", "twolvl.hip", "mlmap.hip"]
 ARCH = os.environ.get("MGPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

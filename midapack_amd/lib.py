@@ -40,7 +40,11 @@ EXPORTS = [
     "mappraiser_gpu_tile_variants", "mappraiser_gpu_pcg_batch", "mappraiser_gpu_server_create",
     "mappraiser_gpu_server_submit", "mappraiser_gpu_server_wait", "mappraiser_gpu_server_stats",
     "mappraiser_gpu_server_destroy", "mappraiser_gpu_noise_invtt", "mappraiser_gpu_pcg_gls",
+    "mappraiser_gpu_x2map_pol", "mappraiser_gpu_write_map",
 ]
+# the reference's own entry points, exported with its signatures (map.h:10-18)
+REFERENCE_EXPORTS = ["MLmap", "MTmap"]
+MGPU_TDOUBLE, MGPU_TINT = 82, 31
 
 # kernel classes of mappraiser_gpu_kernel_times (MGPU_KT_*)
 KERNEL_CLASSES = ["P", "mean", "tiles", "Pt", "map", "refresh"]
@@ -123,6 +127,14 @@ def load():
     L.mappraiser_gpu_pcg_gls.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p,
                                          ct.c_int, _i32p, ct.c_int, _f64p, ct.c_double, ct.c_int, _f64p, _f64p,
                                          _i32p, _i32p, ct.POINTER(Options), ct.POINTER(Report)]
+    L.mappraiser_gpu_x2map_pol.argtypes = [_f64p, _f64p, _f64p, _f64p, _i32p, _f64p, _i32p, _f64p, _i32p, ct.c_int]
+    L.mappraiser_gpu_write_map.argtypes = [_vp, ct.c_int, ct.c_long, ct.c_char_p, ct.c_char, ct.c_char_p]
+    # MLmap with the argtypes of the reference's wrapper (mappraiser_wrapper/__init__.py:58-84),
+    # the communicator passed as an int handle
+    L.MLmap.restype = None
+    L.MLmap.argtypes = [ct.c_int, ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double,
+                        ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _i32p, ct.c_int, _i32p, ct.c_int, _i32p,
+                        _f64p, _f64p, _f64p, ct.c_int, _f64p]
     _LIB = L
     return L
 
@@ -151,6 +163,69 @@ def options(precision=MGPU_FP64, refresh_every=10, tile_log2=0, poll_every=8, pr
     if offset_mode is not None:
         o.offset_mode = int(offset_mode)
     return o
+
+
+def x2map_pol(x, lstid, cond, lhits, nside):
+    """x2map_pol (mappraiser.c:836-853) into zeroed HEALPix maps of 12 nside^2
+    pixels: returns mapI, mapQ, mapU, Cond, hits."""
+    npix = 12 * nside * nside
+    lstid = _c(lstid, np.int32)
+    if lstid.size and (lstid.min() < 0 or lstid.max() // 3 >= npix):
+        raise GPUError("map index outside the HEALPix map")
+    out = [np.zeros(npix) for _ in range(4)]
+    hits = np.zeros(npix, np.int32)
+    x = _c(x, np.float64)
+    _check(load().mappraiser_gpu_x2map_pol(out[0], out[1], out[2], out[3], hits, x, lstid, _c(cond, np.float64),
+                                           _c(lhits, np.int32), int(x.size)), "x2map_pol")
+    return out[0], out[1], out[2], out[3], hits
+
+
+def write_map(data, nside, filename, nest=True, coordsys="C"):
+    """write_map (iofiles.c:19-53): HEALPix FITS map; float64 -> TDOUBLE, int32 -> TINT."""
+    a = np.ascontiguousarray(data)
+    if a.dtype not in (np.float64, np.int32) or a.size != 12 * nside * nside:
+        raise GPUError("write_map: float64 or int32 data of 12 nside^2 values")
+    typ = MGPU_TDOUBLE if a.dtype == np.float64 else MGPU_TINT
+    _check(load().mappraiser_gpu_write_map(a.ctypes.data, typ, int(nside), filename.encode(),
+                                           bytes([1 if nest else 0]), coordsys.encode()), "write_map")
+
+
+def MLmap(outpath, ref, precond, Z_2lvl, tol, maxiter, nside, local_blocks_sizes, nnz, pix, pixweights, signal,
+          noise, lam, invtt, solver=0):
+    """MLmap called exactly as the reference's wrapper calls it (one rank);
+    pix, pixweights and signal are modified in place like the reference's."""
+    bs = _c(local_blocks_sizes, np.int32)
+    m = np.array([_c(signal, np.float64).size], np.int32)
+    load().MLmap(0, outpath.encode(), ref.encode(), int(solver), int(precond), int(Z_2lvl), 6, float(tol),
+                 int(maxiter), 1, 0, 0, int(nside), m, int(bs.size), bs, int(nnz), pix, pixweights, signal,
+                 _c(noise, np.float64), int(lam), _c(invtt, np.float64))
+
+
+def read_map(filename):
+    """Column SIGNAL of a HEALPix FITS binary table as written by write_map."""
+    with open(filename, "rb") as f:
+        raw = f.read()
+    pos, hdus = 0, []
+    while pos < len(raw) and len(hdus) < 2:
+        cards = {}
+        while True:
+            block = raw[pos:pos + 2880]
+            pos += 2880
+            done = False
+            for i in range(0, 2880, 80):
+                c = block[i:i + 80].decode("ascii")
+                if c.startswith("END"):
+                    done = True
+                    break
+                if c[8:10] == "= ":
+                    cards[c[:8].strip()] = c[10:].split("/")[0].strip().strip("'").strip()
+            if done:
+                break
+        hdus.append(cards)
+        if len(hdus) == 2:
+            n = int(cards["NAXIS2"])
+            return np.frombuffer(raw[pos:pos + 8 * n], dtype=">f8").astype(np.float64), cards
+    raise GPUError(f"{filename}: no binary table")
 
 
 def tile_variants(precision: int, tile_log2: int) -> list[str]:

@@ -99,6 +99,24 @@ def ref():
     return _REF
 
 
+def ref_iofiles_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libref_iofiles.so"))
+
+
+def ref_write_map(data, nside, filename, nest=True, coordsys="C"):
+    """The reference's write_map (iofiles.c:19-53, cfitsio) compiled from
+    source into oracle/_ref; data float64 -> TDOUBLE, int32 -> TINT."""
+    R = ct.CDLL(os.path.join(_HERE, "_ref", "libref_iofiles.so"))
+    R.write_map.argtypes = [ct.c_void_p, ct.c_int, ct.c_long, ct.c_char_p, ct.c_char, ct.c_char_p]
+    R.write_map.restype = None
+    a = np.ascontiguousarray(data)
+    typ = 82 if a.dtype == np.float64 else 31
+    assert a.dtype in (np.float64, np.int32) and a.size == 12 * nside * nside
+    if os.path.exists(filename):
+        os.remove(filename)
+    R.write_map(a.ctypes.data, typ, int(nside), filename.encode(), bytes([1 if nest else 0]), coordsys.encode())
+
+
 def _c(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
 

@@ -13,6 +13,9 @@ container, where /root/reference and oracle/_ref exist):
                        (-3,-2,-1 rows), which exercises the trash-pixel branches.
   lapack_3x3.npz       dlange('1'), dgetrf, dgecon('1'), dgetri from scipy's
                        LAPACK on random, SPD, map-like and near-singular 3x3 blocks.
+  ref_write_map.npz    HEALPix FITS files written by the reference's write_map
+                       (iofiles.c:19-53 compiled from source with cfitsio,
+                       oracle/_ref) for seeded maps: the inputs and the bytes.
 
     python tests/golden/make_golden.py
 """
@@ -103,11 +106,38 @@ def lapack_vectors():
     np.savez_compressed(os.path.join(HERE, "lapack_3x3.npz"), A=mats, **{k: np.array(v) for k, v in out.items()})
 
 
+FITS_CASES = [  # (nside, dtype, nest, coordsys)
+    (2, "f8", True, "C"),
+    (2, "i4", True, "C"),
+    (4, "f8", False, "G"),
+    (1, "f8", True, "E"),
+]
+
+
+def fits_goldens():
+    import tempfile
+
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(777)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for i, (nside, dt, nest, cs) in enumerate(FITS_CASES):
+            n = 12 * nside * nside
+            data = rng.standard_normal(n) if dt == "f8" else rng.integers(-5, 1000, n).astype(np.int32)
+            fn = os.path.join(d, f"m{i}.fits")
+            O.ref_write_map(data, nside, fn, nest=nest, coordsys=cs)
+            out[f"data{i}"] = data
+            out[f"fits{i}"] = np.frombuffer(open(fn, "rb").read(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, "ref_write_map.npz"), **out)
+
+
 if __name__ == "__main__":
     pix, w = fixture_rank1()
     ref_goldens("ref_mapmat_rank1.npz", pix, w, seed=101)
     fp, fw = flagged_pointing()
     ref_goldens("ref_mapmat_flags.npz", fp, fw, seed=202)
     lapack_vectors()
+    fits_goldens()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -27,6 +27,10 @@ def test_library_exports_every_declared_symbol():
     out = subprocess.run(["nm", "-D", "--defined-only", G.LIB_PATH], capture_output=True, text=True).stdout
     exported = set(re.findall(r" T (mappraiser_gpu_\w+)", out))
     assert set(declared_functions()) <= exported
+    # the reference's own entry points (map.h:10-18), bound unchanged by its ctypes wrapper
+    for name in G.REFERENCE_EXPORTS:
+        assert re.search(rf" T {name}$", out, flags=re.M), name
+        assert re.search(rf"\bvoid {name}\(", open(HEADER).read()), name
 
 
 def test_host_only_entry_points():

@@ -1,0 +1,63 @@
+"""MLmap through the reference's own ctypes signature (map.h:10-13,
+mappraiser_wrapper/__init__.py:58-84) on one rank: solve on the GPU, the
+reference's side effects on the caller's arrays, the residual file and the
+five HEALPix FITS maps, checked against the drop-in solve and the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from midapack_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("precond,zn", [(0, 0), (2, 4)])
+def test_mlmap_writes_the_reference_outputs(gpu, tmp_path, precond, zn):
+    nside = 8  # 12 * 8^2 = 768 = 32 x 24 raster pixels, used as HEALPix indices
+    wl = W.make_workload("mlmap", 2, 16384, 32, 24, 256, 5)
+    print(f"case: MLmap precond {precond} Zn {zn}, 2 x 16384, nside 8", flush=True)
+    pix, w, sig = wl.pix.astype(np.int32).copy(), wl.weights.copy(), wl.signal.copy()
+    gpu.MLmap(str(tmp_path), "t", precond, zn, 1e-8, 300, nside, wl.block_sizes, 3, pix, w, sig, wl.noise, wl.lam,
+              wl.invtt)
+    g = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                    maxiter=300, opts=gpu.options(precond=precond, z_2lvl=zn))
+    # side effects on the caller's arrays, as PCG_GLS_true leaves them
+    assert np.array_equal(pix, g.pix_local) and np.array_equal(w, g.weights) and np.array_equal(sig, g.signal)
+    # residual file: sqrt(res_k)/sqrt(res_0), k = 0..iterations
+    hist = np.fromfile(os.path.join(tmp_path, "pcg_residuals_t.dat"), dtype=np.float64)
+    assert hist.size == g.report["iterations"] + 1 and hist[0] == 1.0
+    # FITS maps = x2map_pol of the drop-in's map (same kernels: bit-identical)
+    tr = g.report["trash_pix"]
+    lstid = g.lindices[3 * tr: 3 * tr + g.x.size]
+    ref_maps = gpu.x2map_pol(g.x, lstid, g.cond, g.hits, nside)
+    for name, want in zip(("mapI", "mapQ", "mapU", "Cond", "Hits"), ref_maps):
+        got, cards = gpu.read_map(os.path.join(tmp_path, f"{name}_t.fits"))
+        assert cards["ORDERING"] == "NESTED" and int(cards["NSIDE"]) == nside and cards["COORDSYS"] == "C"
+        assert np.array_equal(got, want.astype(np.float64)), name
+    # and against the oracle's solve
+    with O.ninv_mode(O.NINV_REAL_SPECTRUM):
+        r = O.gls_solve(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                        maxiter=300, precond=precond, z_2lvl=zn)
+    I, _ = gpu.read_map(os.path.join(tmp_path, "mapI_t.fits"))
+    pixels = r.lindices[3 * r.trash_pix::3] // 3
+    print(f"map I vs oracle {_rel(I[pixels], r.x[0::3]):.2e}", flush=True)
+    assert _rel(I[pixels], r.x[0::3]) <= 1e-7
+
+
+def test_mlmap_reports_unsupported_requests(gpu, tmp_path):
+    wl = W.make_workload("mlmap", 1, 4096, 32, 24, 64, 6)
+    pix, w, sig = wl.pix.astype(np.int32).copy(), wl.weights.copy(), wl.signal.copy()
+    gpu.MLmap(str(tmp_path), "e", 0, 0, 1e-8, 50, 8, wl.block_sizes, 3, pix, w, sig, wl.noise, wl.lam, wl.invtt,
+              solver=1)
+    assert "solver" in gpu.last_error()
+    assert not os.path.exists(os.path.join(tmp_path, "mapI_e.fits"))
+    # a map index beyond 12 nside^2 pixels is refused before any file is written
+    gpu.MLmap(str(tmp_path), "e", 0, 0, 1e-8, 50, 4, wl.block_sizes, 3, pix, w, sig, wl.noise, wl.lam, wl.invtt)
+    assert "HEALPix" in gpu.last_error()
+    assert not os.path.exists(os.path.join(tmp_path, "mapI_e.fits"))
