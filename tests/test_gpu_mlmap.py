@@ -19,9 +19,9 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("precond,zn", [(0, 0), (2, 4)])
 def test_mlmap_writes_the_reference_outputs(gpu, tmp_path, precond, zn):
-    nside = 8  # 12 * 8^2 = 768 = 32 x 24 raster pixels, used as HEALPix indices
-    wl = W.make_workload("mlmap", 2, 16384, 32, 24, 256, 5)
-    print(f"case: MLmap precond {precond} Zn {zn}, 2 x 16384, nside 8", flush=True)
+    nside = 16  # 12 * 16^2 = 3072 = 64 x 48 raster pixels, used as HEALPix indices
+    wl = W.make_workload("mlmap", 2, 32768, 64, 48, 512, 3)  # converges with reference offsets; 20 pixels cut
+    print(f"case: MLmap precond {precond} Zn {zn}, 2 x 32768, nside 16", flush=True)
     pix, w, sig = wl.pix.astype(np.int32).copy(), wl.weights.copy(), wl.signal.copy()
     gpu.MLmap(str(tmp_path), "t", precond, zn, 1e-8, 300, nside, wl.block_sizes, 3, pix, w, sig, wl.noise, wl.lam,
               wl.invtt)
@@ -31,6 +31,7 @@ def test_mlmap_writes_the_reference_outputs(gpu, tmp_path, precond, zn):
     assert np.array_equal(pix, g.pix_local) and np.array_equal(w, g.weights) and np.array_equal(sig, g.signal)
     # residual file: sqrt(res_k)/sqrt(res_0), k = 0..iterations
     hist = np.fromfile(os.path.join(tmp_path, "pcg_residuals_t.dat"), dtype=np.float64)
+    assert g.report["converged"] and g.report["n_cut"] > 0
     assert hist.size == g.report["iterations"] + 1 and hist[0] == 1.0
     # FITS maps = x2map_pol of the drop-in's map (same kernels: bit-identical)
     tr = g.report["trash_pix"]
@@ -51,13 +52,13 @@ def test_mlmap_writes_the_reference_outputs(gpu, tmp_path, precond, zn):
 
 
 def test_mlmap_reports_unsupported_requests(gpu, tmp_path):
-    wl = W.make_workload("mlmap", 1, 4096, 32, 24, 64, 6)
+    wl = W.make_workload("mlmap", 2, 32768, 64, 48, 512, 3)
     pix, w, sig = wl.pix.astype(np.int32).copy(), wl.weights.copy(), wl.signal.copy()
-    gpu.MLmap(str(tmp_path), "e", 0, 0, 1e-8, 50, 8, wl.block_sizes, 3, pix, w, sig, wl.noise, wl.lam, wl.invtt,
+    gpu.MLmap(str(tmp_path), "e", 0, 0, 1e-8, 50, 16, wl.block_sizes, 3, pix, w, sig, wl.noise, wl.lam, wl.invtt,
               solver=1)
     assert "solver" in gpu.last_error()
     assert not os.path.exists(os.path.join(tmp_path, "mapI_e.fits"))
     # a map index beyond 12 nside^2 pixels is refused before any file is written
-    gpu.MLmap(str(tmp_path), "e", 0, 0, 1e-8, 50, 4, wl.block_sizes, 3, pix, w, sig, wl.noise, wl.lam, wl.invtt)
+    gpu.MLmap(str(tmp_path), "e", 0, 0, 1e-8, 50, 8, wl.block_sizes, 3, pix, w, sig, wl.noise, wl.lam, wl.invtt)
     assert "HEALPix" in gpu.last_error()
     assert not os.path.exists(os.path.join(tmp_path, "mapI_e.fits"))
