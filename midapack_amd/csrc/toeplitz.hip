@@ -85,6 +85,12 @@ constexpr int sched_logns(int LOGNC, int LOGV, int LOGR, bool INV, int p) {
 #define TILE14_F32_LOGR 4
 #endif
 
+// coefficient pairs of the spectral step loaded together (FP64 tiles of 16+
+// values per lane, which run with a scheduling fence between groups)
+#ifndef MGPU_SPEC_GROUP
+#define MGPU_SPEC_GROUP 4
+#endif
+
 constexpr int kPadShift = 4;  // one padding element every 16 in LDS (bank spread)
 __device__ __forceinline__ int lpad(int i) { return i + (i >> kPadShift); }
 constexpr int lds_elems(int NC) { return NC + (NC >> kPadShift); }
@@ -233,18 +239,31 @@ struct Cx {
     R re, im;
 };
 
+// first twiddle of every slot of a pass (loaded ahead of the pass, so that its
+// latency overlaps the preceding LDS exchange)
+template <typename R, int LOGNC, int LOGV, int LR, int LOGNS, bool PAIRED>
+__device__ __forceinline__ void pass_twiddles(const Cx<R> *__restrict__ tw, int t, Cx<R> *w) {
+    constexpr int V = 1 << LOGV, RAD = 1 << LR, B = V / RAD, NT = (1 << LOGNC) / V;
+    if constexpr (LOGNS > 0) {
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int j = slot_j<NT, PAIRED>(t, b);
+            const int k = j & ((1 << LOGNS) - 1);
+            w[b] = tw[k << (LOGNC - LOGNS - LR)];
+        }
+    }
+}
+
 // twiddle + butterfly of every slot of a pass (radix 2^LR, sub-FFT size 2^LOGNS)
 template <typename R, int LOGNC, int LOGV, int LR, int LOGNS, bool PAIRED>
-__device__ __forceinline__ void pass_compute(R *re, R *im, const Cx<R> *__restrict__ tw, int t) {
-    constexpr int V = 1 << LOGV, RAD = 1 << LR, B = V / RAD, NT = (1 << LOGNC) / V;
+__device__ __forceinline__ void pass_compute(R *re, R *im, const Cx<R> *w, int t) {
+    constexpr int V = 1 << LOGV, RAD = 1 << LR, B = V / RAD;
 #pragma unroll
     for (int b = 0; b < B; b++) {
         R *vr = re + b * RAD;
         R *vi = im + b * RAD;
         if constexpr (LOGNS > 0) {
-            const int j = slot_j<NT, PAIRED>(t, b);
-            const int k = j & ((1 << LOGNS) - 1);
-            const Cx<R> w1 = tw[k << (LOGNC - LOGNS - LR)];
+            const Cx<R> w1 = w[b];
             R wr = w1.re, wi = w1.im;
 #pragma unroll
             for (int r = 1; r < RAD; r++) {
@@ -305,16 +324,21 @@ __device__ __forceinline__ void pass_load(const R *__restrict__ plane, R *v, int
 
 template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO, bool INV, int P>
 struct Passes {
-    static __device__ __forceinline__ void run(R *re, R *im, R *lds, const Cx<R> *__restrict__ tw, int t) {
+    // w: the first twiddles of pass P's slots (loaded by the caller)
+    static __device__ __forceinline__ void run(R *re, R *im, R *lds, const Cx<R> *__restrict__ tw, int t,
+                                               const Cx<R> *w) {
         constexpr int NP = n_passes(LOGNC, LOGV, LOGR);
         constexpr int LR = sched_bits(LOGNC, LOGV, LOGR, INV, P);
         constexpr int LOGNS = sched_logns(LOGNC, LOGV, LOGR, INV, P);
         constexpr bool PAIRED = INV ? (P == 0) : (P == NP - 1);
-        pass_compute<R, LOGNC, LOGV, LR, LOGNS, PAIRED>(re, im, tw, t);
+        pass_compute<R, LOGNC, LOGV, LR, LOGNS, PAIRED>(re, im, w, t);
         if constexpr (P + 1 < NP) {
             constexpr int LRN = sched_bits(LOGNC, LOGV, LOGR, INV, P + 1);
+            constexpr int LOGNSN = sched_logns(LOGNC, LOGV, LOGR, INV, P + 1);
             constexpr bool PAIREDN = INV ? false : (P + 1 == NP - 1);
             constexpr int NCP = lds_elems(1 << LOGNC);
+            Cx<R> wn[(1 << LOGV) >> LRN];
+            pass_twiddles<R, LOGNC, LOGV, LRN, LOGNSN, PAIREDN>(tw, t, wn);
             if constexpr (TWO) {
                 pass_store<R, LOGNC, LOGV, LR, LOGNS, PAIRED>(lds, re, t);
                 pass_store<R, LOGNC, LOGV, LR, LOGNS, PAIRED>(lds + NCP, im, t);
@@ -332,7 +356,7 @@ struct Passes {
                 pass_load<R, LOGNC, LOGV, LRN, PAIREDN>(lds, im, t);
                 __syncthreads();
             }
-            Passes<R, LOGNC, LOGV, LOGR, TWO, INV, P + 1>::run(re, im, lds, tw, t);
+            Passes<R, LOGNC, LOGV, LOGR, TWO, INV, P + 1>::run(re, im, lds, tw, t, wn);
         }
     }
 };
@@ -439,15 +463,48 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV, MINW) void k_tiles(TileArgs a
             if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
         }
     }
-    Passes<R, LOGNC, LOGV, LOGR, TWO, false, 0>::run(re, im, lds, tw, t);
+    Cx<R> w0[1];  // the first pass of either direction has no twiddles (sub-FFT size 1)
+    static_assert(sched_logns(LOGNC, LOGV, LOGR, false, 0) == 0 && sched_logns(LOGNC, LOGV, LOGR, true, 0) == 0,
+                  "first passes are twiddle-free");
+    Passes<R, LOGNC, LOGV, LOGR, TWO, false, 0>::run(re, im, lds, tw, t, w0);
 
     // spectral step on the paired slots: slot 0 holds k = t + r 2NT, slot 1 holds
     // k = j1 + r 2NT with j1 = 2NT - t (t > 0) or NT (t == 0)
-    if (t != 0) {
+    if (t != 0 && !SCHED_FENCE) {
 #pragma unroll
         for (int r = 0; r < RL; r++) {
             const int k = t + r * 2 * NT;
             spectral_pair<R>(re[r], im[r], re[RL + RL - 1 - r], im[RL + RL - 1 - r], cf[k], cf[NC - k]);
+        }
+    } else if (t != 0) {
+        // the coefficients are loaded SG pairs at a time, the next group's loads
+        // issued before the current group's arithmetic (with a scheduling fence
+        // after every pair their L2 latencies would add up one after the other)
+        constexpr int SG = RL < MGPU_SPEC_GROUP ? RL : MGPU_SPEC_GROUP;
+        static_assert(RL % SG == 0, "coefficient groups must tile the paired slots");
+        Cx<R> cp[2][SG], cq[2][SG];
+#pragma unroll
+        for (int u = 0; u < SG; u++) {
+            const int k = t + u * 2 * NT;
+            cp[0][u] = cf[k];
+            cq[0][u] = cf[NC - k];
+        }
+#pragma unroll
+        for (int g0 = 0; g0 < RL; g0 += SG) {
+            const int cur = (g0 / SG) & 1;
+            if (g0 + SG < RL) {
+#pragma unroll
+                for (int u = 0; u < SG; u++) {
+                    const int k = t + (g0 + SG + u) * 2 * NT;
+                    cp[cur ^ 1][u] = cf[k];
+                    cq[cur ^ 1][u] = cf[NC - k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SG; u++) {
+                const int r = g0 + u;
+                spectral_pair<R>(re[r], im[r], re[RL + RL - 1 - r], im[RL + RL - 1 - r], cp[cur][u], cq[cur][u]);
+            }
             if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
         }
     } else {
@@ -465,7 +522,7 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV, MINW) void k_tiles(TileArgs a
         }
     }
 
-    Passes<R, LOGNC, LOGV, LOGR, TWO, true, 0>::run(re, im, lds, tw, t);
+    Passes<R, LOGNC, LOGV, LOGR, TWO, true, 0>::run(re, im, lds, tw, t, w0);
 
     // last inverse pass (radix RAD0, not paired): position n = j + r NC/RAD0 holds
     // conj(y[2n] + i y[2n+1]); tile sample i -> block offset rel0 + i

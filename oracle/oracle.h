@@ -26,6 +26,14 @@
  *   orc_getlocalW -- closed form sum_t w w^T T0.
  *   orc_gls_solve -- tiny systems against a dense LU solve of the GLS normal
  *       equations (with mean-removal N^-1), and x_k against a dense-operator PCG.
+ *   orc_pinv / orc_symeig -- against numpy's LAPACK pinv, lstsq and eigh
+ *       (full-rank, rank-deficient, badly scaled), Penrose conditions.
+ *   orc_gls_solve2 (two-level, tests/test_twolvl_oracle.py) -- build_Z column 0
+ *       against brute-force hit fractions and the other columns through the
+ *       reference's own index; A Z against the pinned products; the Lanczos Z
+ *       inside the Krylov space of A from P^T N^-1 d (and not inside a smaller
+ *       one); Em1 against numpy's pinv of Z^T-products; the deflated PCG
+ *       reaches the block-Jacobi solution.
  */
 #ifndef MAPPRAISER_ORACLE_H
 #define MAPPRAISER_ORACLE_H
