@@ -217,6 +217,7 @@ struct Pointing {
     float *d_cw0f = nullptr;
     float2 *d_cw12f = nullptr;
     int *d_pixg = nullptr; // [npix0] global pixel number p (valid pixels, increasing)
+    int *d_chunk_perm = nullptr; // [ceil(m/kChunk)] chunk processed by P's workgroup b
 };
 
 // ---------------------------------------------------------------------------

@@ -28,6 +28,7 @@
 //
 // Every kernel is compiled with -ffp-contract=off: products and sums are
 // separately rounded like the reference's C build.
+#include <algorithm>
 #include <hipcub/hipcub.hpp>
 
 #include "mgpu_internal.h"
@@ -259,6 +260,57 @@ static void release_fp32(Problem &pb) {
     P.d_w0f = nullptr;
 }
 
+// ---------------------------------------------------------------------------
+// P's workgroup order.  Every chunk's outputs and block sums are the same in
+// any order; the order only decides where the map gathers hit.  Chunks are
+// sorted by the smallest pixel they touch and dealt to the XCDs in contiguous
+// runs (workgroup b runs on XCD b % 8), so chunks that gather the same map
+// rows (other detectors' passes over them) run on one XCD close together in
+// time and find those rows in that XCD's L2 instead of the MALL.
+// ---------------------------------------------------------------------------
+constexpr int kXcds = 8;
+
+__global__ __launch_bounds__(256) void k_chunk_key(const int *__restrict__ q, int64_t m, int *__restrict__ key) {
+    __shared__ int red[4];
+    const int64_t c0 = (int64_t) blockIdx.x * kChunk;
+    int v = INT32_MAX;
+    for (int i = threadIdx.x; i < kChunk; i += 256) {
+        const int64_t t = c0 + i;
+        if (t < m) {
+            const int qq = q[t];
+            if (qq >= 0) v = min(v, qq);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) key[blockIdx.x] = min(min(red[0], red[1]), min(red[2], red[3]));
+}
+
+static int build_chunk_order(Problem &pb) {
+    Pointing &P = pb.pt;
+    hipStream_t s = pb.stream;
+    const int nch = (int) ((P.m + kChunk - 1) / kChunk);
+    int *d_key;
+    MGPU_RET(pb.arena.alloc(&d_key, (size_t) nch));
+    k_chunk_key<<<nch, 256, 0, s>>>(P.d_q, P.m, d_key);
+    MGPU_LAUNCHED();
+    std::vector<int> key((size_t) nch), order((size_t) nch), perm((size_t) nch);
+    MGPU_HIP(hipMemcpyAsync(key.data(), d_key, sizeof(int) * nch, hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipStreamSynchronize(s));
+    pb.arena.release(d_key);
+    for (int c = 0; c < nch; c++) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    // XCD x runs workgroups b = x, x + 8, ...: give it a contiguous run of the sorted chunks
+    int off[kXcds + 1] = {0};
+    for (int x = 0; x < kXcds; x++) off[x + 1] = off[x] + (x < nch ? (nch - 1 - x) / kXcds + 1 : 0);
+    for (int b = 0; b < nch; b++) perm[b] = order[off[b % kXcds] + b / kXcds];
+    if (!P.d_chunk_perm) MGPU_RET(pb.arena.alloc(&P.d_chunk_perm, (size_t) nch));
+    MGPU_HIP(hipMemcpyAsync(P.d_chunk_perm, perm.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
+    MGPU_HIP(hipStreamSynchronize(s));
+    return MGPU_OK;
+}
+
 static int build_ell(Problem &pb) {
     Pointing &P = pb.pt;
     hipStream_t s = pb.stream;
@@ -304,7 +356,7 @@ static int build_ell(Problem &pb) {
         MGPU_LAUNCHED();
     }
     release_fp32(pb);  // the FP32 copies are stale now
-    return MGPU_OK;
+    return build_chunk_order(pb);
 }
 
 int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m, int nnz) {
@@ -515,10 +567,10 @@ __global__ __launch_bounds__(kPBlock) void k_matvec(const int *__restrict__ q, c
                                                     const TW2 *__restrict__ w12, const TX *__restrict__ x,
                                                     TX *__restrict__ y, int64_t m, const int64_t *__restrict__ start,
                                                     const int *__restrict__ chunk_block, int nb,
-                                                    double *__restrict__ partials) {
+                                                    double *__restrict__ partials, const int *__restrict__ perm) {
     __shared__ double red[kPBlock / 64];
     constexpr int PER = kChunk / kPBlock;
-    const int64_t c = blockIdx.x;
+    const int64_t c = perm[blockIdx.x];
     const int64_t c0 = c * kChunk;
     int qq[PER];
     TW2 wv[PER];
@@ -576,20 +628,22 @@ static int launch_matvec_t(Problem &pb, const TX *d_x, TX *d_y, bool partials, c
     const int *q = P.d_q;
     const int64_t *st = T.d_start;
     const int *cb = T.d_chunk_block;
+    const int *pm = P.d_chunk_perm;
+    MGPU_ARG(pm, "pointing has no chunk order");
     if (P.unit_w0) {
         if (partials)
             k_matvec<TX, TW, TW2, false, true>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials, pm);
         else
             k_matvec<TX, TW, TW2, false, false>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, nullptr);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, nullptr, pm);
     } else {
         if (partials)
             k_matvec<TX, TW, TW2, true, true>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials, pm);
         else
-            k_matvec<TX, TW, TW2, true, false><<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb,
-                                                                             nullptr);
+            k_matvec<TX, TW, TW2, true, false>
+                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb, nullptr, pm);
     }
     MGPU_LAUNCHED();
     return MGPU_OK;
