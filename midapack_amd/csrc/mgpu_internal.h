@@ -320,6 +320,10 @@ struct Segments {
     int *d_wg_seg = nullptr;  // nwg: segment of each workgroup
     int *d_wg_q0 = nullptr;   // nwg: first local pixel of each workgroup
     int *d_state = nullptr;   // nseg: PCG state (0 running, 1 converged, 2 stopped)
+    // segment of every P chunk and Toeplitz block (-1: none, e.g. padding), so
+    // that P, N^-1 and P^T skip the work of stopped segments
+    int *d_chunk_seg = nullptr;
+    int *d_block_seg = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -351,6 +355,9 @@ struct Problem {
     Precond pc;
     Segments sg;
     TwoLevel tl;
+    // PCG state per segment while the iteration loop runs (P, N^-1 and P^T
+    // skip stopped segments); nullptr outside it, where every segment is computed
+    const int *live_state = nullptr;
     // working TOD / map vectors (allocated lazily)
     double *d_tod = nullptr, *d_tod2 = nullptr;   // [m] FP64
     float *d_todf = nullptr, *d_tod2f = nullptr;  // [m] FP32 (mixed)
@@ -386,6 +393,8 @@ int launch_trmatvec(Problem &pb, const double *d_y, double *d_x, const double *d
 int launch_trmatvec_f32(Problem &pb, const float *d_y, double *d_x, const double *d_hvec,
                         double *d_dotpart, int *nparts);
 int trmatvec_nparts(const Problem &pb);
+// smallest valid local pixel of every P chunk (INT32_MAX: none), on the host
+int pointing_chunk_keys(Problem &pb, std::vector<int> &key);
 
 int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const double *d_invtt,
                   const double *h_invtt);

@@ -413,6 +413,31 @@ int segments_ensure(Problem &pb) {
         MGPU_HIP(hipMemcpyAsync(G.d_wg_q0, wq0.data(), sizeof(int) * wq0.size(), hipMemcpyHostToDevice, s));
     }
     MGPU_HIP(hipStreamSynchronize(s));
+    // segment of every P chunk (by its smallest pixel) and Toeplitz block (by its
+    // first chunk): batch items are padded to whole chunks, so neither straddles two
+    std::vector<int> key;
+    MGPU_RET(pointing_chunk_keys(pb, key));
+    const int nch = (int) key.size();
+    std::vector<int> cseg((size_t) nch, -1);
+    for (int c = 0; c < nch; c++)
+        if (key[c] != INT32_MAX)
+            cseg[c] = (int) (std::upper_bound(G.h_q.begin(), G.h_q.end(), key[c]) - G.h_q.begin()) - 1;
+    const ToeplitzPlan &T = pb.tp;
+    std::vector<int> bseg((size_t) std::max(T.nb, 1), -1);
+    for (int b = 0; b < T.nb; b++) {
+        const int64_t c = T.h_start[b] / kChunk;
+        if (c < nch) bseg[b] = cseg[c];
+    }
+    for (int **p : {&G.d_chunk_seg, &G.d_block_seg})
+        if (*p) {
+            pb.arena.release(*p);
+            *p = nullptr;
+        }
+    MGPU_RET(pb.arena.alloc(&G.d_chunk_seg, (size_t) nch));
+    MGPU_RET(pb.arena.alloc(&G.d_block_seg, bseg.size()));
+    MGPU_HIP(hipMemcpyAsync(G.d_chunk_seg, cseg.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
+    MGPU_HIP(hipMemcpyAsync(G.d_block_seg, bseg.data(), sizeof(int) * bseg.size(), hipMemcpyHostToDevice, s));
+    MGPU_HIP(hipStreamSynchronize(s));
     G.npix = npix;
     return MGPU_OK;
 }
@@ -557,6 +582,14 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     k_init_scalars<<<nseg, 256, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag);
     MGPU_LAUNCHED();
 
+    // from here on P, N^-1 and P^T skip the work of stopped segments (every
+    // segment is computed again after the loop, for the true residual)
+    struct LiveState {
+        Problem &pb;
+        ~LiveState() { pb.live_state = nullptr; }
+    } live{pb};
+    pb.live_state = state;
+
     const int poll = pb.opt.poll_every > 0 ? pb.opt.poll_every : 8;
     const int refresh = mixed ? (pb.opt.refresh_every > 0 ? pb.opt.refresh_every : 10) : 0;
     // Iterations are queued in blocks between host polls.  A graph block must
@@ -646,6 +679,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         if (hf[F_STOP] || launched + 1 >= maxiter) done = true;
     }
     pb.prof.tag = 0;
+    pb.live_state = nullptr;
     MGPU_HIP(hipEventRecord(pb.ev1, s));
     // results
     int hf[F_NFLAGS] = {0};

@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void k_chunk_key(const int *__restrict__ q, in
     if (threadIdx.x == 0) key[blockIdx.x] = min(min(red[0], red[1]), min(red[2], red[3]));
 }
 
-static int build_chunk_order(Problem &pb) {
+int pointing_chunk_keys(Problem &pb, std::vector<int> &key) {
     Pointing &P = pb.pt;
     hipStream_t s = pb.stream;
     const int nch = (int) ((P.m + kChunk - 1) / kChunk);
@@ -295,10 +295,19 @@ static int build_chunk_order(Problem &pb) {
     MGPU_RET(pb.arena.alloc(&d_key, (size_t) nch));
     k_chunk_key<<<nch, 256, 0, s>>>(P.d_q, P.m, d_key);
     MGPU_LAUNCHED();
-    std::vector<int> key((size_t) nch), order((size_t) nch), perm((size_t) nch);
+    key.assign((size_t) nch, 0);
     MGPU_HIP(hipMemcpyAsync(key.data(), d_key, sizeof(int) * nch, hipMemcpyDeviceToHost, s));
     MGPU_HIP(hipStreamSynchronize(s));
     pb.arena.release(d_key);
+    return MGPU_OK;
+}
+
+static int build_chunk_order(Problem &pb) {
+    Pointing &P = pb.pt;
+    hipStream_t s = pb.stream;
+    const int nch = (int) ((P.m + kChunk - 1) / kChunk);
+    std::vector<int> key, order((size_t) nch), perm((size_t) nch);
+    MGPU_RET(pointing_chunk_keys(pb, key));
     for (int c = 0; c < nch; c++) order[c] = c;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
     // XCD x runs workgroups b = x, x + 8, ...: give it a contiguous run of the sorted chunks
@@ -567,10 +576,15 @@ __global__ __launch_bounds__(kPBlock) void k_matvec(const int *__restrict__ q, c
                                                     const TW2 *__restrict__ w12, const TX *__restrict__ x,
                                                     TX *__restrict__ y, int64_t m, const int64_t *__restrict__ start,
                                                     const int *__restrict__ chunk_block, int nb,
-                                                    double *__restrict__ partials, const int *__restrict__ perm) {
+                                                    double *__restrict__ partials, const int *__restrict__ perm,
+                                                    const int *__restrict__ chunk_seg, const int *__restrict__ state) {
     __shared__ double red[kPBlock / 64];
     constexpr int PER = kChunk / kPBlock;
     const int64_t c = perm[blockIdx.x];
+    if (state) {  // the chunk's segment has stopped: nothing of it is used any more
+        const int sg = chunk_seg[c];
+        if (sg >= 0 && state[sg]) return;
+    }
     const int64_t c0 = c * kChunk;
     int qq[PER];
     TW2 wv[PER];
@@ -630,20 +644,21 @@ static int launch_matvec_t(Problem &pb, const TX *d_x, TX *d_y, bool partials, c
     const int *cb = T.d_chunk_block;
     const int *pm = P.d_chunk_perm;
     MGPU_ARG(pm, "pointing has no chunk order");
+    const int *cs = pb.sg.d_chunk_seg, *ls = cs ? pb.live_state : nullptr;
     if (P.unit_w0) {
         if (partials)
             k_matvec<TX, TW, TW2, false, true>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials, pm);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials, pm, cs, ls);
         else
             k_matvec<TX, TW, TW2, false, false>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, nullptr, pm);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, nullptr, w12, d_x, d_y, P.m, st, cb, T.nb, nullptr, pm, cs, ls);
     } else {
         if (partials)
             k_matvec<TX, TW, TW2, true, true>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials, pm);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb, T.d_partials, pm, cs, ls);
         else
             k_matvec<TX, TW, TW2, true, false>
-                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb, nullptr, pm);
+                    <<<g, kPBlock, 0, pb.stream>>>(q, w0, w12, d_x, d_y, P.m, st, cb, T.nb, nullptr, pm, cs, ls);
     }
     MGPU_LAUNCHED();
     return MGPU_OK;
@@ -670,7 +685,8 @@ __global__ __launch_bounds__(kPtBlock) void k_trmatvec(const int64_t *__restrict
                                                        const TY *__restrict__ y, double *__restrict__ x,
                                                        const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
                                                        const int *__restrict__ seg_q, const double *__restrict__ hvec,
-                                                       double *__restrict__ dotpart) {
+                                                       double *__restrict__ dotpart, const int *__restrict__ state) {
+    if (state && state[wg_seg[blockIdx.x]]) return;  // stopped segment (uniform over the workgroup)
     __shared__ double red[kPtBlock / 64];
     const int qq = wg_q0[blockIdx.x] + (int) threadIdx.x;
     const int qend = seg_q[wg_seg[blockIdx.x] + 1];
@@ -746,17 +762,17 @@ static int launch_trmatvec_t(Problem &pb, const TY *d_y, double *d_x, const doub
     if (P.unit_w0) {
         if (dot)
             k_trmatvec<TY, TW, TW2, false, true>
-                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, ws, wq, sq, d_h, d_dot);
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, ws, wq, sq, d_h, d_dot, pb.live_state);
         else
             k_trmatvec<TY, TW, TW2, false, false>
-                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, ws, wq, sq, nullptr, nullptr);
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, nullptr, cw12, d_y, d_x, ws, wq, sq, nullptr, nullptr, pb.live_state);
     } else {
         if (dot)
             k_trmatvec<TY, TW, TW2, true, true>
-                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, ws, wq, sq, d_h, d_dot);
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, ws, wq, sq, d_h, d_dot, pb.live_state);
         else
             k_trmatvec<TY, TW, TW2, true, false>
-                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, ws, wq, sq, nullptr, nullptr);
+                    <<<g, kPtBlock, 0, pb.stream>>>(b, P.d_ct, cw0, cw12, d_y, d_x, ws, wq, sq, nullptr, nullptr, pb.live_state);
     }
     MGPU_LAUNCHED();
     return MGPU_OK;

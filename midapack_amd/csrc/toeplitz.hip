@@ -374,6 +374,8 @@ struct TileArgs {
     const double *mean_out;  // [nb] subtracted from the outputs (0 unless MGPU_OFFSET_SYMMETRIC)
     int lambda;
     int L;
+    const int *block_seg;    // [nb] segment of each block (-1: none), with
+    const int *state;        // the PCG state per segment: stopped segments are skipped (nullptr: none)
 };
 
 template <typename R>
@@ -413,6 +415,10 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV, MINW) void k_tiles(TileArgs a
 
     const int t = threadIdx.x;
     const TileDesc d = a.tiles[blockIdx.x];
+    if (a.state) {  // the tile's segment has stopped (uniform over the workgroup)
+        const int sg = a.block_seg[d.block];
+        if (sg >= 0 && a.state[sg]) return;
+    }
     const R *__restrict__ y = reinterpret_cast<const R *>(a.y);
     R *__restrict__ z = reinterpret_cast<R *>(a.z);
     const Cx<R> *__restrict__ tw = reinterpret_cast<const Cx<R> *>(a.tw);
@@ -626,9 +632,11 @@ __global__ __launch_bounds__(256) void k_block_mean(const double *__restrict__ p
                                                     const double *__restrict__ tail,
                                                     const int *__restrict__ block_filter,
                                                     const int *__restrict__ block_kind, int lambda,
-                                                    double *__restrict__ mean_out) {
+                                                    double *__restrict__ mean_out, const int *__restrict__ block_seg,
+                                                    const int *__restrict__ state) {
     __shared__ double red[256];
     const int b = blockIdx.x;
+    if (state && block_seg[b] >= 0 && state[block_seg[b]]) return;  // stopped segment
     const int64_t lo = start[b], hi = start[b + 1];
     const int64_t first = lo / kChunk + b, last = (hi - 1) / kChunk + b;
     double s = 0.0;
@@ -770,6 +778,20 @@ __global__ void k_twiddles(int NC, double *__restrict__ t64, float *__restrict__
 // host side: plan + dispatch
 // ---------------------------------------------------------------------------
 static int max_log2n(bool fp32) { return fp32 ? 16 : 15; }
+
+// Workgroup order of the tiles: workgroup b runs on XCD b % 8, so XCD x is
+// dealt a contiguous run of tiles (in sample order).  Neighbouring tiles,
+// which read half of each other's input, then run on one XCD at about the
+// same time and share those reads (and their filter's spectrum) in its L2.
+static std::vector<TileDesc> xcd_order(const std::vector<TileDesc> &t) {
+    constexpr int kXcds = 8;
+    const int n = (int) t.size();
+    int off[kXcds + 1] = {0};
+    for (int x = 0; x < kXcds; x++) off[x + 1] = off[x] + (x < n ? (n - 1 - x) / kXcds + 1 : 0);
+    std::vector<TileDesc> o(t.size());
+    for (int b = 0; b < n; b++) o[b] = t[off[b % kXcds] + b / kXcds];
+    return o;
+}
 static int min_log2n() { return 10; }
 
 // tile length choice: minimise N (log2 N + c) / L over the sizes that fit LDS
@@ -950,6 +972,7 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
     T.ntiles = (int) tiles.size();
     if (T.ntiles) {
         MGPU_RET(pb.arena.alloc(&T.d_tiles, tiles.size()));
+        tiles = xcd_order(tiles);
         MGPU_HIP(hipMemcpyAsync(T.d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice, s));
         T.nfilters = (int) filter_src.size();
         const int N = 1 << l2, NC = N / 2;
@@ -995,6 +1018,7 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
             }
             T.ntiles32 = (int) tf.size();
             MGPU_RET(pb.arena.alloc(&T.d_tiles32, tf.size()));
+            tf = xcd_order(tf);
             MGPU_HIP(hipMemcpyAsync(T.d_tiles32, tf.data(), sizeof(TileDesc) * tf.size(), hipMemcpyHostToDevice, s));
             double2 *d_bf;
             double *d_c64;
@@ -1125,7 +1149,8 @@ static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *co
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
         attr_devices |= 1u << pb.device;
     }
-    TileArgs a{d_in, d_out, u.tiles, coef, tw, T.d_mean, T.d_mean_out, T.lambda, u.L};
+    TileArgs a{d_in, d_out, u.tiles, coef, tw, T.d_mean, T.d_mean_out, T.lambda, u.L, pb.sg.d_block_seg,
+               pb.sg.d_block_seg ? pb.live_state : nullptr};
     hipLaunchKernelGGL(kern, dim3(u.ntiles), dim3(NT), lds, pb.stream, a);
     MGPU_LAUNCHED();
     return MGPU_OK;
@@ -1281,7 +1306,8 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
         }
         k_block_mean<R><<<T.nb, 256, 0, s>>>(T.d_partials, T.d_start, T.nb, T.d_mean, d_in, T.d_tail,
                                              T.d_block_filter, T.d_block_kind, T.lambda,
-                                             T.offset_mode == MGPU_OFFSET_SYMMETRIC ? T.d_mean_out : nullptr);
+                                             T.offset_mode == MGPU_OFFSET_SYMMETRIC ? T.d_mean_out : nullptr,
+                                             pb.sg.d_block_seg, pb.sg.d_block_seg ? pb.live_state : nullptr);
         MGPU_LAUNCHED();
         pb.prof.end(s);
     }
