@@ -202,10 +202,12 @@ def main():
                                          "final_rel_residual": float(hr[-1])}
         log(f"reference offsets: converged={rr.converged}, min residual {np.nanmin(hr):.2e}")
 
-    # the same solve from host arrays (the drop-in call: host -> HBM transfers included)
+    # the same solve from host arrays (the drop-in call: host -> HBM transfers and the
+    # side effects on the caller's arrays included; the copies it modifies are made first)
+    hp, hw, hs = wl.pix.astype(np.int32, copy=True), wl.weights.copy(), wl.signal.copy()
     t0 = time.perf_counter()
-    gh = G.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=args.tol,
-                   maxiter=args.maxiter, opts=G.options(offset_mode=offset))
+    gh = G.pcg_gls(hp, hw, hs, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=args.tol,
+                   maxiter=args.maxiter, opts=G.options(offset_mode=offset), inplace=True)
     out["time_to_map_from_host_s"] = round(time.perf_counter() - t0, 4)
     log(f"from host arrays: {out['time_to_map_from_host_s']} s ({gh.report['iterations']} iterations)")
 

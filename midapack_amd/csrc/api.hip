@@ -13,6 +13,8 @@
 
 #include "mgpu_internal.h"
 
+#include <thread>
+
 namespace mgpu {
 
 static thread_local char g_err[1024] = "";
@@ -400,24 +402,42 @@ int mappraiser_gpu_pcg_gls(const char *outpath, const char *ref, int m, int nnz,
     MGPU_RET(mappraiser_gpu_problem_create(&h, m, nnz, pix, pixweights, nb, local_blocks_sizes, lambda, invtt, opt));
     std::unique_ptr<Problem> guard(PB(h));
     Problem &pb = *guard;
-    int nmap = 0;
-    MGPU_RET(mappraiser_gpu_precond_build(h, signal, cond, lhits, &nmap));
+    hipStream_t s = pb.stream;
+    // signal and noise go to the device once: the preconditioner build cuts the
+    // device signal, which the solve then uses as it is
+    MGPU_RET(pb.arena.alloc(&pb.d_sig, (size_t) m));
+    MGPU_RET(pb.arena.alloc(&pb.d_noise, (size_t) m));
+    MGPU_HIP(hipMemcpyAsync(pb.d_sig, signal, sizeof(double) * m, hipMemcpyHostToDevice, s));
+    MGPU_HIP(hipMemcpyAsync(pb.d_noise, noise, sizeof(double) * m, hipMemcpyHostToDevice, s));
+    MGPU_RET(precond_build(pb, pb.d_sig));
+    const int np = pb.pt.npix;
+    MGPU_HIP(hipMemcpyAsync(signal, pb.d_sig, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    if (cond) MGPU_HIP(hipMemcpyAsync(cond, pb.pc.d_cond, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+    if (lhits) MGPU_HIP(hipMemcpyAsync(lhits, pb.pc.d_hits, sizeof(int) * np, hipMemcpyDeviceToHost, s));
+    MGPU_HIP(hipStreamSynchronize(s));
     std::vector<double> hist(maxiter > 0 ? maxiter : 1);
     mgpu_report r{};
-    MGPU_RET(mappraiser_gpu_pcg(h, signal, noise, tol, maxiter, 0, x, hist.data(), &r));
+    MGPU_RET(pcg_run(pb, pb.d_sig, pb.d_noise, tol, maxiter, 0, x, hist.data(), &r));
     // side effects of the reference on the caller's pointing: local indices of
-    // the rebuilt matrix; weights of cut samples zeroed
+    // the rebuilt matrix; weights of cut samples zeroed (host loops split over threads)
     const int64_t mn = (int64_t) m * 3;
     std::vector<int> loc((size_t) mn);
     int lcount = 0, trash = 0;
     MGPU_RET(mappraiser_gpu_local_shape(h, &lcount, &trash, lindices, loc.data()));
-    for (int64_t t = 0; t < m; t++) {
-        const bool was_valid = pix[3 * t] >= 0;
-        const bool now_trash = trash && loc[3 * t] == 0;
-        if (was_valid && now_trash)
-            for (int k = 0; k < 3; k++) pixweights[3 * t + k] = 0.0;
-    }
-    std::memcpy(pix, loc.data(), sizeof(int) * mn);
+    const int nth = (int) std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; k++)
+        th.emplace_back([&, k] {
+            const int64_t t0 = (int64_t) m * k / nth, t1 = (int64_t) m * (k + 1) / nth;
+            for (int64_t t = t0; t < t1; t++) {
+                const bool was_valid = pix[3 * t] >= 0;
+                const bool now_trash = trash && loc[3 * t] == 0;
+                if (was_valid && now_trash)
+                    for (int c = 0; c < 3; c++) pixweights[3 * t + c] = 0.0;
+            }
+            std::memcpy(pix + 3 * t0, loc.data() + 3 * t0, sizeof(int) * 3 * (size_t) (t1 - t0));
+        });
+    for (auto &t : th) t.join();
     r.lcount = lcount;
     r.trash_pix = trash;
     if (outpath && ref) {

@@ -391,13 +391,22 @@ class GLSResult:
 
 
 def pcg_gls(pix, weights, signal, noise, block_sizes, lam, invtt, tol=1e-8, maxiter=500, nnz=3,
-            opts: Options | None = None, outpath: str | None = None, ref: str | None = None) -> GLSResult:
-    """PCG_GLS_true drop-in on host arrays (copies; the reference's in-place side
-    effects are applied to the copies returned in the result)."""
+            opts: Options | None = None, outpath: str | None = None, ref: str | None = None,
+            inplace: bool = False) -> GLSResult:
+    """PCG_GLS_true drop-in on host arrays.  By default it works on copies (the
+    reference's in-place side effects are applied to the copies returned in the
+    result); inplace=True modifies pix, weights and signal themselves, like the
+    reference (they must then be C-contiguous int32 / float64 arrays)."""
     L = load()
-    p = _c(pix, np.int32).copy()
-    w = _c(weights, np.float64).copy()
-    s = _c(signal, np.float64).copy()
+    if inplace:
+        for a, dt in ((pix, np.int32), (weights, np.float64), (signal, np.float64)):
+            if not (isinstance(a, np.ndarray) and a.dtype == dt and a.flags.c_contiguous and a.flags.writeable):
+                raise GPUError("inplace=True needs writable C-contiguous int32 pix and float64 weights/signal")
+        p, w, s = pix, weights, signal
+    else:
+        p = _c(pix, np.int32).copy()
+        w = _c(weights, np.float64).copy()
+        s = _c(signal, np.float64).copy()
     nz = _c(noise, np.float64)
     m = p.size // nnz
     bs = _c(block_sizes, np.int32)

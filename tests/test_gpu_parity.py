@@ -345,3 +345,18 @@ def test_invalid_inputs_are_rejected(gpu):
     with gpu.Problem(wl.pix, wl.weights, wl.block_sizes, wl.lam, wl.invtt) as pb:
         with pytest.raises(gpu.GPUError, match="before precond_build"):
             pb.precond_apply(np.zeros(pb.map_size()))
+
+
+def test_dropin_inplace_equals_copies(gpu, fixture_rank1):
+    f = fixture_rank1
+    print("case: reference data, drop-in in place vs on copies", flush=True)
+    g = gpu.pcg_gls(f["pix"], f["weights"], f["signal"], f["noise"], f["block_sizes"], f["lam"], f["invtt"],
+                    tol=1e-8, maxiter=100)
+    pix, w, sig = f["pix"].copy(), f["weights"].copy(), f["signal"].copy()
+    h = gpu.pcg_gls(pix, w, sig, f["noise"], f["block_sizes"], f["lam"], f["invtt"], tol=1e-8, maxiter=100,
+                    inplace=True)
+    assert np.array_equal(h.x, g.x) and h.report["iterations"] == g.report["iterations"]
+    assert np.array_equal(pix, g.pix_local) and np.array_equal(w, g.weights) and np.array_equal(sig, g.signal)
+    with pytest.raises(gpu.GPUError, match="inplace"):
+        gpu.pcg_gls(f["pix"].astype(np.int64), w, sig, f["noise"], f["block_sizes"], f["lam"], f["invtt"],
+                    inplace=True)
