@@ -213,6 +213,9 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args, reps[-1]["iterations"])
+        # all host cores the process may use (the GPU host shares its CPUs: at most 16)
+        ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        out["cpu_baseline_all_cores"] = cpu_baseline(wl, args, reps[-1]["iterations"], threads=min(16, ncpu))
         out["c1_full_solve"] = c1_section(G, W, args)
     if not args.no_noise_model:
         out["noise_model_c2"] = noise_section(G, wl, args)
@@ -411,13 +414,29 @@ def measured_traffic(kernel_class, args):
     return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
 
 
-def cpu_baseline(wl, args, gpu_iters):
-    """The CPU oracle (a C restatement of the reference path, 1 core) on the
-    first cpu_ndet detectors of the same workload: setup + 1 iteration and
-    setup + 3 iterations are timed, and the time-to-map of the full workload is
-    extrapolated linearly in samples at the GPU's iteration count."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, args, gpu_iters, threads=1):
+    """The CPU oracle (a C restatement of the reference path) on the first
+    cpu_ndet detectors of the same workload: setup + 1 iteration and setup + 5
+    iterations are timed, and the time-to-map of the full workload is
+    extrapolated linearly in samples at the GPU's iteration count.  threads = 1
+    is the reference's default (one rank, no OpenMP); threads > 1 times the
+    OpenMP build of the same restatement."""
     from oracle import oracle as O
 
+    omp = threads > 1
+    if omp:
+        threads = O.set_threads(threads)
     nd = min(args.cpu_ndet, args.ndet)
     n = nd * (wl.m // args.ndet)
     nblk = int(np.searchsorted(np.cumsum(wl.block_sizes), n) + 1)
@@ -425,21 +444,23 @@ def cpu_baseline(wl, args, gpu_iters):
               block_sizes=wl.block_sizes[:nblk], invtt=wl.invtt[:nblk * wl.lam])
     times = {}
     mode = {"symmetric": O.OFFSET_SYMMETRIC, "reference": O.OFFSET_REFERENCE, "none": O.OFFSET_NONE}[args.offset]
-    for k in (1, 3):
+    k_lo, k_hi = 1, 5
+    for k in ((0, k_lo, k_hi) if omp else (k_lo, k_hi)):  # the OpenMP build gets an untimed first run
         t0 = time.perf_counter()
         with O.offset_mode(mode):
             O.gls_solve(sl["pix"], sl["weights"], sl["signal"], sl["noise"], sl["block_sizes"], wl.lam,
-                        sl["invtt"], tol=1e-30, maxiter=k + 2, stop_after=k)
+                        sl["invtt"], tol=1e-30, maxiter=max(k, 1) + 2, stop_after=max(k, 1), omp=omp)
         times[k] = time.perf_counter() - t0
-    per_iter = (times[3] - times[1]) / 2
-    setup = times[1] - per_iter
+    per_iter = (times[k_hi] - times[k_lo]) / (k_hi - k_lo)
+    setup = times[k_lo] - k_lo * per_iter
     scale = args.ndet / nd
     est = (setup + gpu_iters * per_iter) * scale
-    log(f"cpu oracle slice {nd}/{args.ndet} det: setup {setup:.2f}s, {per_iter:.2f}s/iteration")
-    return {"value": round(est, 2), "unit": "s", "cores": 1, "kind": "port",
+    log(f"cpu oracle ({threads} thread(s)) slice {nd}/{args.ndet} det: setup {setup:.2f}s, {per_iter:.2f}s/iteration")
+    return {"value": round(est, 2), "unit": "s", "cores": threads, "kind": "port",
             "sample": f"first {nd} of {args.ndet} detectors ({n} samples): setup {setup:.2f} s + "
                       f"{per_iter:.3f} s/iteration measured, scaled x{scale:g} to the full workload at the "
-                      f"GPU's {gpu_iters} iterations"}
+                      f"GPU's {gpu_iters} iterations" + (" (OpenMP build)" if omp else ""),
+            "cpu": cpu_model()}
 
 
 if __name__ == "__main__":

@@ -91,7 +91,10 @@ constexpr int sched_logns(int LOGNC, int LOGV, int LOGR, bool INV, int p) {
 #define MGPU_SPEC_GROUP 4
 #endif
 
-constexpr int kPadShift = 4;  // one padding element every 16 in LDS (bank spread)
+#ifndef MGPU_LDS_PAD_SHIFT
+#define MGPU_LDS_PAD_SHIFT 4
+#endif
+constexpr int kPadShift = MGPU_LDS_PAD_SHIFT;  // one padding element every 2^kPadShift in LDS (bank spread)
 __device__ __forceinline__ int lpad(int i) { return i + (i >> kPadShift); }
 constexpr int lds_elems(int NC) { return NC + (NC >> kPadShift); }
 

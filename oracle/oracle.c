@@ -9,6 +9,24 @@
  */
 #include "oracle.h"
 
+/* loops that the all-cores timing build (-fopenmp) runs in parallel */
+#ifdef _OPENMP
+#include <omp.h>
+#define ORC_PFOR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_PFOR
+#endif
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void) n;
+    return 1;
+#endif
+}
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -85,12 +103,18 @@ void orc_matvec(int m, int nnz, const int *indices, const double *values, int tr
                 const double *x, double *y) {
     for (int i = 0; i < m; i++) y[i] = 0.0;
     if (trash_pix) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
         for (int64_t r = 0; r < m; r++) {
             const int64_t i = r * nnz;
             if (indices[i] != 0)
                 for (int j = 0; j < nnz; j++) y[r] += values[i + j] * x[indices[i + j] - nnz];
         }
     } else {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
         for (int64_t r = 0; r < m; r++) {
             const int64_t i = r * nnz;
             for (int j = 0; j < nnz; j++) y[r] += values[i + j] * x[indices[i + j]];
@@ -103,16 +127,34 @@ void orc_trmatvec(int m, int nnz, const int *indices, const double *values, int 
                   int lcount, const double *y, double *x) {
     if (trash_pix) {
         for (int i = 0; i < lcount - nnz; i++) x[i] = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
         for (int64_t r = 0; r < m; r++) {
             const int64_t i = r * nnz;
             if (indices[i] != 0)
-                for (int j = 0; j < nnz; j++) x[indices[i + j] - nnz] += values[i + j] * y[r];
+                for (int j = 0; j < nnz; j++) {
+                    const double v = values[i + j] * y[r];
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                    x[indices[i + j] - nnz] += v;
+                }
         }
     } else {
         for (int i = 0; i < lcount; i++) x[i] = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
         for (int64_t r = 0; r < m; r++) {
             const int64_t i = r * nnz;
-            for (int j = 0; j < nnz; j++) x[indices[i + j]] += values[i + j] * y[r];
+            for (int j = 0; j < nnz; j++) {
+                const double v = values[i + j] * y[r];
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                x[indices[i + j]] += v;
+            }
         }
     }
 }
@@ -145,24 +187,38 @@ typedef struct {
     cplx *tw; /* exp(-2 pi i k / n), k < n */
 } twiddle_cache;
 
-#define TW_SLOTS 8
+#define TW_SLOTS 256
 static twiddle_cache g_tw[TW_SLOTS];
 static int g_tw_next = 0;
 
+/* Tables stay allocated once made: with OpenMP (the all-cores timing build)
+ * other threads may be reading them, so a full cache computes an uncached
+ * table instead of evicting one (the caller then owns nothing: it leaks one
+ * table per call, which no workload here reaches with its few FFT sizes). */
 static const cplx *get_twiddles(int n) {
-    for (int s = 0; s < TW_SLOTS; s++)
-        if (g_tw[s].n == n && g_tw[s].tw) return g_tw[s].tw;
-    int s = g_tw_next;
-    g_tw_next = (g_tw_next + 1) % TW_SLOTS;
-    free(g_tw[s].tw);
-    g_tw[s].tw = (cplx *) malloc((size_t) n * sizeof(cplx));
-    g_tw[s].n = n;
-    for (int k = 0; k < n; k++) {
-        double a = -2.0 * M_PI * (double) k / (double) n;
-        g_tw[s].tw[k].re = cos(a);
-        g_tw[s].tw[k].im = sin(a);
+    const cplx *found = NULL;
+#ifdef _OPENMP
+#pragma omp critical(orc_twiddles)
+#endif
+    {
+        for (int s = 0; s < TW_SLOTS && !found; s++)
+            if (g_tw[s].n == n && g_tw[s].tw) found = g_tw[s].tw;
+        if (!found) {
+            cplx *t = (cplx *) malloc((size_t) n * sizeof(cplx));
+            for (int k = 0; k < n; k++) {
+                double a = -2.0 * M_PI * (double) k / (double) n;
+                t[k].re = cos(a);
+                t[k].im = sin(a);
+            }
+            if (g_tw_next < TW_SLOTS) {
+                g_tw[g_tw_next].tw = t;
+                g_tw[g_tw_next].n = n;
+                g_tw_next++;
+            }
+            found = t;
+        }
     }
-    return g_tw[s].tw;
+    return found;
 }
 
 static inline cplx cmul(cplx a, cplx b) {
@@ -489,22 +545,29 @@ int orc_stbmm(int nb, const int *block_sizes, int lambda, const double *invtt, i
         tot += block_sizes[b];
     }
     if (tot > m) return ORC_EINVAL;
-    int64_t t_id = 0;
-    if (g_offset_mode != ORC_OFFSET_NONE) {
-        for (int b = 0; b < nb; b++) {
-            const int n = block_sizes[b];
+    /* blocks are independent: in the all-cores timing build they run in parallel */
+    int64_t *b0 = (int64_t *) malloc((size_t) (nb + 1) * sizeof(int64_t));
+    if (!b0) return ORC_ENOMEM;
+    b0[0] = 0;
+    for (int b = 0; b < nb; b++) b0[b + 1] = b0[b] + block_sizes[b];
+    int err = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic) reduction(min : err)
+#endif
+    for (int b = 0; b < nb; b++) {
+        const int64_t t_id = b0[b];
+        const int n = block_sizes[b];
+        if (g_offset_mode != ORC_OFFSET_NONE) {
             double offset = 0;
             for (int j = 0; j < n; j++) offset += V[t_id + j];
             offset /= n;
             for (int j = 0; j < n; j++) V[t_id + j] -= offset;
-            t_id += n;
         }
-    }
-    t_id = 0;
-    for (int b = 0; b < nb; b++) {
-        const int n = block_sizes[b];
         int rc = orc_stmm_block(V + t_id, n, invtt + (int64_t) b * lambda, lambda, use_fft);
-        if (rc < 0) return rc;
+        if (rc < 0) {
+            err = rc < err ? rc : err;
+            continue;
+        }
         if (g_offset_mode == ORC_OFFSET_SYMMETRIC) {
             /* the projection C applied again after the product: C T C */
             double mo = 0;
@@ -512,9 +575,9 @@ int orc_stbmm(int nb, const int *block_sizes, int lambda, const double *invtt, i
             mo /= n;
             for (int j = 0; j < n; j++) V[t_id + j] -= mo;
         }
-        t_id += n;
     }
-    return ORC_OK;
+    free(b0);
+    return err < 0 ? err : ORC_OK;
 }
 
 /* pcg_true.c:578-600 */
@@ -1310,6 +1373,9 @@ static void twolvl_apply(orc_twolvl *t, int nnz, const double *inv, const double
 
 static double dot_seq(int n, const double *a, const double *b) {
     double s = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : s) schedule(static)
+#endif
     for (int i = 0; i < n; i++) s += a[i] * b[i] * 1.0; /* pixpond = 1 on one rank */
     return s;
 }
@@ -1317,6 +1383,7 @@ static double dot_seq(int n, const double *a, const double *b) {
 /* BJ apply (precond.c:1403-1409 -> MatVecProd on BJ_inv with nnz entries per
  * row: row i of pixel q uses g[nnz q + j]). */
 static void bj_apply(int n, int nnz, const double *inv, const double *g, double *Cg) {
+    ORC_PFOR
     for (int i = 0; i < n; i++) {
         const int q = i / nnz;
         double y = 0.0;
@@ -1447,6 +1514,7 @@ int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
     double *Cg = AtNm1Ah; /* same buffer, pcg_true.c:72 */
 
     orc_matvec(m, nnz, pix, pixweights, trash, x, Ah);
+    ORC_PFOR
     for (int i = 0; i < m; i++) Ah[i] = signal[i] + noise[i] - Ah[i];
     rc = orc_apply_weights(nb, block_sizes, lambda, invtt, m, Ah, 1);
     if (rc < 0) return rc;
@@ -1472,12 +1540,17 @@ int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
         orc_trmatvec(m, nnz, pix, pixweights, trash, lcount, Ah, AtNm1Ah);
         double coeff = dot_seq(n, h, AtNm1Ah);
         double ro = g2pix / coeff;
+        ORC_PFOR
         for (int j = 0; j < n; j++) x[j] = x[j] + ro * h[j];
+        ORC_PFOR
         for (int j = 0; j < n; j++) g[j] = gp[j] - ro * AtNm1Ah[j];
         PRECOND_APPLY(g, Cg);
         double g2pixp = g2pix;
         g2pix = dot_seq(n, Cg, g);
         double g2pix_polak = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : g2pix_polak) schedule(static)
+#endif
         for (int i = 0; i < n; i++) g2pix_polak += Cg[i] * (g[i] - gp[i]) * 1.0;
         res = dot_seq(n, g, g);
         if (res_hist && nh < maxiter) res_hist[nh++] = sqrt(res) / sqrt(res0);
@@ -1487,6 +1560,7 @@ int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
         }
         if (stop_after > 0 && k >= stop_after) break;
         double gamma = g2pix_polak / g2pixp;
+        ORC_PFOR
         for (int j = 0; j < n; j++) h[j] = h[j] * gamma + Cg[j];
     }
     rep->iterations = k;

@@ -152,6 +152,10 @@ int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
                    double *x_out, int *lindices_out, double *cond_out, int *lhits_out,
                    double *res_hist, double *z_out, double *az_out, double *em1_out, orc_report *rep);
 
+/* threads of the all-cores timing build (liboracle_omp.so): sets n (> 0) and
+ * returns the count in use; 1 in the sequential build */
+int orc_set_threads(int n);
+
 /* pinv(a) with singular values <= rcond * s_max dropped (inverse_svd_bis,
  * precond.c:777-796), one-sided Jacobi SVD; row-major n x n. */
 int orc_pinv(int n, const double *a, double rcond, double *out);

@@ -15,7 +15,6 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
 _REF = None
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -35,11 +34,16 @@ class OracleReport(ct.Structure):
     ]
 
 
-def lib():
-    """Load liboracle.so (built by ``make -C oracle``)."""
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+_LIBS = {}
+
+
+def lib(omp: bool = False):
+    """Load liboracle.so (built by ``make -C oracle``), or with omp=True its
+    OpenMP build liboracle_omp.so (all-cores CPU timing; same arithmetic except
+    P^T's summation order, which its atomic adds leave to the thread schedule)."""
+    name = "liboracle_omp.so" if omp else "liboracle.so"
+    if name not in _LIBS:
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run __graft_entry__.build())")
         L = ct.CDLL(path)
@@ -75,8 +79,19 @@ def lib():
                                      ct.POINTER(OracleReport)]
         L.orc_pinv.argtypes = [ct.c_int, _f64p, ct.c_double, _f64p]
         L.orc_symeig.argtypes = [ct.c_int, _f64p, _f64p, _f64p]
-        _LIB = L
-    return _LIB
+        L.orc_set_threads.argtypes = [ct.c_int]
+        # a newly loaded build follows the modes set on the others
+        if _LIBS:
+            other = next(iter(_LIBS.values()))
+            L.orc_set_ninv_mode(other.orc_set_ninv_mode(-1))
+            L.orc_set_offset_mode(other.orc_set_offset_mode(-1))
+        _LIBS[name] = L
+    return _LIBS[name]
+
+
+def set_threads(n: int) -> int:
+    """Threads of the OpenMP build; returns the count in use."""
+    return int(lib(omp=True).orc_set_threads(int(n)))
 
 
 def ref_available() -> bool:
@@ -206,7 +221,10 @@ NINV_REFERENCE, NINV_REAL_SPECTRUM, NINV_DIRECT = 0, 1, 2
 
 def set_ninv_mode(mode: int) -> int:
     """Select how the oracle computes the Toeplitz tiles (see oracle.h); returns the old mode."""
-    return int(lib().orc_set_ninv_mode(int(mode)))
+    old = int(lib().orc_set_ninv_mode(int(mode)))
+    for L in _LIBS.values():
+        L.orc_set_ninv_mode(int(mode))
+    return old
 
 
 class ninv_mode:
@@ -233,9 +251,12 @@ class offset_mode:
 
     def __enter__(self):
         self.old = int(lib().orc_set_offset_mode(int(self.mode)))
+        for L in _LIBS.values():
+            L.orc_set_offset_mode(int(self.mode))
 
     def __exit__(self, *a):
-        lib().orc_set_offset_mode(self.old)
+        for L in _LIBS.values():
+            L.orc_set_offset_mode(self.old)
 
 
 def define_blocksize(n, lam):
@@ -338,7 +359,7 @@ def symeig(a):
 
 
 def gls_solve(pix, weights, signal, noise, block_sizes, lam, invtt, nnz=3, tol=1e-8, maxiter=500,
-              stop_after=0, precond=0, z_2lvl=0, deflation=False):
+              stop_after=0, precond=0, z_2lvl=0, deflation=False, omp=False):
     """MLmap setup + PCG_GLS_true on one rank (copies; inputs are not modified).
     precond / z_2lvl as in MLmap (0 block Jacobi, 1 two-level a priori, 2
     two-level a posteriori).  deflation=True also returns Z, AZ (n_map x Zn)
@@ -363,7 +384,7 @@ def gls_solve(pix, weights, signal, noise, block_sizes, lam, invtt, nnz=3, tol=1
         AZb = np.zeros(zn * cap, np.float64)
         Eb = np.zeros(zn * zn, np.float64)
     ptr = lambda a: None if a is None else a.ctypes.data_as(ct.c_void_p)
-    rc = lib().orc_gls_solve2(m, nnz, idx, w, s, nz, bs.size, bs, int(lam), _c(invtt, np.float64), float(tol),
+    rc = lib(omp).orc_gls_solve2(m, nnz, idx, w, s, nz, bs.size, bs, int(lam), _c(invtt, np.float64), float(tol),
                               int(maxiter), int(stop_after), int(precond), int(z_2lvl), x, lind, cond, hits, hist,
                               ptr(Zb), ptr(AZb), ptr(Eb), ct.byref(rep))
     if rc:

@@ -424,3 +424,21 @@ def test_c1_shape_converges():
                     maxiter=300)
     assert r.converged and r.iterations < 150
     assert r.res_hist[-1] <= 1e-8
+
+
+def test_openmp_build_matches_sequential():
+    """The all-cores timing build computes the same solve (P^T's atomic adds
+    only reorder its sums)."""
+    from midapack_amd import workloads as W
+
+    wl = W.config_small(seed=3, ndet=3, nsamp=8192, nx=24, ny=24, lam=128)
+    with O.offset_mode(O.OFFSET_SYMMETRIC):
+        r = O.gls_solve(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-10,
+                        maxiter=300)
+        O.set_threads(4)
+        q = O.gls_solve(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-10,
+                        maxiter=300, omp=True)
+    assert r.converged and q.converged
+    assert np.array_equal(r.lindices, q.lindices) and np.array_equal(r.hits, q.hits)
+    assert abs(r.iterations - q.iterations) <= 1
+    assert np.max(np.abs(r.x - q.x)) <= 1e-8 * np.max(np.abs(r.x))
