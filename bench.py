@@ -45,6 +45,27 @@ def algorithmic_bytes(m, n_map, w_t, w_p, unit_w0=True):
     return {"P": bp, "Pt": bp, "tiles": 2 * m * w_t}
 
 
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+
+
+def algorithmic_flops(block_sizes, lam):
+    """FLOPs of one N^-1 launch as priced for the reference's tiles: per tile
+    5 bs log2(bs) + 6 (bs/2 + 1) with bs = 2^ceil(log2 4 lambda) (4 lambda when
+    that exceeds the block), bs - 2(lambda - 1) outputs per tile (toeplitz.c:200-206)."""
+    import math
+
+    f = 0.0
+    for n in np.asarray(block_sizes, dtype=np.int64):
+        if lam <= 1 or n < lam - 1:
+            continue
+        bs = 1 << math.ceil(math.log2(4 * lam))
+        if bs > n:
+            bs = 4 * lam
+        eff = bs - 2 * (lam - 1)
+        f += math.ceil(n / eff) * (5.0 * bs * math.log2(bs) + 6.0 * (bs // 2 + 1))
+    return f
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +167,14 @@ def main():
     dk = kernels[dominant]
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": dk["GBps"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": dk["frac"], "traffic": measured_traffic(dominant, args)}
+    # N^-1 sits at the FP64 vector ridge: its FLOP rate beside the bytes
+    flops = algorithmic_flops(wl.block_sizes, wl.lam)
+    roofline_flops = None
+    if "tiles" in kernels and flops > 0:
+        tf = flops / (kernels["tiles"]["avg_ms"] * 1e-3) / 1e12
+        roofline_flops = {"kernel": "tiles", "bound": "fp64 vector", "achieved": round(tf, 2),
+                          "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(tf / FP64_VECTOR_PEAK_TFLOPS, 4), "flops_per_launch": flops}
 
     out = {
         "metric": "time-to-map at rel. residual 1e-8 (s)",
@@ -172,6 +201,7 @@ def main():
         "solve_ms": round(float(np.mean([r["t_solve_ms"] for r in reps])), 2),
         "kernels": kernels,
         "roofline": roofline,
+        "roofline_flops": roofline_flops,
     }
 
     if not args.no_mixed:
