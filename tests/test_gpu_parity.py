@@ -179,6 +179,27 @@ def test_precond_bitexact_with_cut(gpu):
     assert np.array_equal(sig, r.signal)
 
 
+def test_precond_bitexact_with_flags_and_cut(gpu):
+    # flagged samples (trash rows before the cut) plus forced single-hit pixels:
+    # the cut follows each cut pixel's own sample chain (the reference offsets
+    # that chain's index by the trash flag, a quirk kept out of both paths)
+    wl = W.make_workload("cut+flags", 2, 65536, 64, 64, 1024, 4, flag_frac=0.02)
+    pix = wl.pix.copy()
+    for t, p in zip((7, 6000, 71000, 99000), (9100, 9101, 9102, 9103)):
+        pix[3 * t:3 * t + 3] = [3 * p, 3 * p + 1, 3 * p + 2]
+    print("case: flagged samples + forced single-hit pixels", flush=True)
+    with O.ninv_mode(O.NINV_REAL_SPECTRUM):
+        r = O.gls_solve(pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                        maxiter=2)
+    with gpu.Problem(pix, wl.weights, wl.block_sizes, wl.lam, wl.invtt) as pb:
+        sig, cond, hits = pb.precond_build(wl.signal)
+        lind, idx, tr = pb.local_shape()
+    assert r.n_cut >= 4 and r.trash_pix == 1
+    assert np.array_equal(lind, r.lindices) and np.array_equal(idx, r.pix_local) and tr == r.trash_pix
+    assert np.array_equal(hits, r.hits) and np.array_equal(cond, r.cond)
+    assert np.array_equal(sig, r.signal)
+
+
 def test_precond_reference_fixture(gpu, fixture_rank1):
     f = fixture_rank1
     r = O.gls_solve(f["pix"], f["weights"], f["signal"], f["noise"], f["block_sizes"], f["lam"], f["invtt"],
