@@ -172,7 +172,7 @@ struct ToeplitzPlan {
     // partial sums for the per-block mean (toeplitz_wizard.c:86-95)
     int64_t nchunks = 0;
     double *d_partials = nullptr;    // nchunks + nb
-    double *d_mean = nullptr;        // nb (per block: [0] input mean, see d_mean_out)
+    double *d_mean = nullptr;        // nb: input mean per block
     // MGPU_OFFSET_SYMMETRIC: mean of T (v - mean) per block, from the block-edge terms
     int offset_mode = MGPU_OFFSET_REFERENCE;
     int *d_chunk_block = nullptr;    // nchunks: first block overlapping chunk c (partial-sum epilogues)
@@ -184,7 +184,7 @@ struct ToeplitzPlan {
     TileDesc *d_tiles32 = nullptr;
     float *d_coef32L = nullptr;      // nfilters * (N32/2) (a_k, b_k)
     float *d_tw32L = nullptr;
-    double *d_mean_out = nullptr;    // nb
+    double *d_edge_part = nullptr;   // nb * 8 slice sums of the symmetric mode's output mean
     double *d_tail = nullptr;        // nfilters * lambda: tail_f[j] = sum_{k>j} T_f[k]
     int *d_block_filter = nullptr;   // nb
     int *d_block_kind = nullptr;     // nb: 0 Toeplitz product, 1 short block (identity)

@@ -89,11 +89,11 @@ __global__ __launch_bounds__(kMB) void k_init_dots(const double *__restrict__ in
 }
 
 // one workgroup per segment
-__global__ __launch_bounds__(256) void k_init_scalars(const double *__restrict__ part, int nwg,
+__global__ __launch_bounds__(kRB) void k_init_scalars(const double *__restrict__ part, int nwg,
                                                       const int *__restrict__ wg0, double *__restrict__ S,
                                                       double *__restrict__ hist, int maxiter,
                                                       int *__restrict__ state, int nseg, int *__restrict__ flag) {
-    __shared__ double red[256];
+    __shared__ double red[kRB / 64];
     const int r = blockIdx.x;
     const int p0 = wg0[r], np = wg0[r + 1] - p0;
     const double g2pix = reduce_parts(part + p0, np, red);
@@ -117,13 +117,13 @@ __global__ __launch_bounds__(256) void k_init_scalars(const double *__restrict__
 }
 
 // ro = g2pix / coeff, one workgroup per segment
-__global__ __launch_bounds__(256) void k_coeff(const double *__restrict__ part, const int *__restrict__ wg0,
+__global__ __launch_bounds__(kRB) void k_coeff(const double *__restrict__ part, const int *__restrict__ wg0,
                                                double *__restrict__ S, const int *__restrict__ state,
                                                const int *__restrict__ flag) {
     if (stopped(flag)) return;
     const int r = blockIdx.x;
     if (state[r]) return;
-    __shared__ double red[256];
+    __shared__ double red[kRB / 64];
     const double coeff = reduce_parts(part + wg0[r], wg0[r + 1] - wg0[r], red);
     if (threadIdx.x == 0) {
         double *Sr = S + (int64_t) r * S_NSCAL;
@@ -197,14 +197,14 @@ __global__ __launch_bounds__(kMB) void k_update_xg(double *__restrict__ x, const
 }
 
 // dots -> g2pix, polak, res; history; stop test; gamma.  One workgroup per segment.
-__global__ __launch_bounds__(256) void k_scalars(const double *__restrict__ part, int nwg,
+__global__ __launch_bounds__(kRB) void k_scalars(const double *__restrict__ part, int nwg,
                                                  const int *__restrict__ wg0, double *__restrict__ S,
                                                  double *__restrict__ hist, int maxiter, int *__restrict__ state,
                                                  int nseg, int *__restrict__ flag) {
     if (stopped(flag)) return;
     const int r = blockIdx.x;
     if (state[r]) return;
-    __shared__ double red[256];
+    __shared__ double red[kRB / 64];
     const int p0 = wg0[r], np = wg0[r + 1] - p0;
     const double g2pix = reduce_parts(part + p0, np, red);
     __syncthreads();
@@ -324,9 +324,9 @@ __global__ __launch_bounds__(kMB) void k_two_norms(const double *__restrict__ rv
     }
 }
 
-__global__ __launch_bounds__(256) void k_seg_norms(const double *__restrict__ part, int nwg,
+__global__ __launch_bounds__(kRB) void k_seg_norms(const double *__restrict__ part, int nwg,
                                                    const int *__restrict__ wg0, double *__restrict__ S) {
-    __shared__ double red[256];
+    __shared__ double red[kRB / 64];
     const int r = blockIdx.x;
     const int p0 = wg0[r], np = wg0[r + 1] - p0;
     const double a = reduce_parts(part + p0, np, red);
@@ -579,7 +579,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         k_init_dots<<<nwg, kMB, 0, s>>>(pb.pc.d_inv, gA, Cg, h, wseg, wq0, sq, part);
         MGPU_LAUNCHED();
     }
-    k_init_scalars<<<nseg, 256, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag);
+    k_init_scalars<<<nseg, kRB, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag);
     MGPU_LAUNCHED();
 
     // from here on P, N^-1 and P^T skip the work of stopped segments (every
@@ -615,7 +615,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         else
             MGPU_RET(apply_A64(pb, h, qv, part));
         pb.prof.begin(MGPU_KT_MAP, s);
-        k_coeff<<<nseg, 256, 0, s>>>(part, wg0, S, state, flag);
+        k_coeff<<<nseg, kRB, 0, s>>>(part, wg0, S, state, flag);
         MGPU_LAUNCHED();
         if (two) {
             k_update_xg<<<nwg, kMB, 0, s>>>(x, h, g, gp, qv, wseg, wq0, sq, S, state, flag);
@@ -644,7 +644,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             pb.prof.end(s);
         }
         pb.prof.begin(MGPU_KT_MAP, s);
-        k_scalars<<<nseg, 256, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag);
+        k_scalars<<<nseg, kRB, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag);
         MGPU_LAUNCHED();
         k_hupdate<<<nwg, kMB, 0, s>>>(h, Cg, wseg, wq0, sq, S, state, flag);
         MGPU_LAUNCHED();
@@ -708,7 +708,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         MGPU_LAUNCHED();
         k_two_norms<<<nwg, kMB, 0, s>>>(qv, bvec, wseg, wq0, sq, part);
         MGPU_LAUNCHED();
-        k_seg_norms<<<nseg, 256, 0, s>>>(part, nwg, wg0, S);
+        k_seg_norms<<<nseg, kRB, 0, s>>>(part, nwg, wg0, S);
         MGPU_LAUNCHED();
         MGPU_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
         MGPU_HIP(hipStreamSynchronize(s));

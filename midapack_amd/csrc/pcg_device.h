@@ -46,17 +46,21 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
     return s;
 }
 
-// fixed-order reduction of nparts partials (single workgroup of 256)
+// threads of the scalar reductions (one workgroup per segment)
+constexpr int kRB = 1024;
+
+// fixed-order reduction of nparts partials by one workgroup of kRB threads
+// (red: kRB / 64 doubles of LDS); every thread returns the sum
 __device__ __forceinline__ double reduce_parts(const double *__restrict__ p, int nparts, double *red) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += 256) s += p[i];
-    red[threadIdx.x] = s;
+    for (int i = threadIdx.x; i < nparts; i += kRB) s += p[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __syncthreads();  // a previous call may still read red
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    return red[0];
+    double t = 0.0;
+    for (int w = 0; w < kRB / 64; w++) t += red[w];
+    return t;
 }
 
 // pixel handled by this thread of a map-domain workgroup, its segment and the segment end

@@ -367,6 +367,16 @@ struct Passes {
 // ---------------------------------------------------------------------------
 // the tile kernel
 // ---------------------------------------------------------------------------
+// slices of the symmetric mode's output-mean sum (see k_block_mean)
+constexpr int kEdgeSlices = 8;
+
+// output mean of block b of length n: the slice sums added in slice order
+__device__ __forceinline__ double mean_out_of(const double *__restrict__ edge_part, int b, int64_t n) {
+    double s = 0.0;
+    for (int k = 0; k < kEdgeSlices; k++) s += edge_part[(int64_t) b * kEdgeSlices + k];
+    return s / (double) n;
+}
+
 struct TileArgs {
     const void *y;           // TOD in (R)
     void *z;                 // TOD out (R)
@@ -374,7 +384,7 @@ struct TileArgs {
     const void *coef;        // Cx<R> [nfilters * NC]
     const void *tw;          // Cx<R> [NC]
     const double *mean;      // [nb]
-    const double *mean_out;  // [nb] subtracted from the outputs (0 unless MGPU_OFFSET_SYMMETRIC)
+    const double *edge_part; // [nb * kEdgeSlices] output-mean slice sums (0 unless MGPU_OFFSET_SYMMETRIC)
     int lambda;
     int L;
     const int *block_seg;    // [nb] segment of each block (-1: none), with
@@ -427,7 +437,7 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV, MINW) void k_tiles(TileArgs a
     const Cx<R> *__restrict__ tw = reinterpret_cast<const Cx<R> *>(a.tw);
     const Cx<R> *__restrict__ cf = reinterpret_cast<const Cx<R> *>(a.coef) + (int64_t) d.filter * NC;
     const R mean = (R) a.mean[d.block];
-    const R mean_out = (R) a.mean_out[d.block];
+    const R mean_out = (R) mean_out_of(a.edge_part, d.block, d.blk_hi - d.blk_lo);
     const int64_t in0 = d.out_lo - (a.lambda - 1);
 
     // block-relative 32-bit offsets from a uniform base: loads are clamped into
@@ -622,12 +632,15 @@ __device__ __forceinline__ double tree256(double v, double *red) {
 }
 
 // mean[b] = (sum of the block's partials, fixed order) / n_b.
-// With mean_out (MGPU_OFFSET_SYMMETRIC) also the mean of T (v - mean) over the
+// With edge_part (MGPU_OFFSET_SYMMETRIC) also the mean of T (v - mean) over the
 // block: T 1 equals its interior value c = T0 + 2 sum_k T_k except within
 // lambda-1 of the block ends, where it is c + e_i with
 // e_i = -tail(i) - tail(n-1-i), tail(j) = sum_{k>j} T_k; since v - mean sums
 // to zero, mean(T (v - mean)) = (1/n) sum_i e_i (v_i - mean), a sum over the
-// 2(lambda-1) edge samples only.
+// 2(lambda-1) edge samples only.  That sum is split over kEdgeSlices
+// workgroups per block (blockIdx.y), each writing its slice's sum to
+// edge_part[b * kEdgeSlices + slice]; the kernels that subtract the output mean
+// add the slices in slice order and divide by the block length (mean_out_of).
 template <typename R>
 __global__ __launch_bounds__(256) void k_block_mean(const double *__restrict__ partials,
                                                     const int64_t *__restrict__ start, int nb,
@@ -635,34 +648,36 @@ __global__ __launch_bounds__(256) void k_block_mean(const double *__restrict__ p
                                                     const double *__restrict__ tail,
                                                     const int *__restrict__ block_filter,
                                                     const int *__restrict__ block_kind, int lambda,
-                                                    double *__restrict__ mean_out, const int *__restrict__ block_seg,
-                                                    const int *__restrict__ state) {
+                                                    double *__restrict__ edge_part,
+                                                    const int *__restrict__ block_seg, const int *__restrict__ state) {
     __shared__ double red[256];
-    const int b = blockIdx.x;
+    const int b = blockIdx.x, sl = blockIdx.y;
     if (state && block_seg[b] >= 0 && state[block_seg[b]]) return;  // stopped segment
     const int64_t lo = start[b], hi = start[b + 1];
     const int64_t first = lo / kChunk + b, last = (hi - 1) / kChunk + b;
     double s = 0.0;
     for (int64_t i = first + threadIdx.x; i <= last; i += 256) s += partials[i];
     const double mu = tree256(s, red) / (double) (hi - lo);
-    if (threadIdx.x == 0) mean[b] = mu;
-    if (!mean_out) return;
-    if (block_kind[b] != 0) {  // short block: identity product
-        if (threadIdx.x == 0) mean_out[b] = 0.0;
-        return;
+    if (sl == 0 && threadIdx.x == 0) mean[b] = mu;
+    if (!edge_part) return;
+    double part = 0.0;
+    if (block_kind[b] == 0) {  // short blocks: identity product, output mean 0
+        const int n = (int) (hi - lo), L1 = lambda - 1;
+        const double *tl = tail + (int64_t) block_filter[b] * lambda;
+        const int a_end = min(L1, n);
+        const int b_beg = max(n - L1, a_end);
+        const int tot_e = a_end + (n - b_beg);
+        const int per = (tot_e + kEdgeSlices - 1) / kEdgeSlices;
+        const int i0 = sl * per, i1 = min(tot_e, i0 + per);
+        double acc = 0.0;
+        for (int i = i0 + (int) threadIdx.x; i < i1; i += 256) {
+            const int k = i < a_end ? i : b_beg + (i - a_end);
+            const double e = -(k < L1 ? tl[k] : 0.0) - (n - 1 - k < L1 ? tl[n - 1 - k] : 0.0);
+            acc += e * ((double) v[lo + k] - mu);
+        }
+        part = tree256(acc, red);
     }
-    const int n = (int) (hi - lo), L1 = lambda - 1;
-    const double *tl = tail + (int64_t) block_filter[b] * lambda;
-    const int a_end = min(L1, n);
-    const int b_beg = max(n - L1, a_end);
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < a_end + (n - b_beg); i += 256) {
-        const int k = i < a_end ? i : b_beg + (i - a_end);
-        const double e = -(k < L1 ? tl[k] : 0.0) - (n - 1 - k < L1 ? tl[n - 1 - k] : 0.0);
-        acc += e * ((double) v[lo + k] - mu);
-    }
-    const double tot = tree256(acc, red);
-    if (threadIdx.x == 0) mean_out[b] = tot / (double) n;
+    if (threadIdx.x == 0) edge_part[(int64_t) b * kEdgeSlices + sl] = part;
 }
 
 // z = y - mean over the listed (short) blocks
@@ -697,10 +712,10 @@ __global__ void k_diag(const R *__restrict__ y, R *__restrict__ z, int64_t m, co
 template <typename R>
 __global__ void k_direct(const R *__restrict__ y, R *__restrict__ z, const int64_t *__restrict__ start,
                          const int *__restrict__ blocks, const double *__restrict__ mean,
-                         const double *__restrict__ mean_out, const double *__restrict__ invtt, int lambda) {
+                         const double *__restrict__ edge_part, const double *__restrict__ invtt, int lambda) {
     const int b = blocks[blockIdx.y];
     const int64_t lo = start[b], hi = start[b + 1];
-    const double mu = mean[b], mo = mean_out[b];
+    const double mu = mean[b], mo = mean_out_of(edge_part, b, hi - lo);
     const double *T = invtt + (int64_t) b * lambda;
     for (int64_t i = lo + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < hi;
          i += (int64_t) gridDim.x * blockDim.x) {
@@ -874,9 +889,9 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
         MGPU_HIP(hipStreamSynchronize(s));
     }
     MGPU_RET(pb.arena.alloc(&T.d_mean, (size_t) nb));
-    MGPU_RET(pb.arena.alloc(&T.d_mean_out, (size_t) nb));
+    MGPU_RET(pb.arena.alloc(&T.d_edge_part, (size_t) nb * kEdgeSlices));
     MGPU_HIP(hipMemsetAsync(T.d_mean, 0, sizeof(double) * nb, s));
-    MGPU_HIP(hipMemsetAsync(T.d_mean_out, 0, sizeof(double) * nb, s));
+    MGPU_HIP(hipMemsetAsync(T.d_edge_part, 0, sizeof(double) * nb * kEdgeSlices, s));
     T.offset_mode = pb.opt.offset_mode;
     MGPU_ARG(T.offset_mode >= MGPU_OFFSET_REFERENCE && T.offset_mode <= MGPU_OFFSET_NONE, "unknown offset_mode %d",
              T.offset_mode);
@@ -1152,7 +1167,7 @@ static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *co
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
         attr_devices |= 1u << pb.device;
     }
-    TileArgs a{d_in, d_out, u.tiles, coef, tw, T.d_mean, T.d_mean_out, T.lambda, u.L, pb.sg.d_block_seg,
+    TileArgs a{d_in, d_out, u.tiles, coef, tw, T.d_mean, T.d_edge_part, T.lambda, u.L, pb.sg.d_block_seg,
                pb.sg.d_block_seg ? pb.live_state : nullptr};
     hipLaunchKernelGGL(kern, dim3(u.ntiles), dim3(NT), lds, pb.stream, a);
     MGPU_LAUNCHED();
@@ -1307,10 +1322,10 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
                                                                T.d_partials);
             MGPU_LAUNCHED();
         }
-        k_block_mean<R><<<T.nb, 256, 0, s>>>(T.d_partials, T.d_start, T.nb, T.d_mean, d_in, T.d_tail,
-                                             T.d_block_filter, T.d_block_kind, T.lambda,
-                                             T.offset_mode == MGPU_OFFSET_SYMMETRIC ? T.d_mean_out : nullptr,
-                                             pb.sg.d_block_seg, pb.sg.d_block_seg ? pb.live_state : nullptr);
+        const bool sym = T.offset_mode == MGPU_OFFSET_SYMMETRIC;
+        k_block_mean<R><<<dim3(T.nb, sym ? kEdgeSlices : 1), 256, 0, s>>>(
+                T.d_partials, T.d_start, T.nb, T.d_mean, d_in, T.d_tail, T.d_block_filter, T.d_block_kind, T.lambda,
+                sym ? T.d_edge_part : nullptr, pb.sg.d_block_seg, pb.sg.d_block_seg ? pb.live_state : nullptr);
         MGPU_LAUNCHED();
         pb.prof.end(s);
     }
@@ -1322,7 +1337,7 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
     }
     if (!T.h_direct_blocks.empty()) {
         k_direct<R><<<dim3(64, (unsigned) T.h_direct_blocks.size()), 256, 0, s>>>(
-                d_in, d_out, T.d_start, T.d_direct_blocks, T.d_mean, T.d_mean_out, T.d_direct_invtt, T.lambda);
+                d_in, d_out, T.d_start, T.d_direct_blocks, T.d_mean, T.d_edge_part, T.d_direct_invtt, T.lambda);
         MGPU_LAUNCHED();
     }
     if (!T.h_short_blocks.empty()) {
