@@ -244,3 +244,102 @@ def c5_pool(seed, n, max_det=128, progress=None):
 
     with ThreadPoolExecutor(max(1, min(8, os.cpu_count() or 1))) as ex:
         return list(ex.map(one, range(n)))
+
+
+# ----------------------------------------------------------------------------
+# Template map-making (MTmap) inputs
+# ----------------------------------------------------------------------------
+def make_templates_workload(name, nces, ndet, ces_len, nx, ny, seed, npoly=3, ground=1, n_sss_bins=16, nhwp=2,
+                            delta_t=20.0, fhwp=1.3, sigma=1.0, flag_frac=0.0, amp=1.0):
+    """One rank's MTmap inputs (mappraiser.c:387-392): nces constant-elevation
+    scans of ndet detectors, blocks CES-major, each as long as its CES.
+
+    The boresight sweeps back and forth along the nx columns (one sweep = nx
+    samples, the row advances after every sweep); az is the boresight column
+    (+0.5, az range [0, nx]); detector j adds a seeded (row, column) offset.
+    The HWP turns at fhwp Hz.  signal = sky + templates with seeded amplitudes
+    (sweep offsets and Legendre drifts, a ground profile in az, HWP harmonics
+    per delta_t bin) scaled by amp; noise is white (sigma per detector), so
+    invtt holds one weight 1/sigma^2 per block (lambda 1)."""
+    fs = FSAMP
+    m = nces * ndet * ces_len
+    npix_grid = nx * ny
+    rs = _rng(seed, 0)
+    sky = np.empty(npix_grid * 3)
+    sky[0::3] = rs.standard_normal(npix_grid)
+    sky[1::3] = 0.1 * rs.standard_normal(npix_grid)
+    sky[2::3] = 0.1 * rs.standard_normal(npix_grid)
+    pix = np.empty(m * 3, np.int32)
+    weights = np.empty(m * 3, np.float64)
+    signal = np.empty(m, np.float64)
+    noise = np.empty(m, np.float64)
+    stamps, az, azmin, azmax, hwp = [], [], [], [], []
+    bsizes, invtt = [], []
+    t = np.arange(ces_len, dtype=np.int64)
+    t0 = 0
+    for c in range(nces):
+        rc = _rng(seed, 10, c)
+        oy = int(rc.integers(0, ny))
+        sweep = t // nx
+        col = np.where(sweep % 2 == 0, t % nx, nx - 1 - t % nx)
+        row = (sweep + oy) % ny
+        st = np.arange(0, ces_len, nx, dtype=np.int64)
+        stamps.append(np.append(st, ces_len).astype(np.int32))
+        a = col.astype(np.float64) + 0.5
+        az.append(a)
+        azmin.append(0.0)
+        azmax.append(float(nx))
+        ang = np.mod(2 * np.pi * fhwp * t / fs + rc.uniform(0, 2 * np.pi), 2 * np.pi)
+        hwp.append(ang)
+        for j in range(ndet):
+            rd = _rng(seed, 11, c, j)
+            dx, dy = int(rd.integers(0, nx)), int(rd.integers(0, ny))
+            p = ((row + dy) % ny) * nx + (col + dx) % nx
+            psi = rd.random(ces_len) * np.pi
+            c2, s2 = np.cos(2 * psi), np.sin(2 * psi)
+            sl = slice(t0, t0 + ces_len)
+            base = 3 * p
+            for k in range(3):
+                pix[3 * t0 + k: 3 * (t0 + ces_len): 3] = base + k
+            weights[3 * t0: 3 * (t0 + ces_len): 3] = 1.0
+            weights[3 * t0 + 1: 3 * (t0 + ces_len): 3] = c2
+            weights[3 * t0 + 2: 3 * (t0 + ces_len): 3] = s2
+            sig = sky[base] + c2 * sky[base + 1] + s2 * sky[base + 2]
+            # template contamination
+            ns = len(st)
+            offs = rd.standard_normal((max(npoly, 1), ns))
+            for o in range(npoly):
+                for s_ in range(ns):
+                    a0, a1 = int(stamps[-1][s_]), int(stamps[-1][s_ + 1])
+                    if a1 - a0 < 2:
+                        continue
+                    u = (2 * np.arange(a0, a1) / fs - a0 / fs - (a1 - 1) / fs) / ((a1 - 1) / fs - a0 / fs)
+                    sig[a0:a1] += amp * offs[o, s_] * np.polynomial.legendre.legval(u, [0] * o + [1])
+            if ground:
+                prof = rd.standard_normal(n_sss_bins)
+                sig += amp * prof[np.minimum(n_sss_bins - 1, np.floor(a / (nx / n_sss_bins)).astype(np.int64))]
+            for h in range(1, nhwp + 1):
+                sig += amp * (rd.standard_normal() * np.cos(h * ang) + rd.standard_normal() * np.sin(h * ang))
+            sd = sigma * (1.0 + 0.2 * rd.random())
+            signal[sl] = sig
+            noise[sl] = sd * _rng(seed, 12, c, j).standard_normal(ces_len)
+            if flag_frac > 0:
+                fl = _rng(seed, 13, c, j).random(ces_len) < flag_frac
+                idx = np.nonzero(fl)[0] + t0
+                for k in range(3):
+                    pix[3 * idx + k] = k - 3
+            bsizes.append(ces_len)
+            invtt.append(1.0 / sd ** 2)
+            t0 += ces_len
+    meta = dict(nces=nces, ndet=ndet, npoly=npoly, ground=ground, n_sss_bins=n_sss_bins, nhwp=nhwp,
+                delta_t=float(delta_t), sampling_freq=fs, sweeptstamps=stamps, az=az, az_min=np.array(azmin),
+                az_max=np.array(azmax), hwp_angle=hwp, m=m, seed=seed)
+    return Workload(name=name, pix=pix, weights=weights, signal=signal, noise=noise,
+                    block_sizes=np.asarray(bsizes, np.int32), lam=1, invtt=np.asarray(invtt, np.float64), nx=nx,
+                    ny=ny, sky=sky, meta=meta)
+
+
+def config_templates_small(seed=21, nces=2, ndet=2, ces_len=2048, nx=32, ny=16, **kw):
+    """Small MTmap case for parity tests (the oracle solves it in about a second)."""
+    return make_templates_workload(f"templates {nces}x{ndet}x{ces_len} {nx}x{ny}", nces, ndet, ces_len, nx, ny,
+                                   seed, **kw)

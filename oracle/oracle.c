@@ -1012,7 +1012,9 @@ void orc_getlocalW(int m, int nnz, const int *indices, const double *values, int
  * Anal. Appl. 13 (1992)): rotate column pairs of W = a until they are
  * orthogonal, so that W = U S and a = U S V^T with V the accumulated rotations;
  * then pinv(a) = sum_j v_j w_j^T / s_j^2 over the kept j. */
-int orc_pinv(int n, const double *a, double rcond, double *out) {
+int orc_pinv(int n, const double *a, double rcond, double *out) { return orc_pinv_rank(n, a, rcond, out, NULL); }
+
+int orc_pinv_rank(int n, const double *a, double rcond, double *out, int *rank) {
     if (n <= 0) return ORC_EINVAL;
     double *W = (double *) malloc((size_t) n * n * sizeof(double));
     double *V = (double *) calloc((size_t) n * n, sizeof(double));
@@ -1058,14 +1060,17 @@ int orc_pinv(int n, const double *a, double rcond, double *out) {
         if (sqrt(t) > smax) smax = sqrt(t);
     }
     for (int i = 0; i < n * n; i++) out[i] = 0.0;
+    int kept = 0;
     for (int j = 0; j < n; j++) {
         if (!(sqrt(s2[j]) > rcond * smax)) continue;
+        kept++;
         for (int i = 0; i < n; i++)
             for (int k = 0; k < n; k++) out[i * n + k] += V[i * n + j] * W[k * n + j] / s2[j];
     }
     free(s2);
     free(W);
     free(V);
+    if (rank) *rank = kept;
     return ORC_OK;
 }
 
@@ -1402,11 +1407,64 @@ int orc_gls_solve(int m, int nnz, int *pix, double *pixweights, double *signal,
                           rep);
 }
 
+static int gls_core(int m, int nnz, int *pix, double *pixweights, double *signal, const double *noise, int nb,
+                    const int *block_sizes, int lambda, const double *invtt, double tol, int maxiter,
+                    int stop_after, int precond, int Z_2lvl, double *x_out, int *lindices_out, double *cond_out,
+                    int *lhits_out, double *res_hist, double *z_out, double *az_out, double *em1_out,
+                    orc_report *rep, const orc_tmpl *tm, const double *tm_binv);
+
 int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
                    const double *noise, int nb, const int *block_sizes, int lambda,
                    const double *invtt, double tol, int maxiter, int stop_after, int precond, int Z_2lvl,
                    double *x_out, int *lindices_out, double *cond_out, int *lhits_out,
                    double *res_hist, double *z_out, double *az_out, double *em1_out, orc_report *rep) {
+    return gls_core(m, nnz, pix, pixweights, signal, noise, nb, block_sizes, lambda, invtt, tol, maxiter, stop_after,
+                    precond, Z_2lvl, x_out, lindices_out, cond_out, lhits_out, res_hist, z_out, az_out, em1_out, rep,
+                    NULL, NULL);
+}
+
+/* MTmap on one rank (mappraiser.c:387-700): the white-noise model (lambda 1,
+ * weight invtt[b] per block, create_toeplitz.c:27-47 with lambda_block_avg 1),
+ * the template kernels B_b = w_b T_b^T T_b and their pseudo-inverses, then
+ * PCG_GLS_templates (pcg_true.c:243-576): PCG_GLS_true with the template part
+ * of the noise weighting subtracted after every apply_weights. */
+int orc_gls_solve_tmpl(int m, int nnz, int *pix, double *pixweights, double *signal, const double *noise,
+                       const int *block_sizes, const double *invtt, double tol, int maxiter, int stop_after,
+                       int precond, int Z_2lvl, const orc_tmpl *tm, double *x_out, int *lindices_out,
+                       double *cond_out, int *lhits_out, double *res_hist, int *ranks_out, orc_report *rep) {
+    const int nb = tm->nces * tm->ndet;
+    int nt_tot = 0;
+    int64_t bsz = 0;
+    int *nt = (int *) malloc((size_t) nb * sizeof(int));
+    if (!nt) return ORC_ENOMEM;
+    const int rs = orc_tmpl_sizes(tm, nt);
+    if (rs < 0) {
+        free(nt);
+        return rs;
+    }
+    nt_tot = rs;
+    for (int b = 0; b < nb; b++) bsz += (int64_t) nt[b] * nt[b];
+    free(nt);
+    (void) nt_tot;
+    double *B = (double *) malloc((size_t) (bsz > 0 ? bsz : 1) * sizeof(double));
+    double *Bi = (double *) malloc((size_t) (bsz > 0 ? bsz : 1) * sizeof(double));
+    int rc = (!B || !Bi) ? ORC_ENOMEM : ORC_OK;
+    if (!rc) rc = orc_tmpl_kernel(tm, B);
+    if (!rc) rc = orc_tmpl_invkernel(tm, B, Bi, ranks_out);
+    if (!rc)
+        rc = gls_core(m, nnz, pix, pixweights, signal, noise, nb, block_sizes, 1, invtt, tol, maxiter, stop_after,
+                      precond, Z_2lvl, x_out, lindices_out, cond_out, lhits_out, res_hist, NULL, NULL, NULL, rep, tm,
+                      Bi);
+    free(B);
+    free(Bi);
+    return rc;
+}
+
+static int gls_core(int m, int nnz, int *pix, double *pixweights, double *signal, const double *noise, int nb,
+                    const int *block_sizes, int lambda, const double *invtt, double tol, int maxiter,
+                    int stop_after, int precond, int Z_2lvl, double *x_out, int *lindices_out, double *cond_out,
+                    int *lhits_out, double *res_hist, double *z_out, double *az_out, double *em1_out,
+                    orc_report *rep, const orc_tmpl *tm, const double *tm_binv) {
     if (precond < 0 || precond > 2 || Z_2lvl < 0) return ORC_EINVAL;
     /* PCG_GLS_true: Z_2lvl == 0 means the communicator size (pcg_true.c:50) */
     const int Zn = Z_2lvl == 0 ? 1 : Z_2lvl;
@@ -1518,6 +1576,7 @@ int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
     for (int i = 0; i < m; i++) Ah[i] = signal[i] + noise[i] - Ah[i];
     rc = orc_apply_weights(nb, block_sizes, lambda, invtt, m, Ah, 1);
     if (rc < 0) return rc;
+    if (tm && (rc = orc_tmpl_filter(tm, tm_binv, Ah)) < 0) return rc;
     orc_trmatvec(m, nnz, pix, pixweights, trash, lcount, Ah, g);
     PRECOND_APPLY(g, Cg);
     for (int j = 0; j < n; j++) h[j] = Cg[j];
@@ -1537,6 +1596,7 @@ int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
         orc_matvec(m, nnz, pix, pixweights, trash, h, Ah);
         rc = orc_apply_weights(nb, block_sizes, lambda, invtt, m, Ah, 1);
         if (rc < 0) return rc;
+        if (tm && (rc = orc_tmpl_filter(tm, tm_binv, Ah)) < 0) return rc;
         orc_trmatvec(m, nnz, pix, pixweights, trash, lcount, Ah, AtNm1Ah);
         double coeff = dot_seq(n, h, AtNm1Ah);
         double ro = g2pix / coeff;

@@ -152,6 +152,42 @@ int orc_gls_solve2(int m, int nnz, int *pix, double *pixweights, double *signal,
                    double *x_out, int *lindices_out, double *cond_out, int *lhits_out,
                    double *res_hist, double *z_out, double *az_out, double *em1_out, orc_report *rep);
 
+/* ---- template marginalisation (MTmap, mappraiser.c:387-700; PCG_GLS_templates,
+ *      pcg_true.c:243-576; algebra/src/templates/templates.c) -- templates.c ---- */
+typedef struct {
+    int npoly;          /* polynomial orders 0..npoly-1 on every sweep */
+    int ground;         /* 1: one scan-synchronous class of n_sss_bins azimuth bins */
+    int n_sss_bins;
+    int nhwp;           /* HWP harmonics 1..nhwp (cos and sin) per time bin of delta_t */
+    double delta_t, sampling_freq;
+    int nces, ndet;     /* blocks c*ndet + j (CES-major), each as long as its CES */
+    const int *block_sizes;     /* [nces*ndet] */
+    const int *nsweeps;         /* [nces] */
+    const int *sweeptstamps;    /* CES c: nsweeps[c]+1 stamps, concatenated */
+    const int *az_binned;       /* per CES sample, concatenated (ground) */
+    const double *hwp_cos;      /* cos / sin of the HWP angle per CES sample, concatenated (nhwp) */
+    const double *hwp_sin;
+    const double *w;            /* [nces*ndet] white-noise weight per block (invtt, lambda 1) */
+} orc_tmpl;
+/* bin_az (templates.c:736-757); ORC_EINVAL for a sample below az_min */
+int orc_bin_az(int n, const double *az, double az_min, double az_max, int nbins, int *out);
+/* cos and sin of the HWP angle (MTmap, mappraiser.c:586-597) */
+void orc_hwp_mod(int n, const double *angle, double *c, double *s);
+/* templates per block (nt_block may be NULL); returns their total or < 0 */
+int orc_tmpl_sizes(const orc_tmpl *t, int *nt_block);
+int orc_tmpl_tvec(const orc_tmpl *t, const double *tau, double *out);    /* TVecProd */
+int orc_tmpl_trtvec(const orc_tmpl *t, const double *d, double *out);    /* TrTVecProd */
+int orc_tmpl_kernel(const orc_tmpl *t, double *B);   /* BuildKernel, blocks nt_b^2 concatenated */
+int orc_tmpl_invkernel(const orc_tmpl *t, const double *B, double *Binv, int *ranks); /* InvKernel */
+/* tod (already weighted by W) -= W T Binv T^T tod */
+int orc_tmpl_filter(const orc_tmpl *t, const double *Binv, double *tod);
+/* MTmap + PCG_GLS_templates on one rank; invtt[nb] (lambda 1); ranks_out[nb]
+ * (may be NULL) receives the kernel ranks InvKernel reports. */
+int orc_gls_solve_tmpl(int m, int nnz, int *pix, double *pixweights, double *signal, const double *noise,
+                       const int *block_sizes, const double *invtt, double tol, int maxiter, int stop_after,
+                       int precond, int Z_2lvl, const orc_tmpl *tm, double *x_out, int *lindices_out,
+                       double *cond_out, int *lhits_out, double *res_hist, int *ranks_out, orc_report *rep);
+
 /* threads of the all-cores timing build (liboracle_omp.so): sets n (> 0) and
  * returns the count in use; 1 in the sequential build */
 int orc_set_threads(int n);
@@ -159,6 +195,8 @@ int orc_set_threads(int n);
 /* pinv(a) with singular values <= rcond * s_max dropped (inverse_svd_bis,
  * precond.c:777-796), one-sided Jacobi SVD; row-major n x n. */
 int orc_pinv(int n, const double *a, double rcond, double *out);
+/* the same, with the number of kept singular values in *rank (may be NULL) */
+int orc_pinv_rank(int n, const double *a, double rcond, double *out, int *rank);
 /* symmetric eigen-decomposition (LAPACKE_dsyev row-major 'V', precond.c:1240):
  * ascending eigenvalues w, eigenvector j in column j of row-major v. */
 int orc_symeig(int n, const double *a, double *w, double *v);

@@ -34,6 +34,26 @@ class OracleReport(ct.Structure):
     ]
 
 
+class _Tmpl(ct.Structure):
+    _fields_ = [
+        ("npoly", ct.c_int),
+        ("ground", ct.c_int),
+        ("n_sss_bins", ct.c_int),
+        ("nhwp", ct.c_int),
+        ("delta_t", ct.c_double),
+        ("sampling_freq", ct.c_double),
+        ("nces", ct.c_int),
+        ("ndet", ct.c_int),
+        ("block_sizes", ct.c_void_p),
+        ("nsweeps", ct.c_void_p),
+        ("sweeptstamps", ct.c_void_p),
+        ("az_binned", ct.c_void_p),
+        ("hwp_cos", ct.c_void_p),
+        ("hwp_sin", ct.c_void_p),
+        ("w", ct.c_void_p),
+    ]
+
+
 _LIBS = {}
 
 
@@ -78,6 +98,19 @@ def lib(omp: bool = False):
                                      _f64p, _i32p, _f64p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                      ct.POINTER(OracleReport)]
         L.orc_pinv.argtypes = [ct.c_int, _f64p, ct.c_double, _f64p]
+        L.orc_bin_az.argtypes = [ct.c_int, _f64p, ct.c_double, ct.c_double, ct.c_int, _i32p]
+        L.orc_hwp_mod.argtypes = [ct.c_int, _f64p, _f64p, _f64p]
+        L.orc_hwp_mod.restype = None
+        _tp = ct.POINTER(_Tmpl)
+        L.orc_tmpl_sizes.argtypes = [_tp, ct.c_void_p]
+        L.orc_tmpl_tvec.argtypes = [_tp, _f64p, _f64p]
+        L.orc_tmpl_trtvec.argtypes = [_tp, _f64p, _f64p]
+        L.orc_tmpl_kernel.argtypes = [_tp, _f64p]
+        L.orc_tmpl_invkernel.argtypes = [_tp, _f64p, _f64p, _i32p]
+        L.orc_tmpl_filter.argtypes = [_tp, _f64p, _f64p]
+        L.orc_gls_solve_tmpl.argtypes = [ct.c_int, ct.c_int, _i32p, _f64p, _f64p, _f64p, _i32p, _f64p, ct.c_double,
+                                         ct.c_int, ct.c_int, ct.c_int, ct.c_int, _tp, _f64p, _i32p, _f64p, _i32p,
+                                         _f64p, _i32p, ct.POINTER(OracleReport)]
         L.orc_symeig.argtypes = [ct.c_int, _f64p, _f64p, _f64p]
         L.orc_set_threads.argtypes = [ct.c_int]
         # a newly loaded build follows the modes set on the others
@@ -400,4 +433,128 @@ def gls_solve(pix, weights, signal, noise, block_sizes, lam, invtt, nnz=3, tol=1
         res.Z = Zb[: zn * n].reshape(zn, n).T.copy()
         res.AZ = AZb[: zn * n].reshape(zn, n).T.copy()
         res.Em1 = Eb.reshape(zn, zn).copy()
+    return res
+
+
+# ----------------------------------------------------------------------------
+# template marginalisation (MTmap / PCG_GLS_templates, templates.c)
+# ----------------------------------------------------------------------------
+class Templates:
+    """Template classes of one rank, as MTmap receives them (mappraiser.c:387-392):
+    nces CES of ndet detectors (blocks CES-major, each as long as its CES),
+    sweeptstamps / az / hwp_angle as one array per CES.  az is binned with
+    bin_az and the HWP angle turned into cos/sin here, as MTmap does.  w is the
+    white-noise weight of each block (invtt with lambda 1)."""
+
+    def __init__(self, block_sizes, nces, ndet, w, npoly=0, ground=0, n_sss_bins=0, nhwp=0, delta_t=0.0,
+                 sampling_freq=1.0, sweeptstamps=None, az=None, az_min=None, az_max=None, hwp_angle=None):
+        L = lib()
+        self.nces, self.ndet = int(nces), int(ndet)
+        self.block_sizes = _c(block_sizes, np.int32).copy()
+        self.w = _c(w, np.float64).copy()
+        lens = [int(self.block_sizes[c * self.ndet]) for c in range(self.nces)]
+        self.lens = lens
+        sw = sweeptstamps if sweeptstamps is not None else [np.array([0, n], np.int32) for n in lens]
+        self.nsweeps = np.array([len(x) - 1 for x in sw], np.int32)
+        self.stamps = np.concatenate([_c(x, np.int32) for x in sw]).astype(np.int32)
+        self.az_binned = None
+        if ground:
+            parts = []
+            for c in range(self.nces):
+                out = np.zeros(lens[c], np.int32)
+                if L.orc_bin_az(lens[c], _c(az[c], np.float64), float(az_min[c]), float(az_max[c]),
+                                int(n_sss_bins), out):
+                    raise ValueError("azimuth below az_min")
+                parts.append(out)
+            self.az_binned = np.concatenate(parts).astype(np.int32)
+        self.hwp_cos = self.hwp_sin = None
+        if nhwp:
+            ang = np.concatenate([_c(a, np.float64)[: lens[c]] for c, a in enumerate(hwp_angle)])
+            self.hwp_cos = np.zeros_like(ang)
+            self.hwp_sin = np.zeros_like(ang)
+            L.orc_hwp_mod(ang.size, ang, self.hwp_cos, self.hwp_sin)
+        p = lambda a: None if a is None else a.ctypes.data_as(ct.c_void_p)
+        self.st = _Tmpl(int(npoly), int(ground), int(n_sss_bins), int(nhwp), float(delta_t), float(sampling_freq),
+                        self.nces, self.ndet, p(self.block_sizes), p(self.nsweeps), p(self.stamps),
+                        p(self.az_binned), p(self.hwp_cos), p(self.hwp_sin), p(self.w))
+        nb = self.nces * self.ndet
+        self.nt = np.zeros(nb, np.int32)
+        tot = L.orc_tmpl_sizes(ct.byref(self.st), self.nt.ctypes.data_as(ct.c_void_p))
+        if tot < 0:
+            raise ValueError(f"orc_tmpl_sizes rc={tot}")
+        self.ntot = int(tot)
+        self.m = int(self.block_sizes.sum())
+
+    def tvec(self, tau):
+        out = np.zeros(self.m, np.float64)
+        if lib().orc_tmpl_tvec(ct.byref(self.st), _c(tau, np.float64), out):
+            raise RuntimeError("orc_tmpl_tvec failed")
+        return out
+
+    def trtvec(self, d):
+        out = np.zeros(self.ntot, np.float64)
+        if lib().orc_tmpl_trtvec(ct.byref(self.st), _c(d, np.float64), out):
+            raise RuntimeError("orc_tmpl_trtvec failed")
+        return out
+
+    def kernel(self):
+        """BuildKernel of every block: concatenated row-major nt_b x nt_b blocks."""
+        out = np.zeros(int((self.nt.astype(np.int64) ** 2).sum()), np.float64)
+        if lib().orc_tmpl_kernel(ct.byref(self.st), out):
+            raise RuntimeError("orc_tmpl_kernel failed")
+        return out
+
+    def invkernel(self, B):
+        out = np.zeros_like(B)
+        ranks = np.zeros(self.nt.size, np.int32)
+        if lib().orc_tmpl_invkernel(ct.byref(self.st), _c(B, np.float64), out, ranks):
+            raise RuntimeError("orc_tmpl_invkernel failed")
+        return out, ranks
+
+    def blocks(self, flat):
+        """split a concatenated kernel array into its nt_b x nt_b blocks"""
+        out, o = [], 0
+        for n in self.nt:
+            n = int(n)
+            out.append(flat[o: o + n * n].reshape(n, n))
+            o += n * n
+        return out
+
+    def filter(self, Binv, tod):
+        t = _c(tod, np.float64).copy()
+        if lib().orc_tmpl_filter(ct.byref(self.st), _c(Binv, np.float64), t):
+            raise RuntimeError("orc_tmpl_filter failed")
+        return t
+
+
+def gls_solve_tmpl(pix, weights, signal, noise, tm: Templates, invtt, nnz=3, tol=1e-8, maxiter=500, stop_after=0,
+                   precond=0, z_2lvl=0):
+    """MTmap setup + PCG_GLS_templates on one rank (copies; inputs are not
+    modified).  invtt holds one white-noise weight per block (lambda 1).
+    Returns a SolveResult with .ranks (the kernel ranks InvKernel reports)."""
+    idx = _c(pix, np.int32).copy()
+    w = _c(weights, np.float64).copy()
+    s = _c(signal, np.float64).copy()
+    nz = _c(noise, np.float64)
+    m = idx.size // nnz
+    cap = max(idx.size, 1)
+    x = np.zeros(cap, np.float64)
+    lind = np.zeros(cap, np.int32)
+    cond = np.zeros(max(m, 1) + cap // nnz, np.float64)
+    hits = np.zeros(max(m, 1) + cap // nnz, np.int32)
+    hist = np.zeros(max(maxiter, 1), np.float64)
+    ranks = np.zeros(tm.nt.size, np.int32)
+    rep = OracleReport()
+    rc = lib().orc_gls_solve_tmpl(m, nnz, idx, w, s, nz, tm.block_sizes, _c(invtt, np.float64), float(tol),
+                                  int(maxiter), int(stop_after), int(precond), int(z_2lvl), ct.byref(tm.st), x, lind,
+                                  cond, hits, hist, ranks, ct.byref(rep))
+    if rc:
+        raise RuntimeError(f"orc_gls_solve_tmpl rc={rc}")
+    n = rep.n_map
+    nh = min(rep.iterations + 1, maxiter) if maxiter > 0 else 0
+    res = SolveResult(x=x[:n].copy(), lindices=lind[: rep.lcount].copy(), cond=cond[: n // nnz].copy(),
+                      hits=hits[: n // nnz].copy(), res_hist=hist[:nh].copy(), iterations=rep.iterations,
+                      converged=bool(rep.converged), res0=rep.res0, res=rep.res, n_cut=rep.n_cut,
+                      trash_pix=rep.trash_pix, pix_local=idx, weights=w, signal=s)
+    res.ranks = ranks
     return res
