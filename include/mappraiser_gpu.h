@@ -82,7 +82,8 @@ typedef struct mgpu_options {
 #define MGPU_KT_PT 3       /* P^T (TrMatVecProd) with the fused dot */
 #define MGPU_KT_MAP 4      /* PCG map-domain updates, dots and preconditioner */
 #define MGPU_KT_REFRESH 5  /* FP64 residual recomputation (MGPU_MIXED) */
-#define MGPU_NKT 6
+#define MGPU_KT_TMPL 6     /* template marginalisation (T^T, Binv, T) */
+#define MGPU_NKT 7
 
 typedef struct mgpu_report {
     int    iterations;  /* value of the loop counter k when PCG stopped (maxiter if not converged) */
@@ -314,10 +315,53 @@ void MLmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int
            int nb_blocks_loc, void *local_blocks_sizes, int Nnz, void *pix, void *pixweights, void *signal,
            double *noise, int lambda, double *invtt);
 
-/* MTmap (mappraiser.c:387 ff., map.h:15-18): exported with the reference's
- * signature so its ctypes wrapper loads, but template map-making
- * (PCG_GLS_templates) is not provided: it prints an error, sets
- * mappraiser_gpu_last_error and returns. */
+/* ---- template marginalisation (MTmap, mappraiser.c:387-700; PCG_GLS_templates,
+ *      pcg_true.c:243-576; algebra/src/templates/templates.c) ---- */
+typedef struct mgpu_templates {
+    int npoly;              /* Legendre orders 0..npoly-1 on every sweep (Polyinit) */
+    int ground;             /* 1: a scan-synchronous class of n_sss_bins azimuth bins (SSSinit) */
+    int n_sss_bins;
+    int nhwp;               /* HWP harmonics 1..nhwp, cos and sin, per time bin of delta_t s (HWPSSinit) */
+    double delta_t;
+    double sampling_freq;
+    int nces;               /* the problem's nb blocks are nces x ndet, CES-major, each as long as its CES */
+    int *const *sweeptstamps;  /* per CES: nsweeps[c]+1 non-decreasing stamps, the last = the CES length */
+    const int *nsweeps;
+    double *const *az;      /* per CES: boresight azimuth per sample (ground) */
+    const double *az_min, *az_max;
+    double *const *hwp_angle;  /* per CES: HWP angle per sample (nhwp) */
+} mgpu_templates;
+
+/* Attach template classes to a problem created with lambda = 1 (MTmap's
+ * white-noise model: invtt[b] is block b's weight, create_toeplitz.c:27-47 with
+ * lambda_block_avg = 1).  Builds the tables of Tlist_init/bin_az
+ * (templates.c:560-757), the kernels B_b = w_b T_b^T T_b (BuildKernel,
+ * templates.c:191-436) and their pseudo-inverses (InvKernel, templates.c:449-465:
+ * singular values <= 1e-12 s_max dropped).  Later pcg / solve_device calls
+ * marginalise over the templates (PCG_GLS_templates) in FP64; MGPU_MIXED is
+ * refused.  A second call replaces the classes. */
+int mappraiser_gpu_templates_set(mgpu_problem *pb, const mgpu_templates *t);
+/* Template count, per-block counts nt_block[nb] and kernel ranks[nb] (the
+ * effective ranks InvKernel prints); any pointer may be NULL. */
+int mappraiser_gpu_templates_info(mgpu_problem *pb, long long *ntot, int *nt_block, int *ranks);
+#define MGPU_TMPL_T 0       /* out[m] = T in[ntot]           (TVecProd, templates.c:27-103) */
+#define MGPU_TMPL_TT 1      /* out[ntot] = T^T in[m]         (TrTVecProd, templates.c:106-188) */
+#define MGPU_TMPL_FILTER 2  /* out[m] = in - W T Binv T^T in (pcg_true.c:426-458; in already weighted by W) */
+int mappraiser_gpu_templates_apply(mgpu_problem *pb, int op, const double *in, double *out);
+/* The kernels and their pseudo-inverses as row-major nt_b x nt_b blocks,
+ * concatenated in block order (sum of nt_b^2 entries each); either may be NULL. */
+int mappraiser_gpu_templates_kernel(mgpu_problem *pb, double *B, double *Binv);
+
+/* MTmap (mappraiser.c:387-700, map.h:15-18) with the reference's signature, so
+ * its ctypes wrapper (mappraiser_wrapper/__init__.py:86-122) binds it
+ * unchanged.  One rank as for MLmap: m = data_size_proc[0], the
+ * nb_blocks_loc blocks are nces x (nb_blocks_loc / nces) CES-major and each
+ * must be as long as its CES (sweeptstamps[c][nsweeps[c]]).  The noise model is
+ * MTmap's: lambda 1 with invtt[b] per block.  The solve is PCG_GLS_templates
+ * (mappraiser_gpu_templates_set on the problem, then the drop-in solve), with
+ * the side effects, residual file and FITS maps of MLmap.  void like the
+ * reference: failures are printed to stderr and left in
+ * mappraiser_gpu_last_error (no exit). */
 void MTmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int Z_2lvl, int pointing_commflag,
            int npoly, int nhwp, double delta_t, int ground, int n_sss_bins, double tol, int maxiter, int enlFac,
            int ortho_alg, int bs_red, int nside, int **sweeptstamps, int *nsweeps, double **az, double *az_min,

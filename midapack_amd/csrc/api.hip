@@ -397,12 +397,25 @@ int mappraiser_gpu_pcg_gls(const char *outpath, const char *ref, int m, int nnz,
                            double *signal, const double *noise, int nb, const int *local_blocks_sizes, int lambda,
                            const double *invtt, double tol, int maxiter, double *x, double *cond, int *lhits,
                            int *lindices, const mgpu_options *opt, mgpu_report *rep) {
+    return mgpu::pcg_gls_impl(outpath, ref, m, nnz, pix, pixweights, signal, noise, nb, local_blocks_sizes, lambda,
+                              invtt, tol, maxiter, x, cond, lhits, lindices, opt, rep, nullptr);
+}
+
+}  // extern "C"
+
+namespace mgpu {
+
+int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix, double *pixweights, double *signal,
+                 const double *noise, int nb, const int *local_blocks_sizes, int lambda, const double *invtt,
+                 double tol, int maxiter, double *x, double *cond, int *lhits, int *lindices,
+                 const mgpu_options *opt, mgpu_report *rep, const mgpu_templates *tm) {
     mgpu_problem *h = nullptr;
     MGPU_ARG(pix && pixweights && signal && noise && x, "NULL argument");
     MGPU_RET(mappraiser_gpu_problem_create(&h, m, nnz, pix, pixweights, nb, local_blocks_sizes, lambda, invtt, opt));
     std::unique_ptr<Problem> guard(PB(h));
     Problem &pb = *guard;
     hipStream_t s = pb.stream;
+    if (tm) MGPU_RET(templates_build(pb, tm));
     // signal and noise go to the device once: the preconditioner build cuts the
     // device signal, which the solve then uses as it is
     MGPU_RET(pb.arena.alloc(&pb.d_sig, (size_t) m));
@@ -455,4 +468,4 @@ int mappraiser_gpu_pcg_gls(const char *outpath, const char *ref, int m, int nnz,
     return MGPU_OK;
 }
 
-}  // extern "C"
+}  // namespace mgpu

@@ -343,6 +343,58 @@ struct TwoLevel {
     double build_ms = 0.0;      // device time of the last build
 };
 
+// ---------------------------------------------------------------------------
+// Template marginalisation (MTmap / PCG_GLS_templates): T, T^T, the per-block
+// kernels B_b = w_b T_b^T T_b and their pseudo-inverses.  Blocks are CES-major
+// (block c*ndet + j) and span their CES, so every table indexed by a CES
+// sample is shared by the CES's detectors.
+// ---------------------------------------------------------------------------
+struct TItem {           // one workgroup of T^T: a sweep, an azimuth bin or a time bin of a block
+    int b, kind, lo, hi; // kind 0 sweep, 1 azimuth bin (range in the azimuth order), 2 time bin
+    int64_t out0;        // first output amplitude
+    int stride, pad;     // distance between the class outputs (sweeps or time bins)
+};
+struct BTask {           // one kernel entry (and its mirror) summed in sample order
+    int kind;            // 0 samples [lo, hi); 1 weighted, 2 unweighted over perm[lo, hi)
+    int row, col;        // entry within the block
+    int lo, hi;
+    int ka, kb;          // weight slots (0..15 Legendre order, 16.. HWP slot)
+    int perm;            // kind 1/2: 0 azimuth order, 1 (sweep, az), 2 (time bin, az)
+    int sym;             // also write (col, row)
+    int pad;
+};
+
+struct Templates {
+    bool on = false;
+    bool live = false;          // applied after N^-1 inside the PCG loop and its right-hand side
+    int npoly = 0, ground = 0, nsss = 0, nhwp = 0;
+    double fsdt = 0.0;          // sampling_freq * delta_t
+    int nces = 0, ndet = 0;
+    int64_t ncs = 0;            // entries of a per-CES table (all CES, workgroup-aligned)
+    int64_t ntot = 0, nB = 0;   // amplitudes; entries of all kernels
+    std::vector<int> h_nt, h_rank;
+    std::vector<int64_t> h_off, h_boff;
+    int *d_ns = nullptr, *d_nh = nullptr;   // [nces] sweeps, time bins
+    int64_t *d_c0 = nullptr;                // [nces+1] first entry of each CES in the per-CES tables
+                                            // (starts rounded up to whole 256-sample workgroups)
+    int *d_len = nullptr;                   // [nces] CES length
+    int *d_ces_of = nullptr;                // [nwg_samples] CES of each sample-kernel workgroup
+    int nwg_samples = 0;
+    int *d_sweep = nullptr;                 // [ncs] sweep of the sample (-1: none)
+    int *d_az = nullptr;                    // [ncs] azimuth bin
+    double *d_leg = nullptr;                // [npoly * ncs] Legendre weights, order-major
+    double *d_hc = nullptr;                 // [2 nhwp * ncs] cos_1, sin_1, cos_2, ...
+    int *d_perm = nullptr;                  // [3 * ncs] sample orders: azimuth, (sweep, az), (time bin, az)
+    int64_t *d_off = nullptr;               // [nb+1] first amplitude of each block
+    int *d_row_block = nullptr;             // [ntot] block of each amplitude
+    int64_t *d_boff = nullptr;              // [nb+1] first kernel entry of each block
+    double *d_B = nullptr, *d_Binv = nullptr; // [nB] row-major nt_b x nt_b blocks
+    double *d_a1 = nullptr, *d_a2 = nullptr;  // [ntot]
+    TItem *d_items = nullptr;
+    int nitems = 0;
+    double build_ms = 0.0;
+};
+
 struct Problem {
     Profiler prof;
     int device = 0;
@@ -355,9 +407,13 @@ struct Problem {
     Precond pc;
     Segments sg;
     TwoLevel tl;
+    Templates tm;
     // PCG state per segment while the iteration loop runs (P, N^-1 and P^T
     // skip stopped segments); nullptr outside it, where every segment is computed
     const int *live_state = nullptr;
+    // N^-1 products follow stbmmProd rather than apply_weights: with lambda 1
+    // they remove the block means too (the two-level products, precond.c:1017-1022)
+    bool stbmm_offsets = false;
     // working TOD / map vectors (allocated lazily)
     double *d_tod = nullptr, *d_tod2 = nullptr;   // [m] FP64
     float *d_todf = nullptr, *d_tod2f = nullptr;  // [m] FP32 (mixed)
@@ -420,6 +476,18 @@ void twolvl_release(Problem &pb);
 int twolvl_apply(Problem &pb, const double *d_g, const double *d_gp, double *d_Cg, double *d_h, double *d_part,
                  bool init, const int *d_state, const int *d_flag);
 int launch_precond_apply(Problem &pb, const double *d_g, double *d_Cg);
+// template marginalisation (MTmap): tables, kernels and pseudo-inverses
+int templates_build(Problem &pb, const mgpu_templates *t);
+// y (a TOD already weighted by N^-1) -= W T Binv T^T y
+int templates_filter(Problem &pb, double *d_y);
+// out = T a (TVecProd) or out = T^T y (TrTVecProd) on device buffers
+int templates_tvec(Problem &pb, const double *d_a, double *d_out);
+int templates_trtvec(Problem &pb, const double *d_y, double *d_out);
+// mappraiser_gpu_pcg_gls, with template classes attached when tm is not NULL (MTmap)
+int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix, double *pixweights, double *signal,
+                 const double *noise, int nb, const int *local_blocks_sizes, int lambda, const double *invtt,
+                 double tol, int maxiter, double *x, double *cond, int *lhits, int *lindices,
+                 const mgpu_options *opt, mgpu_report *rep, const mgpu_templates *tm);
 
 // per-segment PCG outcome (batched problems)
 struct SegReport {

@@ -165,24 +165,22 @@ int mappraiser_gpu_write_map(const void *signal, int type, long nside, const cha
     return MGPU_OK;
 }
 
-void MLmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int Z_2lvl, int pointing_commflag,
-           double tol, int maxiter, int enlFac, int ortho_alg, int bs_red, int nside, void *data_size_proc,
-           int nb_blocks_loc, void *local_blocks_sizes, int Nnz, void *pix, void *pixweights, void *signal,
-           double *noise, int lambda, double *invtt) {
-    (void) comm;
-    (void) pointing_commflag;
-    (void) enlFac;
-    (void) ortho_alg;
-    (void) bs_red;
-    std::printf("\n############# MAPPRAISER (MI355X drop-in, one rank) ################\n");
+// MLmap / MTmap after their own setup (mappraiser.c:34-380, 387-838): the solve
+// of pcg_gls_impl (with template classes when tm is not NULL), the residual
+// file, x2map_pol and the five FITS maps
+static void map_driver(const char *what, char *outpath, char *ref, int solver, int precond, int Z_2lvl, double tol,
+                       int maxiter, int nside, void *data_size_proc, int nb_blocks_loc, void *local_blocks_sizes,
+                       int Nnz, void *pix, void *pixweights, void *signal, double *noise, int lambda, double *invtt,
+                       const mgpu_templates *tm) {
+    std::printf("\n############# MAPPRAISER %s (MI355X drop-in, one rank) ################\n", what);
     if (solver != 0) {
         set_error("solver=%d: only PCG (solver = 0) is available", solver);
         std::fprintf(stderr, "The choice of solver is %d, but only PCG (solver = 0) is available.\n", solver);
         return;
     }
     if (!data_size_proc || !pix || !pixweights || !signal || !noise || !invtt || !local_blocks_sizes) {
-        set_error("MLmap: NULL argument");
-        std::fprintf(stderr, "MLmap: NULL argument\n");
+        set_error("%s: NULL argument", what);
+        std::fprintf(stderr, "%s: NULL argument\n", what);
         return;
     }
     const int m = static_cast<int *>(data_size_proc)[0];
@@ -195,7 +193,7 @@ void MLmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int
     mappraiser_gpu_options_default(&opt);
     opt.precond = precond;
     opt.z_2lvl = Z_2lvl;
-    if (const char *e = std::getenv("MGPU_PRECISION")) opt.precision = std::strcmp(e, "mixed") == 0;
+    if (const char *e = std::getenv("MGPU_PRECISION"); e && !tm) opt.precision = std::strcmp(e, "mixed") == 0;
     if (const char *e = std::getenv("MGPU_OFFSET_MODE"))
         opt.offset_mode = std::strcmp(e, "symmetric") == 0 ? MGPU_OFFSET_SYMMETRIC
                           : std::strcmp(e, "none") == 0    ? MGPU_OFFSET_NONE
@@ -204,10 +202,10 @@ void MLmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int
     std::fflush(stdout);
     const auto t0 = std::chrono::steady_clock::now();
     mgpu_report rep{};
-    const int st = mappraiser_gpu_pcg_gls(outpath, ref, m, Nnz, static_cast<int *>(pix),
-                                          static_cast<double *>(pixweights), static_cast<double *>(signal), noise,
-                                          nb_blocks_loc, static_cast<int *>(local_blocks_sizes), lambda, invtt, tol,
-                                          maxiter, x.data(), cond.data(), lhits.data(), lind.data(), &opt, &rep);
+    const int st = pcg_gls_impl(outpath, ref, m, Nnz, static_cast<int *>(pix), static_cast<double *>(pixweights),
+                                static_cast<double *>(signal), noise, nb_blocks_loc,
+                                static_cast<int *>(local_blocks_sizes), lambda, invtt, tol, maxiter, x.data(),
+                                cond.data(), lhits.data(), lind.data(), &opt, &rep, tm);
     if (st != MGPU_OK) {
         std::fprintf(stderr, "mappraiser_gpu: %s\n", mappraiser_gpu_last_error());
         return;
@@ -262,20 +260,41 @@ void MLmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int
     std::fflush(stdout);
 }
 
+void MLmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int Z_2lvl, int pointing_commflag,
+           double tol, int maxiter, int enlFac, int ortho_alg, int bs_red, int nside, void *data_size_proc,
+           int nb_blocks_loc, void *local_blocks_sizes, int Nnz, void *pix, void *pixweights, void *signal,
+           double *noise, int lambda, double *invtt) {
+    (void) comm, (void) pointing_commflag, (void) enlFac, (void) ortho_alg, (void) bs_red;
+    map_driver("MLmap", outpath, ref, solver, precond, Z_2lvl, tol, maxiter, nside, data_size_proc, nb_blocks_loc,
+               local_blocks_sizes, Nnz, pix, pixweights, signal, noise, lambda, invtt, nullptr);
+}
+
 void MTmap(intptr_t comm, char *outpath, char *ref, int solver, int precond, int Z_2lvl, int pointing_commflag,
            int npoly, int nhwp, double delta_t, int ground, int n_sss_bins, double tol, int maxiter, int enlFac,
            int ortho_alg, int bs_red, int nside, int **sweeptstamps, int *nsweeps, double **az, double *az_min,
            double *az_max, double **hwp_angle, int nces, void *data_size_proc, int nb_blocks_loc,
            void *local_blocks_sizes, int Nnz, void *pix, void *pixweights, void *signal, double *noise,
            double sampling_freq, double *invtt) {
-    (void) comm, (void) outpath, (void) ref, (void) solver, (void) precond, (void) Z_2lvl, (void) pointing_commflag;
-    (void) npoly, (void) nhwp, (void) delta_t, (void) ground, (void) n_sss_bins, (void) tol, (void) maxiter;
-    (void) enlFac, (void) ortho_alg, (void) bs_red, (void) nside, (void) sweeptstamps, (void) nsweeps, (void) az;
-    (void) az_min, (void) az_max, (void) hwp_angle, (void) nces, (void) data_size_proc, (void) nb_blocks_loc;
-    (void) local_blocks_sizes, (void) Nnz, (void) pix, (void) pixweights, (void) signal, (void) noise;
-    (void) sampling_freq, (void) invtt;
-    set_error("MTmap (template map-making, PCG_GLS_templates) is not provided by the GPU library");
-    std::fprintf(stderr, "mappraiser_gpu: %s\n", mappraiser_gpu_last_error());
+    (void) comm, (void) pointing_commflag, (void) enlFac, (void) ortho_alg, (void) bs_red;
+    // the template classes of mappraiser.c:560-600 (bin_az, the HWP cos/sin and
+    // Tlist_init are built inside templates_set) and the white-noise model with
+    // lambda_block_avg = 1 (mappraiser.c:531-554)
+    mgpu_templates t{};
+    t.npoly = npoly;
+    t.ground = ground;
+    t.n_sss_bins = n_sss_bins;
+    t.nhwp = nhwp;
+    t.delta_t = delta_t;
+    t.sampling_freq = sampling_freq;
+    t.nces = nces;
+    t.sweeptstamps = sweeptstamps;
+    t.nsweeps = nsweeps;
+    t.az = az;
+    t.az_min = az_min;
+    t.az_max = az_max;
+    t.hwp_angle = hwp_angle;
+    map_driver("MTmap", outpath, ref, solver, precond, Z_2lvl, tol, maxiter, nside, data_size_proc, nb_blocks_loc,
+               local_blocks_sizes, Nnz, pix, pixweights, signal, noise, 1, invtt, &t);
 }
 
 }  // extern "C"

@@ -489,6 +489,7 @@ int apply_A64(Problem &pb, const double *d_v, double *d_out, double *d_dot) {
     MGPU_RET(launch_matvec(pb, d_v, pb.d_tod, true));
     pb.prof.end(pb.stream);
     MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, false));
+    if (pb.tm.live) MGPU_RET(templates_filter(pb, pb.d_tod2));  // PCG_GLS_templates, pcg_true.c:426-458
     pb.prof.begin(MGPU_KT_PT, pb.stream);
     MGPU_RET(launch_trmatvec(pb, pb.d_tod2, d_out, d_dot ? d_v : nullptr, d_dot, nullptr));
     pb.prof.end(pb.stream);
@@ -526,6 +527,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     MGPU_RET(segments_ensure(pb));
     MGPU_RET(ensure_work(pb));
     const bool mixed = pb.opt.precision == MGPU_MIXED;
+    MGPU_ARG(!(mixed && pb.tm.on), "templates are solved in FP64 only");
     if (mixed) MGPU_RET(pointing_make_fp32(pb));
     // two-level preconditioner: Z, A Z and pinv(E) (build_precond, precond.c:1342-1398), timed apart
     const bool two = pb.opt.precond != 0;
@@ -568,9 +570,17 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     MGPU_HIP(hipEventRecord(pb.ev0, s));
 
     // g0 = P^T N^-1 (s + n) (FP64 in both modes: it is b, kept for refreshes)
+    // the template part of N^-1 (PCG_GLS_templates) from here on; the two-level
+    // space above is built on the plain N^-1, as build_precond does in MTmap
+    struct TemplatesLive {
+        Problem &pb;
+        ~TemplatesLive() { pb.tm.live = false; }
+    } tm_live{pb};
+    pb.tm.live = pb.tm.on;
     k_rhs_tod<<<grid_for(m, 256), 256, 0, s>>>(d_signal, d_noise, pb.d_tod, m);
     MGPU_LAUNCHED();
     MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, true));
+    if (pb.tm.live) MGPU_RET(templates_filter(pb, pb.d_tod2));  // pcg_true.c:318-371
     MGPU_RET(launch_trmatvec(pb, pb.d_tod2, gA, nullptr, nullptr, nullptr));
     MGPU_HIP(hipMemcpyAsync(bvec, gA, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     if (two) {

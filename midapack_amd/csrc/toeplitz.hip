@@ -708,6 +708,18 @@ __global__ void k_diag(const R *__restrict__ y, R *__restrict__ z, int64_t m, co
     z[t] = b < nb ? (R) T0[b] * y[t] : y[t];
 }
 
+// lambda == 1 with stbmmProd's offsets (toeplitz_wizard.c:86-95 ahead of the
+// diagonal product): z = T_b[0] * (y - mean_b) inside blocks, copy after them
+template <typename R>
+__global__ void k_diag_mean(const R *__restrict__ y, R *__restrict__ z, int64_t m,
+                            const int64_t *__restrict__ start, int nb, const double *__restrict__ T0,
+                            const double *__restrict__ mean) {
+    int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int b = find_block_t(start, nb, t);
+    z[t] = b < nb ? (R) (T0[b] * ((double) y[t] - mean[b])) : y[t];
+}
+
 // direct banded product for bands too wide for an LDS tile (slow path)
 template <typename R>
 __global__ void k_direct(const R *__restrict__ y, R *__restrict__ z, const int64_t *__restrict__ start,
@@ -1311,7 +1323,19 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
     ToeplitzPlan &T = pb.tp;
     hipStream_t s = pb.stream;
     if (T.lambda == 1) {
-        k_diag<R><<<grid_for(T.m, 256), 256, 0, s>>>(d_in, d_out, T.m, T.d_start, T.nb, T.d_T0);
+        if (pb.stbmm_offsets && T.offset_mode != MGPU_OFFSET_NONE) {
+            // stbmmProd with lambda 1 (the two-level products): block means removed
+            // (with lambda 1, C T C = T C, so the symmetric mode is the same product)
+            k_partials<R><<<(unsigned) T.nchunks, 256, 0, s>>>(d_in, T.m, T.d_start, T.d_chunk_block, T.nb,
+                                                               T.d_partials);
+            MGPU_LAUNCHED();
+            k_block_mean<R><<<dim3(T.nb, 1), 256, 0, s>>>(T.d_partials, T.d_start, T.nb, T.d_mean, d_in, nullptr,
+                                                          nullptr, nullptr, 1, nullptr, nullptr, nullptr);
+            MGPU_LAUNCHED();
+            k_diag_mean<R><<<grid_for(T.m, 256), 256, 0, s>>>(d_in, d_out, T.m, T.d_start, T.nb, T.d_T0, T.d_mean);
+        } else {
+            k_diag<R><<<grid_for(T.m, 256), 256, 0, s>>>(d_in, d_out, T.m, T.d_start, T.nb, T.d_T0);
+        }
         MGPU_LAUNCHED();
         return MGPU_OK;
     }

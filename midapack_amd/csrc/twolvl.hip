@@ -460,10 +460,15 @@ int twolvl_build(Problem &pb, const double *d_signal, const double *d_noise) {
              precond);
     MGPU_ARG(pb.opt.z_2lvl >= 0 && zn <= kMaxZn && zn <= n, "z_2lvl=%d must be in [0, min(%d, n_map=%d)]",
              pb.opt.z_2lvl, kMaxZn, n);
-    MGPU_ARG(pb.tp.lambda > 1,
-             "two-level preconditioners need lambda > 1 (their products remove offsets even for lambda = 1)");
     MGPU_ARG(pb.sg.nseg == 1, "two-level preconditioners solve one problem at a time");
     twolvl_release(pb);
+    // the products of build_AZ and Lanczos_eig call stbmmProd (precond.c:1017-1022,
+    // 1151-1153, 1205-1207): with lambda 1 they also remove the block means
+    struct StbmmOffsets {
+        Problem &pb;
+        ~StbmmOffsets() { pb.stbmm_offsets = false; }
+    } stbmm{pb};
+    pb.stbmm_offsets = true;
     TwoLevel &T = pb.tl;
     hipStream_t s = pb.stream;
     MGPU_HIP(hipEventRecord(pb.ev0, s));

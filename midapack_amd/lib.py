@@ -40,14 +40,16 @@ EXPORTS = [
     "mappraiser_gpu_tile_variants", "mappraiser_gpu_pcg_batch", "mappraiser_gpu_server_create",
     "mappraiser_gpu_server_submit", "mappraiser_gpu_server_wait", "mappraiser_gpu_server_stats",
     "mappraiser_gpu_server_destroy", "mappraiser_gpu_noise_invtt", "mappraiser_gpu_pcg_gls",
-    "mappraiser_gpu_x2map_pol", "mappraiser_gpu_write_map",
+    "mappraiser_gpu_x2map_pol", "mappraiser_gpu_write_map", "mappraiser_gpu_templates_set",
+    "mappraiser_gpu_templates_info", "mappraiser_gpu_templates_apply", "mappraiser_gpu_templates_kernel",
 ]
 # the reference's own entry points, exported with its signatures (map.h:10-18)
 REFERENCE_EXPORTS = ["MLmap", "MTmap"]
 MGPU_TDOUBLE, MGPU_TINT = 82, 31
 
 # kernel classes of mappraiser_gpu_kernel_times (MGPU_KT_*)
-KERNEL_CLASSES = ["P", "mean", "tiles", "Pt", "map", "refresh"]
+KERNEL_CLASSES = ["P", "mean", "tiles", "Pt", "map", "refresh", "templates"]
+MGPU_TMPL_T, MGPU_TMPL_TT, MGPU_TMPL_FILTER = 0, 1, 2
 
 
 class Options(ct.Structure):
@@ -73,6 +75,52 @@ class BatchItem(ct.Structure):
                 ("nb", ct.c_int), ("local_blocks_sizes", _vp), ("lam", ct.c_int), ("invtt", _vp),
                 ("x", _vp), ("pixels", _vp), ("cond", _vp), ("lhits", _vp), ("x_cap", ct.c_int),
                 ("rep", Report)]
+
+
+class TemplatesSpec(ct.Structure):
+    """mgpu_templates (include/mappraiser_gpu.h)"""
+    _fields_ = [
+        ("npoly", ct.c_int),
+        ("ground", ct.c_int),
+        ("n_sss_bins", ct.c_int),
+        ("nhwp", ct.c_int),
+        ("delta_t", ct.c_double),
+        ("sampling_freq", ct.c_double),
+        ("nces", ct.c_int),
+        ("sweeptstamps", ct.c_void_p),
+        ("nsweeps", ct.c_void_p),
+        ("az", ct.c_void_p),
+        ("az_min", ct.c_void_p),
+        ("az_max", ct.c_void_p),
+        ("hwp_angle", ct.c_void_p),
+    ]
+
+
+def _row_pointers(rows, dt):
+    """C-contiguous copies of the rows and an array of their addresses (the
+    int** / double** arguments of MTmap, as the reference's wrapper builds them)"""
+    keep = [np.ascontiguousarray(r, dtype=dt) for r in rows]
+    ptrs = np.array([k.ctypes.data for k in keep], dtype=np.uintp)
+    return keep, ptrs
+
+
+def templates_spec(meta: dict):
+    """mgpu_templates from a workload's template metadata; returns (spec, keep),
+    keep holding (by name) the arrays the spec points to."""
+    keep = {}
+    keep["sw_rows"], keep["sweeptstamps"] = _row_pointers(meta["sweeptstamps"], np.int32)
+    keep["nsweeps"] = np.array([len(x) - 1 for x in keep["sw_rows"]], np.int32)
+    if meta.get("ground"):
+        keep["az_rows"], keep["az"] = _row_pointers(meta["az"], np.float64)
+        keep["az_min"] = np.ascontiguousarray(meta["az_min"], np.float64)
+        keep["az_max"] = np.ascontiguousarray(meta["az_max"], np.float64)
+    if meta.get("nhwp"):
+        keep["hwp_rows"], keep["hwp_angle"] = _row_pointers(meta["hwp_angle"], np.float64)
+    p = lambda k: keep[k].ctypes.data if k in keep else None
+    spec = TemplatesSpec(int(meta["npoly"]), int(meta["ground"]), int(meta["n_sss_bins"]), int(meta["nhwp"]),
+                         float(meta["delta_t"]), float(meta["sampling_freq"]), int(meta["nces"]), p("sweeptstamps"),
+                         p("nsweeps"), p("az"), p("az_min"), p("az_max"), p("hwp_angle"))
+    return spec, keep
 
 
 class GPUError(RuntimeError):
@@ -131,6 +179,17 @@ def load():
     L.mappraiser_gpu_write_map.argtypes = [_vp, ct.c_int, ct.c_long, ct.c_char_p, ct.c_char, ct.c_char_p]
     # MLmap with the argtypes of the reference's wrapper (mappraiser_wrapper/__init__.py:58-84),
     # the communicator passed as an int handle
+    L.mappraiser_gpu_templates_set.argtypes = [_vp, ct.POINTER(TemplatesSpec)]
+    L.mappraiser_gpu_templates_info.argtypes = [_vp, ct.POINTER(ct.c_longlong), _vp, _vp]
+    L.mappraiser_gpu_templates_apply.argtypes = [_vp, ct.c_int, _f64p, _f64p]
+    L.mappraiser_gpu_templates_kernel.argtypes = [_vp, _vp, _vp]
+    # MTmap with the argtypes of the reference's wrapper (mappraiser_wrapper/__init__.py:86-122)
+    _up = np.ctypeslib.ndpointer(dtype=np.uintp, ndim=1, flags="C_CONTIGUOUS")
+    L.MTmap.restype = None
+    L.MTmap.argtypes = [ct.c_int, ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                        ct.c_int, ct.c_double, ct.c_int, ct.c_int, ct.c_double, ct.c_int, ct.c_int, ct.c_int,
+                        ct.c_int, ct.c_int, _up, _i32p, _vp, _vp, _vp, _vp, ct.c_int, _i32p, ct.c_int, _i32p,
+                        ct.c_int, _i32p, _f64p, _f64p, _f64p, ct.c_double, _f64p]
     L.MLmap.restype = None
     L.MLmap.argtypes = [ct.c_int, ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double,
                         ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _i32p, ct.c_int, _i32p, ct.c_int, _i32p,
@@ -201,6 +260,25 @@ def MLmap(outpath, ref, precond, Z_2lvl, tol, maxiter, nside, local_blocks_sizes
                  _c(noise, np.float64), int(lam), _c(invtt, np.float64))
 
 
+def MTmap(outpath, ref, precond, Z_2lvl, tol, maxiter, nside, meta, local_blocks_sizes, nnz, pix, pixweights,
+          signal, noise, invtt, solver=0):
+    """MTmap called as the reference's wrapper calls it (one rank); meta holds
+    the template inputs (npoly, nhwp, delta_t, ground, n_sss_bins,
+    sampling_freq, nces and per-CES sweeptstamps / az / az_min / az_max /
+    hwp_angle).  pix, pixweights and signal are modified in place."""
+    bs = _c(local_blocks_sizes, np.int32)
+    m = np.array([_c(signal, np.float64).size], np.int32)
+    _, keep = templates_spec(meta)
+    vp = lambda k: keep[k].ctypes.data_as(ct.c_void_p) if k in keep else None
+    load().MTmap(0, outpath.encode(), ref.encode(), int(solver), int(precond), int(Z_2lvl), 6, int(meta["npoly"]),
+                 int(meta["nhwp"]), float(meta["delta_t"]), int(meta["ground"]), int(meta["n_sss_bins"]), float(tol),
+                 int(maxiter), 1, 0, 0, int(nside), keep["sweeptstamps"], keep["nsweeps"], vp("az"), vp("az_min"),
+                 vp("az_max"), vp("hwp_angle"),
+                 int(meta["nces"]), m, int(bs.size), bs, int(nnz), pix, pixweights, signal, _c(noise, np.float64),
+                 float(meta["sampling_freq"]), _c(invtt, np.float64))
+    del keep
+
+
 def read_map(filename):
     """Column SIGNAL of a HEALPix FITS binary table as written by write_map."""
     with open(filename, "rb") as f:
@@ -265,6 +343,7 @@ class Problem:
                                                _c(invtt, np.float64), ct.byref(self.opts)),
                "problem_create")
         self.n_map = None
+        self._block_count = bs.size
 
     @classmethod
     def from_device(cls, m, d_pix, d_w, block_sizes, lam, d_invtt, nnz=3, opts: Options | None = None):
@@ -278,6 +357,7 @@ class Problem:
                                                       bs, int(lam), _vp(d_invtt), ct.byref(self.opts)),
                "problem_create_device")
         self.n_map = None
+        self._block_count = bs.size
         return self
 
     def close(self):
@@ -296,6 +376,43 @@ class Problem:
 
     def __exit__(self, *a):
         self.close()
+
+    # -- template marginalisation (MTmap) -----------------------------------
+    def set_templates(self, meta: dict):
+        """Attach the template classes described by meta (see MTmap)."""
+        spec, keep = templates_spec(meta)
+        _check(load().mappraiser_gpu_templates_set(self._h, ct.byref(spec)), "templates_set")
+        del keep
+        ntot = ct.c_longlong()
+        _check(load().mappraiser_gpu_templates_info(self._h, ct.byref(ntot), None, None), "templates_info")
+        self.n_templates = int(ntot.value)
+
+    def templates_info(self):
+        nb = self._nb()
+        nt = np.zeros(nb, np.int32)
+        rk = np.zeros(nb, np.int32)
+        ntot = ct.c_longlong()
+        _check(load().mappraiser_gpu_templates_info(self._h, ct.byref(ntot), nt.ctypes.data_as(_vp),
+                                                    rk.ctypes.data_as(_vp)), "templates_info")
+        return int(ntot.value), nt, rk
+
+    def _nb(self):
+        return int(self._block_count)
+
+    def templates_apply(self, op, v):
+        n_out = self.n_templates if op == MGPU_TMPL_TT else self.m
+        out = np.zeros(n_out, np.float64)
+        _check(load().mappraiser_gpu_templates_apply(self._h, int(op), _c(v, np.float64), out), "templates_apply")
+        return out
+
+    def templates_kernel(self):
+        _, nt, _ = self.templates_info()
+        n = int((nt.astype(np.int64) ** 2).sum())
+        B = np.zeros(max(n, 1), np.float64)
+        Bi = np.zeros(max(n, 1), np.float64)
+        _check(load().mappraiser_gpu_templates_kernel(self._h, B.ctypes.data_as(_vp), Bi.ctypes.data_as(_vp)),
+               "templates_kernel")
+        return B[:n], Bi[:n]
 
     # -- MatLocalShape ------------------------------------------------------
     def local_shape(self, want_indices=True):

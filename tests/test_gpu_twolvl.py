@@ -82,11 +82,27 @@ def test_twolvl_graph_and_mixed(gpu):
     assert _rel(mx.x, e.x) <= 1e-6
 
 
-def test_twolvl_rejected_where_unsupported(gpu):
+@pytest.mark.parametrize("precond,zn", [(1, 2), (2, 4)])
+def test_twolvl_lambda1_iterates(gpu, precond, zn):
+    """White noise (lambda 1): the two-level products call stbmmProd, which
+    still removes the block means, while the PCG's own N^-1 (apply_weights) does
+    not (pcg_true.c:578-600)."""
     wl = W.config_small(seed=11, ndet=2, nsamp=4096, nx=16, ny=16, lam=1)
-    with pytest.raises(gpu.GPUError, match="lambda"):
-        gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
-                    maxiter=20, opts=gpu.options(precond=1, z_2lvl=2))
+    print(f"case: lambda 1, precond {precond} Zn {zn}, x_3", flush=True)
+    r = O.gls_solve(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-30,
+                    maxiter=50, stop_after=3, precond=precond, z_2lvl=zn)
+    with gpu.Problem(wl.pix, wl.weights, wl.block_sizes, wl.lam, wl.invtt,
+                     opts=gpu.options(precond=precond, z_2lvl=zn)) as pb:
+        sig, _, _ = pb.precond_build(wl.signal)
+        x, hist, rep = pb.pcg(sig, wl.noise, tol=1e-30, maxiter=50, stop_after=3)
+    assert rep.iterations == r.iterations == 3
+    print(f"x_3 rel diff {_rel(x, r.x):.2e}", flush=True)
+    assert _rel(x, r.x) <= 1e-10
+    # block Jacobi is exact for white noise: the residuals after the first step are rounding
+    np.testing.assert_allclose(hist, r.res_hist, rtol=1e-9, atol=1e-12)
+
+
+def test_twolvl_rejected_where_unsupported(gpu):
     wl = W.config_small(seed=11, ndet=2, nsamp=4096, nx=16, ny=16, lam=64)
     item = dict(pix=wl.pix, weights=wl.weights, signal=wl.signal, noise=wl.noise, block_sizes=wl.block_sizes,
                 lam=wl.lam, invtt=wl.invtt)
