@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--serve-rates", default="1,2,5,10", help="Poisson arrival rates (req/s)")
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded requests cycled by the load")
     ap.add_argument("--no-noise-model", action="store_true", help="skip the invtt-from-noise section")
+    ap.add_argument("--no-templates", action="store_true", help="skip the template map-making (MTmap) section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device to device once before the solves (byte-counter calibration)")
     args = ap.parse_args()
@@ -251,11 +252,49 @@ def main():
         out["noise_model_c2"] = noise_section(G, wl, args)
     del d_pix, d_w, d_sig0, d_sig, d_noise, d_inv, wl, gh
     torch.cuda.empty_cache()
+    if not args.no_templates:
+        out["templates_c6"] = templates_section(G, W, args)
     if args.c4_real > 0:
         out["batch_c4"] = c4_section(G, W, torch, args)
     if args.serve_requests > 0:
         out["serve_c5"] = serve_section(G, W, args)
     print(json.dumps(out), flush=True)
+
+
+def templates_section(G, W, args):
+    """C6: MTmap's template marginalisation at the C2 scale, from host arrays
+    (problem, template tables, kernels and their pseudo-inverses, block-Jacobi
+    preconditioner, PCG_GLS_templates), FP64; per-kernel-class device times of
+    the solve."""
+    wl = W.config_c6()
+    log(f"C6: generated {wl.name}")
+
+    def run(profile):
+        t0 = time.perf_counter()
+        pb = G.Problem(wl.pix, wl.weights, wl.block_sizes, 1, wl.invtt, opts=G.options(profile=profile))
+        pb.set_templates(wl.meta)
+        t_setup = time.perf_counter() - t0
+        sig, _, _ = pb.precond_build(wl.signal)
+        x, hist, rep = pb.pcg(sig, wl.noise, tol=args.tol, maxiter=args.maxiter)
+        total = time.perf_counter() - t0
+        kt = pb.kernel_times() if profile else None
+        ntot, nt, ranks = pb.templates_info()
+        pb.close()
+        return total, t_setup, rep, kt, ntot, nt, ranks
+
+    run(False)  # warm-up (library and solver initialisation)
+    total, t_setup, rep, _, ntot, nt, ranks = run(False)
+    _, _, rep_p, kt, _, _, _ = run(True)
+    it = max(rep.iterations, 1)
+    tm_ms = kt["templates"][0]
+    log(f"C6: {total:.3f} s from host arrays (templates setup {t_setup:.3f} s), {rep.iterations} iterations, "
+        f"templates {tm_ms / max(kt['templates'][1], 1):.3f} ms per application")
+    return {"workload": wl.name, "time_to_map_from_host_s": round(total, 4), "setup_s": round(t_setup, 4),
+            "pcg_device_ms": round(rep.t_solve_ms, 2), "iterations": int(rep.iterations),
+            "converged": bool(rep.converged), "templates_total": int(ntot), "templates_per_block": int(nt[0]),
+            "kernel_rank_min": int(ranks.min()), "kernel_rank_max": int(ranks.max()),
+            "templates_ms_per_application": round(tm_ms / max(kt["templates"][1], 1), 4),
+            "kernel_ms_per_map": {k: round(v[0], 2) for k, v in kt.items() if v[1] > 0}}
 
 
 def noise_section(G, wl, args):

@@ -350,8 +350,8 @@ struct TwoLevel {
 // sample is shared by the CES's detectors.
 // ---------------------------------------------------------------------------
 struct TItem {           // one workgroup of T^T: a sweep, an azimuth bin or a time bin of a block
-    int b, kind, lo, hi; // kind 0 sweep, 1 azimuth bin (range in the azimuth order), 2 time bin
-    int64_t out0;        // first output amplitude
+    int b, kind, lo, hi; // block; kind 0 sweep, 1 azimuth bin (range in the azimuth order), 2 time bin
+    int64_t rel;         // first output amplitude, relative to the block's amplitudes
     int stride, pad;     // distance between the class outputs (sweeps or time bins)
 };
 struct BTask {           // one kernel entry (and its mirror) summed in sample order
@@ -480,6 +480,8 @@ int launch_precond_apply(Problem &pb, const double *d_g, double *d_Cg);
 int templates_build(Problem &pb, const mgpu_templates *t);
 // y (a TOD already weighted by N^-1) -= W T Binv T^T y
 int templates_filter(Problem &pb, double *d_y);
+// y = W q - W T Binv T^T W q: apply_weights (lambda 1) and the template part in one pass
+int templates_noise(Problem &pb, const double *d_q, double *d_y);
 // out = T a (TVecProd) or out = T^T y (TrTVecProd) on device buffers
 int templates_tvec(Problem &pb, const double *d_a, double *d_out);
 int templates_trtvec(Problem &pb, const double *d_y, double *d_out);

@@ -488,8 +488,10 @@ int apply_A64(Problem &pb, const double *d_v, double *d_out, double *d_dot) {
     pb.prof.begin(MGPU_KT_P, pb.stream);
     MGPU_RET(launch_matvec(pb, d_v, pb.d_tod, true));
     pb.prof.end(pb.stream);
-    MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, false));
-    if (pb.tm.live) MGPU_RET(templates_filter(pb, pb.d_tod2));  // PCG_GLS_templates, pcg_true.c:426-458
+    if (pb.tm.live)  // apply_weights and the template part of PCG_GLS_templates (pcg_true.c:426-458)
+        MGPU_RET(templates_noise(pb, pb.d_tod, pb.d_tod2));
+    else
+        MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, false));
     pb.prof.begin(MGPU_KT_PT, pb.stream);
     MGPU_RET(launch_trmatvec(pb, pb.d_tod2, d_out, d_dot ? d_v : nullptr, d_dot, nullptr));
     pb.prof.end(pb.stream);
@@ -579,8 +581,10 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     pb.tm.live = pb.tm.on;
     k_rhs_tod<<<grid_for(m, 256), 256, 0, s>>>(d_signal, d_noise, pb.d_tod, m);
     MGPU_LAUNCHED();
-    MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, true));
-    if (pb.tm.live) MGPU_RET(templates_filter(pb, pb.d_tod2));  // pcg_true.c:318-371
+    if (pb.tm.live)  // pcg_true.c:318-371
+        MGPU_RET(templates_noise(pb, pb.d_tod, pb.d_tod2));
+    else
+        MGPU_RET(toeplitz_apply(pb, pb.d_tod, pb.d_tod2, true));
     MGPU_RET(launch_trmatvec(pb, pb.d_tod2, gA, nullptr, nullptr, nullptr));
     MGPU_HIP(hipMemcpyAsync(bvec, gA, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     if (two) {

@@ -69,46 +69,53 @@ __device__ __forceinline__ double wg_sum(double v, double *red) {
 // ---------------------------------------------------------------------------
 // kernels: PCG iteration
 // ---------------------------------------------------------------------------
-// T^T y: item = (block, group); kind 0 sums leg[o] * y over a sweep for every
-// order, kind 1 sums y over an azimuth bin (its samples listed in the azimuth
-// order), kind 2 sums hc[h] * y over a time bin for every HWP slot.
+// T^T (s_b y): one workgroup per item (a sweep, azimuth bin or time bin of a
+// block; blocks in order, so the detectors of a CES follow each other and
+// reuse its table entries from L2 / MALL); kind 0 sums leg[o] * v over a
+// sweep for every order, kind 1 sums v over an azimuth bin (its samples listed
+// in the azimuth order), kind 2 sums hc[h] * v over a time bin for every HWP
+// slot, with v = s_b * y (s_b = T0[b], the weighting of apply_weights fused in;
+// 1 when scale is NULL)
 __global__ __launch_bounds__(kTW) void k_tmpl_tr(const TItem *__restrict__ items, const double *__restrict__ y,
+                                                 const double *__restrict__ scale,
                                                  const int64_t *__restrict__ start, const int64_t *__restrict__ c0,
                                                  int ndet, const int *__restrict__ perm_az,
                                                  const double *__restrict__ leg, const double *__restrict__ hc,
-                                                 int64_t ncs, int npoly, int nh2, double *__restrict__ out) {
+                                                 int64_t ncs, int npoly, int nh2, const int64_t *__restrict__ off,
+                                                 double *__restrict__ out) {
     __shared__ double red[kTW / 64];
     const TItem it = items[blockIdx.x];
-    const double *__restrict__ yb = y + start[it.b];
-    const int64_t cb = c0[it.b / ndet];
+    const int b = it.b;
+    const int64_t cb = c0[b / ndet];
+    const int nk = it.kind == 0 ? npoly : it.kind == 1 ? 1 : nh2;
+    const double *__restrict__ yb = y + start[b];
+    const double sc = scale ? scale[b] : 1.0;
     double acc[kSlots];
 #pragma unroll
     for (int k = 0; k < kSlots; k++) acc[k] = 0.0;
-    int nk = 1;
     if (it.kind == 0) {
-        nk = npoly;
         for (int i = it.lo + (int) threadIdx.x; i < it.hi; i += kTW) {
-            const double v = yb[i];
+            const double v = sc * yb[i];
 #pragma unroll
             for (int k = 0; k < kSlots; k++)
                 if (k < npoly) acc[k] += leg[(int64_t) k * ncs + cb + i] * v;
         }
     } else if (it.kind == 1) {
-        for (int i = it.lo + (int) threadIdx.x; i < it.hi; i += kTW) acc[0] += yb[perm_az[cb + i]];
+        for (int i = it.lo + (int) threadIdx.x; i < it.hi; i += kTW) acc[0] += sc * yb[perm_az[cb + i]];
     } else {
-        nk = nh2;
         for (int i = it.lo + (int) threadIdx.x; i < it.hi; i += kTW) {
-            const double v = yb[i];
+            const double v = sc * yb[i];
 #pragma unroll
             for (int k = 0; k < kSlots; k++)
                 if (k < nh2) acc[k] += hc[(int64_t) k * ncs + cb + i] * v;
         }
     }
+    const int64_t o = off[b] + it.rel;
 #pragma unroll
     for (int k = 0; k < kSlots; k++) {
         if (k >= nk) break;
         const double s = wg_sum(acc[k], red);
-        if (threadIdx.x == 0) out[it.out0 + (int64_t) k * it.stride] = s;
+        if (threadIdx.x == 0) out[o + (int64_t) k * it.stride] = s;
     }
 }
 
@@ -130,11 +137,14 @@ __global__ __launch_bounds__(256) void k_tmpl_binv(const double *__restrict__ Bi
     if (lane == 0) a2[r] = s;
 }
 
-// y[t] -= w_b (T a2)[t] for every detector of the CES at CES sample i; the
-// classes are added to 0 in class order (TVecProd), then scaled by w_b
-// (apply_weights) and subtracted (pcg_true.c:453-454)
-__global__ __launch_bounds__(256) void k_tmpl_sub(double *__restrict__ y, const int64_t *__restrict__ start,
-                                                  const double *__restrict__ T0, int nces, int ndet,
+// y[t] = v[t] - w_b (T a2)[t], v = in_scale_b q (the fused apply_weights) or q,
+// for every detector of the CES at CES sample i; the classes are added to 0 in
+// class order (TVecProd), then scaled by w_b (apply_weights) and subtracted
+// (pcg_true.c:453-454).  INPLACE: q is y.
+template <bool INPLACE>
+__global__ __launch_bounds__(256) void k_tmpl_sub(const double *__restrict__ q, const double *__restrict__ in_scale,
+                                                  double *__restrict__ y, const int64_t *__restrict__ start,
+                                                  const double *__restrict__ T0, int ndet,
                                                   const int64_t *__restrict__ c0, const int *__restrict__ ns_c,
                                                   const int *__restrict__ nh_c, const int *__restrict__ sweep,
                                                   const int *__restrict__ az, const double *__restrict__ leg,
@@ -156,6 +166,7 @@ __global__ __launch_bounds__(256) void k_tmpl_sub(double *__restrict__ y, const 
     }
     const int a = ground ? az[g] : 0;
     const int tb = nh2 ? (int) floor((double) i / fsdt) : 0;
+    const double *__restrict__ src = INPLACE ? y : q;
     for (int j = 0; j < ndet; j++) {
         const int b = c * ndet + j;
         int64_t base = off[b];
@@ -175,7 +186,8 @@ __global__ __launch_bounds__(256) void k_tmpl_sub(double *__restrict__ y, const 
             if (k < nh2) r += wh[k] * a2[base + (int64_t) k * nh + tb];
         const int64_t t = start[b] + i;
         const double tv = T0[b] * r;
-        y[t] = y[t] - tv;
+        const double v = in_scale ? in_scale[b] * src[t] : src[t];
+        y[t] = v - tv;
     }
 }
 
@@ -585,27 +597,26 @@ int templates_build(Problem &pb, const mgpu_templates *t) {
     std::vector<int> row_block((size_t) std::max<int64_t>(M.ntot, 1), 0);
     for (int b = 0; b < nb; b++)
         for (int64_t r = M.h_off[b]; r < M.h_off[b + 1]; r++) row_block[r] = b;
-    // T^T items: (block, sweep), (block, azimuth bin), (block, time bin)
+    // T^T items of every block in block order: sweeps, azimuth bins, time bins
+    // (rel: the output offset inside the block's amplitudes)
     std::vector<TItem> items;
     for (int b = 0; b < nb; b++) {
-        const int c = b / ndet;
-        const CesTables &C = ct[c];
-        const int64_t o = M.h_off[b];
+        const CesTables &C = ct[b / ndet];
         if (M.npoly)
-            for (int sw = 0; sw < C.ns; sw++) items.push_back(TItem{b, 0, C.stamps[sw], C.stamps[sw + 1], o + sw, C.ns, 0});
+            for (int sw = 0; sw < C.ns; sw++) items.push_back(TItem{b, 0, C.stamps[sw], C.stamps[sw + 1], sw, C.ns, 0});
         if (M.ground) {
             // azimuth bin a occupies a contiguous range of the azimuth order
             int k = 0;
             for (int a = 0; a < M.nsss; a++) {
                 int e = k;
                 while (e < C.len && C.az[C.perm[0][e]] == a) e++;
-                items.push_back(TItem{b, 1, k, e, o + (int64_t) M.npoly * C.ns + a, 0, 0});
+                items.push_back(TItem{b, 1, k, e, (int64_t) M.npoly * C.ns + a, 0, 0});
                 k = e;
             }
         }
         if (nh2)
             for (int q = 0; q < C.nh; q++)
-                items.push_back(TItem{b, 2, C.tbs[q], C.tbs[q + 1], o + (int64_t) M.npoly * C.ns + M.nsss + q, C.nh, 0});
+                items.push_back(TItem{b, 2, C.tbs[q], C.tbs[q + 1], (int64_t) M.npoly * C.ns + M.nsss + q, C.nh, 0});
     }
     M.nitems = (int) items.size();
     // CES of every k_tmpl_sub workgroup (a workgroup never straddles two CES)
@@ -753,14 +764,18 @@ int templates_build(Problem &pb, const mgpu_templates *t) {
 // ---------------------------------------------------------------------------
 // application
 // ---------------------------------------------------------------------------
-int templates_trtvec(Problem &pb, const double *d_y, double *d_out) {
+static int launch_tr(Problem &pb, const double *d_y, const double *scale, double *d_out) {
     Templates &M = pb.tm;
-    MGPU_ARG(M.on, "no templates attached");
     if (M.nitems == 0) return MGPU_OK;
-    k_tmpl_tr<<<M.nitems, kTW, 0, pb.stream>>>(M.d_items, d_y, pb.tp.d_start, M.d_c0, M.ndet, M.d_perm, M.d_leg,
-                                              M.d_hc, M.ncs, M.npoly, 2 * M.nhwp, d_out);
+    k_tmpl_tr<<<(unsigned) M.nitems, kTW, 0, pb.stream>>>(M.d_items, d_y, scale, pb.tp.d_start, M.d_c0, M.ndet, M.d_perm,
+                                              M.d_leg, M.d_hc, M.ncs, M.npoly, 2 * M.nhwp, M.d_off, d_out);
     MGPU_LAUNCHED();
     return MGPU_OK;
+}
+
+int templates_trtvec(Problem &pb, const double *d_y, double *d_out) {
+    MGPU_ARG(pb.tm.on, "no templates attached");
+    return launch_tr(pb, d_y, nullptr, d_out);
 }
 
 int templates_tvec(Problem &pb, const double *d_a, double *d_out) {
@@ -774,22 +789,40 @@ int templates_tvec(Problem &pb, const double *d_a, double *d_out) {
     return MGPU_OK;
 }
 
-int templates_filter(Problem &pb, double *d_y) {
+// y = s q - W T B^+ T^T (s q), s = W (in_scale) or 1
+static int apply_tm(Problem &pb, const double *d_q, const double *in_scale, double *d_y) {
     Templates &M = pb.tm;
     MGPU_ARG(M.on, "no templates attached");
     hipStream_t s = pb.stream;
-    if (M.ntot == 0) return MGPU_OK;
     pb.prof.begin(MGPU_KT_TMPL, s);
-    MGPU_RET(templates_trtvec(pb, d_y, M.d_a1));
-    k_tmpl_binv<<<grid_for(M.ntot, 4), 256, 0, s>>>(M.d_Binv, M.d_boff, M.d_off, M.d_row_block, M.ntot, M.d_a1,
-                                                    M.d_a2);
-    MGPU_LAUNCHED();
-    k_tmpl_sub<<<M.nwg_samples, 256, 0, s>>>(d_y, pb.tp.d_start, pb.tp.d_T0, M.nces, M.ndet, M.d_c0, M.d_ns,
-                                             M.d_nh, M.d_sweep, M.d_az, M.d_leg, M.d_hc, M.ncs, M.npoly, M.ground,
-                                             M.nsss, 2 * M.nhwp, M.fsdt, M.d_off, M.d_a2, M.d_ces_of, M.d_len);
+    MGPU_RET(launch_tr(pb, d_q, in_scale, M.d_a1));
+    if (M.ntot > 0) {
+        k_tmpl_binv<<<grid_for(M.ntot, 4), 256, 0, s>>>(M.d_Binv, M.d_boff, M.d_off, M.d_row_block, M.ntot, M.d_a1,
+                                                        M.d_a2);
+        MGPU_LAUNCHED();
+    } else {
+        MGPU_HIP(hipMemsetAsync(M.d_a2, 0, sizeof(double), s));
+    }
+    if (d_q == d_y)
+        k_tmpl_sub<true><<<M.nwg_samples, 256, 0, s>>>(nullptr, in_scale, d_y, pb.tp.d_start, pb.tp.d_T0, M.ndet,
+                                                       M.d_c0, M.d_ns, M.d_nh, M.d_sweep, M.d_az, M.d_leg, M.d_hc,
+                                                       M.ncs, M.npoly, M.ground, M.nsss, 2 * M.nhwp, M.fsdt, M.d_off,
+                                                       M.d_a2, M.d_ces_of, M.d_len);
+    else
+        k_tmpl_sub<false><<<M.nwg_samples, 256, 0, s>>>(d_q, in_scale, d_y, pb.tp.d_start, pb.tp.d_T0, M.ndet,
+                                                        M.d_c0, M.d_ns, M.d_nh, M.d_sweep, M.d_az, M.d_leg, M.d_hc,
+                                                        M.ncs, M.npoly, M.ground, M.nsss, 2 * M.nhwp, M.fsdt,
+                                                        M.d_off, M.d_a2, M.d_ces_of, M.d_len);
     MGPU_LAUNCHED();
     pb.prof.end(s);
     return MGPU_OK;
+}
+
+int templates_filter(Problem &pb, double *d_y) { return apply_tm(pb, d_y, nullptr, d_y); }
+
+int templates_noise(Problem &pb, const double *d_q, double *d_y) {
+    MGPU_ARG(pb.tp.lambda == 1, "templates need lambda = 1");
+    return apply_tm(pb, d_q, pb.tp.d_T0, d_y);
 }
 
 }  // namespace mgpu

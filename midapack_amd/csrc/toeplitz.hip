@@ -699,24 +699,34 @@ __global__ void k_copy_tail(const R *__restrict__ y, R *__restrict__ z, int64_t 
 }
 
 // lambda == 1: z = T_b[0] * y inside blocks (no mean removal), copy after them
+// block of sample t for the threads of one workgroup: the block of the
+// workgroup's first sample is found once (uniform, scalar loads), each lane then
+// steps over the few block boundaries inside the workgroup
+__device__ __forceinline__ int wg_block_of(const int64_t *__restrict__ start, int nb, int64_t t) {
+    int b = find_block_t(start, nb, (int64_t) blockIdx.x * blockDim.x);
+    while (b < nb && t >= start[b + 1]) b++;
+    return b;
+}
+
 template <typename R>
-__global__ void k_diag(const R *__restrict__ y, R *__restrict__ z, int64_t m, const int64_t *__restrict__ start,
-                       int nb, const double *__restrict__ T0) {
+__global__ __launch_bounds__(256) void k_diag(const R *__restrict__ y, R *__restrict__ z, int64_t m,
+                                              const int64_t *__restrict__ start, int nb,
+                                              const double *__restrict__ T0) {
     int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
-    const int b = find_block_t(start, nb, t);
+    const int b = wg_block_of(start, nb, t);
     z[t] = b < nb ? (R) T0[b] * y[t] : y[t];
 }
 
 // lambda == 1 with stbmmProd's offsets (toeplitz_wizard.c:86-95 ahead of the
 // diagonal product): z = T_b[0] * (y - mean_b) inside blocks, copy after them
 template <typename R>
-__global__ void k_diag_mean(const R *__restrict__ y, R *__restrict__ z, int64_t m,
-                            const int64_t *__restrict__ start, int nb, const double *__restrict__ T0,
-                            const double *__restrict__ mean) {
+__global__ __launch_bounds__(256) void k_diag_mean(const R *__restrict__ y, R *__restrict__ z, int64_t m,
+                                                   const int64_t *__restrict__ start, int nb,
+                                                   const double *__restrict__ T0, const double *__restrict__ mean) {
     int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
-    const int b = find_block_t(start, nb, t);
+    const int b = wg_block_of(start, nb, t);
     z[t] = b < nb ? (R) (T0[b] * ((double) y[t] - mean[b])) : y[t];
 }
 

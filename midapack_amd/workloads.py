@@ -343,3 +343,13 @@ def config_templates_small(seed=21, nces=2, ndet=2, ces_len=2048, nx=32, ny=16, 
     """Small MTmap case for parity tests (the oracle solves it in about a second)."""
     return make_templates_workload(f"templates {nces}x{ndet}x{ces_len} {nx}x{ny}", nces, ndet, ces_len, nx, ny,
                                    seed, **kw)
+
+
+def config_c6(seed=6):
+    """Template map-making (MTmap) at the C2 scale: 8 CES x 8 detectors x 2^20
+    samples on 1024^2, white noise (lambda 1), Legendre order 0 on every
+    1024-sample sweep, 32 azimuth bins of ground pickup, the first HWP harmonic
+    on 128 time bins per CES (1312 templates per detector and CES)."""
+    return make_templates_workload("C6: MTmap 8 CES x 8 det x 2^20 on 1024^2, 1312 templates per block", 8, 8,
+                                   1 << 20, 1024, 1024, seed, npoly=1, ground=1, n_sss_bins=32, nhwp=1,
+                                   delta_t=(1 << 20) / FSAMP / 128)
