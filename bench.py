@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded requests cycled by the load")
     ap.add_argument("--no-noise-model", action="store_true", help="skip the invtt-from-noise section")
     ap.add_argument("--no-templates", action="store_true", help="skip the template map-making (MTmap) section")
+    ap.add_argument("--only-templates", action="store_true",
+                    help="run only the template map-making section (C6) and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device to device once before the solves (byte-counter calibration)")
     args = ap.parse_args()
@@ -98,6 +100,11 @@ def main():
 
     from midapack_amd import lib as G
     from midapack_amd import workloads as W
+
+    if args.only_templates:
+        G.init(0)
+        print(json.dumps({"templates_c6": templates_section(G, W, args)}), flush=True)
+        return
 
     t = time.time()
     wl = W.config_c2(ndet=args.ndet)

@@ -82,8 +82,9 @@ __global__ __launch_bounds__(kTW) void k_tmpl_tr(const TItem *__restrict__ items
                                                  int ndet, const int *__restrict__ perm_az,
                                                  const double *__restrict__ leg, const double *__restrict__ hc,
                                                  int64_t ncs, int npoly, int nh2, const int64_t *__restrict__ off,
-                                                 double *__restrict__ out) {
+                                                 double *__restrict__ out, const int *__restrict__ state) {
     __shared__ double red[kTW / 64];
+    if (state && *state) return;  // the solve has stopped: queued iterations do no work
     const TItem it = items[blockIdx.x];
     const int b = it.b;
     const int64_t cb = c0[b / ndet];
@@ -123,7 +124,9 @@ __global__ __launch_bounds__(kTW) void k_tmpl_tr(const TItem *__restrict__ items
 __global__ __launch_bounds__(256) void k_tmpl_binv(const double *__restrict__ Binv, const int64_t *__restrict__ boff,
                                                    const int64_t *__restrict__ off,
                                                    const int *__restrict__ row_block, int64_t ntot,
-                                                   const double *__restrict__ a1, double *__restrict__ a2) {
+                                                   const double *__restrict__ a1, double *__restrict__ a2,
+                                                   const int *__restrict__ state) {
+    if (state && *state) return;
     const int64_t r = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= ntot) return;  // uniform per wave
     const int lane = threadIdx.x & 63;
@@ -151,7 +154,8 @@ __global__ __launch_bounds__(256) void k_tmpl_sub(const double *__restrict__ q, 
                                                   const double *__restrict__ hc, int64_t ncs, int npoly, int ground,
                                                   int nsss, int nh2, double fsdt, const int64_t *__restrict__ off,
                                                   const double *__restrict__ a2, const int *__restrict__ ces_of,
-                                                  const int *__restrict__ len) {
+                                                  const int *__restrict__ len, const int *__restrict__ state) {
+    if (state && *state) return;
     const int64_t g = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     const int c = ces_of[blockIdx.x];
     const int i = (int) (g - c0[c]);
@@ -529,6 +533,7 @@ int templates_build(Problem &pb, const mgpu_templates *t) {
     MGPU_ARG(T.lambda == 1, "templates need the white-noise model (lambda = 1, as MTmap), the problem has lambda %d",
              T.lambda);
     MGPU_ARG(pb.opt.precision == MGPU_FP64, "templates are solved in FP64 only");
+    MGPU_ARG(pb.sg.nseg == 1, "templates solve one problem at a time");
     MGPU_ARG(t->nces > 0 && T.nb % t->nces == 0, "nb = %d blocks is not a multiple of nces = %d", T.nb, t->nces);
     MGPU_ARG(t->npoly >= 0 && t->npoly <= kSlots, "npoly = %d outside [0, %d]", t->npoly, kSlots);
     MGPU_ARG(t->nhwp >= 0 && 2 * t->nhwp <= kSlots, "nhwp = %d outside [0, %d]", t->nhwp, kSlots / 2);
@@ -767,8 +772,9 @@ int templates_build(Problem &pb, const mgpu_templates *t) {
 static int launch_tr(Problem &pb, const double *d_y, const double *scale, double *d_out) {
     Templates &M = pb.tm;
     if (M.nitems == 0) return MGPU_OK;
-    k_tmpl_tr<<<(unsigned) M.nitems, kTW, 0, pb.stream>>>(M.d_items, d_y, scale, pb.tp.d_start, M.d_c0, M.ndet, M.d_perm,
-                                              M.d_leg, M.d_hc, M.ncs, M.npoly, 2 * M.nhwp, M.d_off, d_out);
+    k_tmpl_tr<<<(unsigned) M.nitems, kTW, 0, pb.stream>>>(M.d_items, d_y, scale, pb.tp.d_start, M.d_c0, M.ndet,
+                                                          M.d_perm, M.d_leg, M.d_hc, M.ncs, M.npoly, 2 * M.nhwp,
+                                                          M.d_off, d_out, pb.live_state);
     MGPU_LAUNCHED();
     return MGPU_OK;
 }
@@ -798,7 +804,7 @@ static int apply_tm(Problem &pb, const double *d_q, const double *in_scale, doub
     MGPU_RET(launch_tr(pb, d_q, in_scale, M.d_a1));
     if (M.ntot > 0) {
         k_tmpl_binv<<<grid_for(M.ntot, 4), 256, 0, s>>>(M.d_Binv, M.d_boff, M.d_off, M.d_row_block, M.ntot, M.d_a1,
-                                                        M.d_a2);
+                                                        M.d_a2, pb.live_state);
         MGPU_LAUNCHED();
     } else {
         MGPU_HIP(hipMemsetAsync(M.d_a2, 0, sizeof(double), s));
@@ -807,12 +813,12 @@ static int apply_tm(Problem &pb, const double *d_q, const double *in_scale, doub
         k_tmpl_sub<true><<<M.nwg_samples, 256, 0, s>>>(nullptr, in_scale, d_y, pb.tp.d_start, pb.tp.d_T0, M.ndet,
                                                        M.d_c0, M.d_ns, M.d_nh, M.d_sweep, M.d_az, M.d_leg, M.d_hc,
                                                        M.ncs, M.npoly, M.ground, M.nsss, 2 * M.nhwp, M.fsdt, M.d_off,
-                                                       M.d_a2, M.d_ces_of, M.d_len);
+                                                       M.d_a2, M.d_ces_of, M.d_len, pb.live_state);
     else
         k_tmpl_sub<false><<<M.nwg_samples, 256, 0, s>>>(d_q, in_scale, d_y, pb.tp.d_start, pb.tp.d_T0, M.ndet,
                                                         M.d_c0, M.d_ns, M.d_nh, M.d_sweep, M.d_az, M.d_leg, M.d_hc,
                                                         M.ncs, M.npoly, M.ground, M.nsss, 2 * M.nhwp, M.fsdt,
-                                                        M.d_off, M.d_a2, M.d_ces_of, M.d_len);
+                                                        M.d_off, M.d_a2, M.d_ces_of, M.d_len, pb.live_state);
     MGPU_LAUNCHED();
     pb.prof.end(s);
     return MGPU_OK;
