@@ -28,6 +28,11 @@
  *       equations (with mean-removal N^-1), and x_k against a dense-operator PCG.
  *   orc_pinv / orc_symeig -- against numpy's LAPACK pinv, lstsq and eigh
  *       (full-rank, rank-deficient, badly scaled), Penrose conditions.
+ *   orc_tmpl_* / orc_gls_solve_tmpl (templates.c, tests/test_templates_oracle.py) --
+ *       T, T^T and the kernels against a dense numpy T from the definitions, one
+ *       kernel entry against its ordered sum, InvKernel against the Penrose
+ *       conditions and numpy's pinv/rank, the filter against the dense projector,
+ *       the template PCG against the dense normal equations and template invariance.
  *   orc_gls_solve2 (two-level, tests/test_twolvl_oracle.py) -- build_Z column 0
  *       against brute-force hit fractions and the other columns through the
  *       reference's own index; A Z against the pinned products; the Lanczos Z
