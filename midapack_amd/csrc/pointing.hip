@@ -37,6 +37,7 @@ namespace mgpu {
 
 constexpr int kSlice = 64;          // ELL slice height (one wavefront)
 constexpr int kPBlock = 256;        // P workgroup size (kChunk / 4 samples per lane)
+constexpr int kChunkGroup = 32;     // consecutive P chunks kept together in the chunk order
 constexpr int kClassifyGrid = 2048; // grid-stride validation: 8 workgroups per CU
 
 // ---------------------------------------------------------------------------
@@ -308,8 +309,22 @@ static int build_chunk_order(Problem &pb) {
     const int nch = (int) ((P.m + kChunk - 1) / kChunk);
     std::vector<int> key, order((size_t) nch), perm((size_t) nch);
     MGPU_RET(pointing_chunk_keys(pb, key));
-    for (int c = 0; c < nch; c++) order[c] = c;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    // Groups of kChunkGroup consecutive chunks, sorted by their smallest pixel:
+    // the map rows a group gathers stay in L2 while the detectors' passes over
+    // them run back to back, and each group streams its pointing from HBM as
+    // one contiguous run.  On the 1024-sample sweeps of the 64 x 2^20 raster,
+    // P takes 0.388 ms with single chunks, 0.361 ms with groups of 16, 0.353 ms
+    // with 32 or 64, 0.417 ms with 128 (the gathers start to miss L2).
+    {
+        const int ng = (nch + kChunkGroup - 1) / kChunkGroup;
+        std::vector<int> gkey((size_t) ng, INT32_MAX), gord((size_t) ng);
+        for (int c = 0; c < nch; c++) gkey[c / kChunkGroup] = std::min(gkey[c / kChunkGroup], key[c]);
+        for (int g = 0; g < ng; g++) gord[g] = g;
+        std::stable_sort(gord.begin(), gord.end(), [&](int a, int b) { return gkey[a] < gkey[b]; });
+        int k = 0;
+        for (int g : gord)
+            for (int c = g * kChunkGroup; c < std::min(nch, (g + 1) * kChunkGroup); c++) order[k++] = c;
+    }
     // XCD x runs workgroups b = x, x + 8, ...: give it a contiguous run of the sorted chunks
     int off[kXcds + 1] = {0};
     for (int x = 0; x < kXcds; x++) off[x + 1] = off[x] + (x < nch ? (nch - 1 - x) / kXcds + 1 : 0);
