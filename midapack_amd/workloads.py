@@ -20,6 +20,7 @@ same inputs are regenerated bit-for-bit on every host with this image.
 """
 from __future__ import annotations
 
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -353,3 +354,25 @@ def config_c6(seed=6):
     return make_templates_workload("C6: MTmap 8 CES x 8 det x 2^20 on 1024^2, 1312 templates per block", 8, 8,
                                    1 << 20, 1024, 1024, seed, npoly=1, ground=1, n_sss_bins=32, nhwp=1,
                                    delta_t=(1 << 20) / FSAMP / 128)
+
+
+def templates_irregular(wl, cut=1500, first=5):
+    """A template workload with every CES after the first cut to `cut` samples and
+    sweeps that start at sample `first` of each CES and include a one-sample
+    sweep (samples before the first stamp carry no polynomial template)."""
+    md = dict(wl.meta)
+    nces, ndet = md["nces"], md["ndet"]
+    L0 = int(wl.block_sizes[0])
+    lens = [L0 if c == 0 else cut for c in range(nces)]
+    keep = np.concatenate([np.arange(b * L0, b * L0 + lens[b // ndet]) for b in range(nces * ndet)])
+    pix = wl.pix.reshape(-1, 3)[keep].ravel().copy()
+    w = wl.weights.reshape(-1, 3)[keep].ravel().copy()
+    stamps = []
+    for c in range(nces):
+        st = [first] + [int(v) for v in md["sweeptstamps"][c][1:] if v < lens[c]]
+        st = sorted(set(st + [st[1] + 1 if len(st) > 1 else first + 1, lens[c]]))
+        stamps.append(np.array(st, np.int32))
+    md.update(sweeptstamps=stamps, az=[a[:n] for a, n in zip(md["az"], lens)],
+              hwp_angle=[h[:n] for h, n in zip(md["hwp_angle"], lens)])
+    return dataclasses.replace(wl, pix=pix, weights=w, signal=wl.signal[keep].copy(), noise=wl.noise[keep].copy(),
+                               block_sizes=np.repeat(np.array(lens, np.int32), ndet), meta=md)

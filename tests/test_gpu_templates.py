@@ -140,3 +140,31 @@ def test_mtmap_writes_the_reference_outputs(gpu, tmp_path):
     dI -= dI.mean()
     print(f"map I vs oracle (mean removed) {np.abs(dI).max():.2e}", flush=True)
     assert np.abs(dI).max() <= 1e-5 * np.abs(r.x[0::3]).max()
+
+
+def test_irregular_sweeps_and_ces_lengths(gpu):
+    """CES of different lengths, sweeps starting after the CES start and a
+    one-sample sweep (npoly 1: order 0 only)."""
+    iw = W.templates_irregular(W.config_templates_small(flag_frac=0.01))
+    print("case: irregular sweeps, CES lengths 2048 / 1500", flush=True)
+    tm = _otm(iw, npoly=1)
+    md = dict(iw.meta)
+    md["npoly"] = 1
+    with gpu.Problem(iw.pix, iw.weights, iw.block_sizes, 1, iw.invtt) as pb:
+        pb.set_templates(md)
+        ntot, nt, ranks = pb.templates_info()
+        assert ntot == tm.ntot and np.array_equal(nt, tm.nt)
+        rng = np.random.default_rng(4)
+        tau = rng.standard_normal(ntot)
+        assert np.array_equal(pb.templates_apply(gpu.MGPU_TMPL_T, tau), tm.tvec(tau))
+        B, _ = pb.templates_kernel()
+        assert np.array_equal(B, tm.kernel())
+        sig, _, _ = pb.precond_build(iw.signal)
+        x, hist, rep = pb.pcg(sig, iw.noise, tol=1e-10, maxiter=300)
+    r = O.gls_solve_tmpl(iw.pix, iw.weights, iw.signal, iw.noise, tm, iw.invtt, tol=1e-10, maxiter=300)
+    assert rep.converged and r.converged and abs(rep.iterations - r.iterations) <= 1
+    k = min(hist.size, r.res_hist.size)
+    assert np.abs(hist[:k] - r.res_hist[:k]).max() <= 1e-9
+    dx = x - r.x
+    dx[0::3] -= dx[0::3].mean()
+    assert np.abs(dx).max() <= 1e-6 * np.abs(r.x).max()

@@ -38,10 +38,10 @@ def dense_T(wl, npoly, ground, nsss, nhwp, delta_t):
     cols, blocks = [], []
     ntot = 0
     nces, ndet = md["nces"], md["ndet"]
-    L = int(wl.block_sizes[0])
     m = wl.m
     rows = []
     for c in range(nces):
+        L = int(wl.block_sizes[c * ndet])
         st = md["sweeptstamps"][c]
         ns = len(st) - 1
         nh = int(np.ceil(L / (delta_t * fs))) if nhwp else 0
@@ -50,9 +50,12 @@ def dense_T(wl, npoly, ground, nsss, nhwp, delta_t):
             Tb = np.zeros((L, npoly * ns + ground * nsss + 2 * nhwp * nh))
             for s in range(ns):
                 a, b = st[s], st[s + 1]
-                u = (2 * (np.arange(a, b) / fs) - a / fs - (b - 1) / fs) / ((b - 1) / fs - a / fs)
-                for o in range(npoly):
-                    Tb[a:b, o * ns + s] = np.polynomial.legendre.legval(u, [0] * o + [1])
+                if npoly:
+                    Tb[a:b, s] = 1.0  # P_0 (its argument is undefined on a one-sample sweep)
+                if npoly > 1:
+                    u = (2 * (np.arange(a, b) / fs) - a / fs - (b - 1) / fs) / ((b - 1) / fs - a / fs)
+                    for o in range(1, npoly):
+                        Tb[a:b, o * ns + s] = np.polynomial.legendre.legval(u, [0] * o + [1])
             k0 = npoly * ns
             if ground:
                 width = (md["az_max"][c] - md["az_min"][c]) / nsss
@@ -198,3 +201,21 @@ def test_template_solve_matches_dense_and_ignores_templates(wl):
     assert np.abs(V[:, keep].T @ (x - r.x)).max() <= 1e-7 * np.abs(V[:, keep].T @ r.x).max()
     print("iterations", r.iterations, "ranks", list(r.ranks), "of", list(tm.nt), "modes kept", keep.sum(),
           "of", keep.size)
+
+
+def test_irregular_sweeps_and_ces_lengths(wl):
+    iw = W.templates_irregular(wl)
+    md = iw.meta
+    npoly = 1  # order 0 only: a one-sample sweep has no Legendre argument (b == a)
+    tm = _tm(iw, npoly=npoly)
+    T, blocks = dense_T(iw, npoly, md["ground"], md["n_sss_bins"], md["nhwp"], md["delta_t"])
+    assert T[: md["sweeptstamps"][0][0], : md["sweeptstamps"][0].size - 1].sum() == 0
+    rng = np.random.default_rng(2)
+    tau = rng.standard_normal(tm.ntot)
+    d = rng.standard_normal(iw.m)
+    assert np.allclose(tm.tvec(tau), T @ tau, rtol=0, atol=1e-12 * np.abs(T @ tau).max())
+    ref = T.T @ d
+    assert np.allclose(tm.trtvec(d), ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+    for b, (Bb, Tb) in enumerate(zip(tm.blocks(tm.kernel()), blocks)):
+        want = iw.invtt[b] * Tb.T @ Tb
+        assert np.abs(Bb - want).max() <= 1e-12 * np.abs(want).max()
