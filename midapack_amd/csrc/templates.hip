@@ -35,9 +35,13 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <string>
+#include <thread>
 #include <vector>
 
 namespace mgpu {
@@ -357,16 +361,28 @@ struct CesTables {
 
 }  // namespace
 
-static int ces_tables(const mgpu_templates *t, int c, int len, double fsdt, CesTables &T) {
+// argument check inside a worker thread: the message goes to err (the
+// library's last-error slot is per thread), the caller reports it
+#define TARG(cond, ...)                                                          \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            char b_[256];                                                        \
+            std::snprintf(b_, sizeof(b_), __VA_ARGS__);                          \
+            err = b_;                                                            \
+            return MGPU_EINVAL;                                                  \
+        }                                                                        \
+    } while (0)
+
+static int ces_tables(const mgpu_templates *t, int c, int len, double fsdt, CesTables &T, std::string &err) {
     const int npoly = t->npoly, nsss = t->n_sss_bins, nh2 = 2 * t->nhwp;
     T.len = len;
     T.ns = t->nsweeps[c];
-    MGPU_ARG(T.ns >= 0 && t->sweeptstamps && t->sweeptstamps[c], "CES %d: invalid sweeps", c);
+    TARG(T.ns >= 0 && t->sweeptstamps && t->sweeptstamps[c], "CES %d: invalid sweeps", c);
     T.stamps.assign(t->sweeptstamps[c], t->sweeptstamps[c] + T.ns + 1);
-    MGPU_ARG(T.stamps[T.ns] == len, "CES %d: last sweep stamp %d differs from the CES length %d (mappraiser.c:577)",
+    TARG(T.stamps[T.ns] == len, "CES %d: last sweep stamp %d differs from the CES length %d (mappraiser.c:577)",
              c, T.stamps[T.ns], len);
-    MGPU_ARG(T.stamps[0] >= 0, "CES %d: negative sweep stamp", c);
-    for (int s = 0; s < T.ns; s++) MGPU_ARG(T.stamps[s + 1] >= T.stamps[s], "CES %d: decreasing sweep stamps", c);
+    TARG(T.stamps[0] >= 0, "CES %d: negative sweep stamp", c);
+    for (int s = 0; s < T.ns; s++) TARG(T.stamps[s + 1] >= T.stamps[s], "CES %d: decreasing sweep stamps", c);
     T.sweep.assign((size_t) len, -1);
     for (int s = 0; s < T.ns; s++)
         for (int i = T.stamps[s]; i < T.stamps[s + 1]; i++) T.sweep[i] = s;
@@ -383,23 +399,23 @@ static int ces_tables(const mgpu_templates *t, int c, int len, double fsdt, CesT
     }
     T.az.assign((size_t) len, 0);
     if (t->ground) {
-        MGPU_ARG(t->az && t->az[c] && t->az_min && t->az_max, "CES %d: azimuth missing", c);
+        TARG(t->az && t->az[c] && t->az_min && t->az_max, "CES %d: azimuth missing", c);
         const double lo = t->az_min[c], width = (t->az_max[c] - t->az_min[c]) / nsss;
         for (int i = 0; i < len; i++) {
             const int k = (int) floor((t->az[c][i] - lo) / width);  // bin_az, templates.c:746
             T.az[i] = std::min(nsss - 1, k);
-            MGPU_ARG(T.az[i] >= 0, "CES %d sample %d: azimuth below az_min", c, i);
+            TARG(T.az[i] >= 0, "CES %d sample %d: azimuth below az_min", c, i);
         }
     }
     T.nh = 0;
     if (nh2) {
-        MGPU_ARG(t->hwp_angle && t->hwp_angle[c], "CES %d: HWP angle missing", c);
+        TARG(t->hwp_angle && t->hwp_angle[c], "CES %d: HWP angle missing", c);
         T.nh = (int) ceil(len / (t->delta_t * t->sampling_freq));
         T.tbs.assign((size_t) T.nh + 1, len);
         int prev = -1;
         for (int i = 0; i < len; i++) {
             const int tb = (int) floor((double) i / fsdt);
-            MGPU_ARG(tb < T.nh, "CES %d: time bin %d beyond the %d HWP amplitudes", c, tb, T.nh);
+            TARG(tb < T.nh, "CES %d: time bin %d beyond the %d HWP amplitudes", c, tb, T.nh);
             for (int q = prev + 1; q <= tb; q++) T.tbs[q] = i;
             prev = tb;
         }
@@ -432,6 +448,7 @@ static int ces_tables(const mgpu_templates *t, int c, int len, double fsdt, CesT
     }
     return MGPU_OK;
 }
+#undef TARG
 
 // kernel entry tasks of one CES (entries shared by its detectors)
 static void ces_tasks(const mgpu_templates *t, const CesTables &T, std::vector<BTask> &tk) {
@@ -555,19 +572,30 @@ int templates_build(Problem &pb, const mgpu_templates *t) {
     M.fsdt = fsdt;
     M.nces = nces;
     M.ndet = ndet;
-    // per-CES tables (host), in parallel over CES
+    // per-CES tables (host), the CES in parallel
     std::vector<CesTables> ct((size_t) nces);
-    std::vector<int64_t> c0((size_t) nces + 1, 0);
+    std::vector<int> lens((size_t) nces);
     for (int c = 0; c < nces; c++) {
         const int64_t len = T.h_start[(size_t) c * ndet + 1] - T.h_start[(size_t) c * ndet];
         for (int j = 0; j < ndet; j++) {
             const int b = c * ndet + j;
             MGPU_ARG(T.h_start[b + 1] - T.h_start[b] == len, "block %d: length differs from its CES's", b);
         }
-        MGPU_RET(ces_tables(t, c, (int) len, fsdt, ct[c]));
-        c0[c + 1] = c0[c] + len;
+        lens[c] = (int) len;
     }
-    M.ncs = c0[nces];
+    {
+        std::vector<int> rcs((size_t) nces, MGPU_OK);
+        std::vector<std::string> errs((size_t) nces);
+        std::atomic<int> next{0};
+        const int nth = (int) std::max(1u, std::min({16u, std::thread::hardware_concurrency(), (unsigned) nces}));
+        std::vector<std::thread> th;
+        for (int k = 0; k < nth; k++)
+            th.emplace_back([&] {
+                for (int c; (c = next++) < nces;) rcs[c] = ces_tables(t, c, lens[c], fsdt, ct[c], errs[c]);
+            });
+        for (auto &x : th) x.join();
+        for (int c = 0; c < nces; c++) MGPU_ARG(rcs[c] == MGPU_OK, "%s", errs[c].c_str());
+    }
     // amplitude layout per block (Tlist_init)
     M.h_nt.assign((size_t) nb, 0);
     M.h_off.assign((size_t) nb + 1, 0);
@@ -580,24 +608,36 @@ int templates_build(Problem &pb, const mgpu_templates *t) {
     }
     M.ntot = M.h_off[nb];
     M.nB = M.h_boff[nb];
-    // device tables
+    // per-CES tables on the device with workgroup-aligned CES starts c0w[c], so
+    // that thread g of the sample kernels is CES sample g - c0w[c]; ces_of
+    // names the CES of every sample-kernel workgroup
+    std::vector<int> ces_of;
+    std::vector<int64_t> c0w((size_t) nces + 1, 0);
+    for (int c = 0; c < nces; c++) {
+        const int64_t nwg = (lens[c] + 255) / 256;
+        for (int64_t k = 0; k < nwg; k++) ces_of.push_back(c);
+        c0w[c + 1] = c0w[c] + nwg * 256;
+    }
+    M.ncs = c0w[nces];
     const int64_t ncs = M.ncs;
     const int nh2 = 2 * M.nhwp;
-    std::vector<int> h_ns(nces), h_nh(nces), sweep((size_t) ncs), az((size_t) ncs), perm((size_t) 3 * ncs);
-    std::vector<double> leg((size_t) M.npoly * ncs), hc((size_t) nh2 * ncs);
+    std::vector<int> h_ns(nces), h_nh(nces), sweep((size_t) ncs, -1), az((size_t) ncs, 0),
+            perm((size_t) 3 * ncs, 0);
+    std::vector<double> leg((size_t) M.npoly * ncs, 0.0), hc((size_t) nh2 * ncs, 0.0);
     for (int c = 0; c < nces; c++) {
         const CesTables &C = ct[c];
+        const int64_t o = c0w[c];
         h_ns[c] = C.ns;
         h_nh[c] = C.nh;
-        std::copy(C.sweep.begin(), C.sweep.end(), sweep.begin() + c0[c]);
-        std::copy(C.az.begin(), C.az.end(), az.begin() + c0[c]);
-        for (int k = 0; k < 3; k++) std::copy(C.perm[k].begin(), C.perm[k].end(), perm.begin() + k * ncs + c0[c]);
-        for (int o = 0; o < M.npoly; o++)
-            std::copy(C.leg.begin() + (size_t) o * C.len, C.leg.begin() + (size_t) (o + 1) * C.len,
-                      leg.begin() + o * ncs + c0[c]);
-        for (int h = 0; h < nh2; h++)
-            std::copy(C.hc.begin() + (size_t) h * C.len, C.hc.begin() + (size_t) (h + 1) * C.len,
-                      hc.begin() + h * ncs + c0[c]);
+        std::copy(C.sweep.begin(), C.sweep.end(), sweep.begin() + o);
+        std::copy(C.az.begin(), C.az.end(), az.begin() + o);
+        for (int k = 0; k < 3; k++) std::copy(C.perm[k].begin(), C.perm[k].end(), perm.begin() + k * ncs + o);
+        for (int r = 0; r < M.npoly; r++)
+            std::copy(C.leg.begin() + (size_t) r * C.len, C.leg.begin() + (size_t) (r + 1) * C.len,
+                      leg.begin() + r * ncs + o);
+        for (int r = 0; r < nh2; r++)
+            std::copy(C.hc.begin() + (size_t) r * C.len, C.hc.begin() + (size_t) (r + 1) * C.len,
+                      hc.begin() + r * ncs + o);
     }
     std::vector<int> row_block((size_t) std::max<int64_t>(M.ntot, 1), 0);
     for (int b = 0; b < nb; b++)
@@ -624,47 +664,6 @@ int templates_build(Problem &pb, const mgpu_templates *t) {
                 items.push_back(TItem{b, 2, C.tbs[q], C.tbs[q + 1], (int64_t) M.npoly * C.ns + M.nsss + q, C.nh, 0});
     }
     M.nitems = (int) items.size();
-    // CES of every k_tmpl_sub workgroup (a workgroup never straddles two CES)
-    std::vector<int> ces_of;
-    std::vector<int64_t> c0w;  // per-CES start rounded up to whole workgroups
-    {
-        int64_t g = 0;
-        c0w.assign((size_t) nces + 1, 0);
-        for (int c = 0; c < nces; c++) {
-            c0w[c] = g;
-            const int64_t nwg = (ct[c].len + 255) / 256;
-            for (int64_t k = 0; k < nwg; k++) ces_of.push_back(c);
-            g += nwg * 256;
-        }
-        c0w[nces] = g;
-    }
-    // the per-CES tables are stored with workgroup-aligned CES starts, so that
-    // thread g of the sample kernels is CES sample g - c0[c]
-    {
-        const int64_t ncw = c0w[nces];
-        auto realign_i = [&](std::vector<int> &v, int rows) {
-            std::vector<int> w((size_t) rows * ncw, -1);
-            for (int r = 0; r < rows; r++)
-                for (int c = 0; c < nces; c++)
-                    std::copy(v.begin() + r * ncs + c0[c], v.begin() + r * ncs + c0[c + 1], w.begin() + r * ncw + c0w[c]);
-            v.swap(w);
-        };
-        auto realign_d = [&](std::vector<double> &v, int rows) {
-            std::vector<double> w((size_t) rows * ncw, 0.0);
-            for (int r = 0; r < rows; r++)
-                for (int c = 0; c < nces; c++)
-                    std::copy(v.begin() + r * ncs + c0[c], v.begin() + r * ncs + c0[c + 1], w.begin() + r * ncw + c0w[c]);
-            v.swap(w);
-        };
-        realign_i(sweep, 1);
-        realign_i(az, 1);
-        realign_i(perm, 3);
-        realign_d(leg, M.npoly);
-        realign_d(hc, nh2);
-        M.ncs = ncw;
-    }
-    std::vector<int> lens((size_t) nces);
-    for (int c = 0; c < nces; c++) lens[c] = ct[c].len;
     MGPU_RET(pb.arena.alloc(&M.d_ns, (size_t) nces));
     MGPU_RET(pb.arena.alloc(&M.d_nh, (size_t) nces));
     MGPU_RET(pb.arena.alloc(&M.d_len, (size_t) nces));
@@ -720,35 +719,44 @@ int templates_build(Problem &pb, const mgpu_templates *t) {
         MGPU_HIP(hipStreamSynchronize(s));
         pb.arena.release(d_tk);
     }
-    // pseudo-inverses, one batched eigensolve per CES (its blocks share nt)
+    // pseudo-inverses: one batched eigensolve per run of consecutive CES with
+    // the same template count (their blocks are contiguous nt x nt matrices),
+    // each run's working copy kept under kEigBatchBytes
     M.h_rank.assign((size_t) nb, 0);
     if (M.nB > 0) {
+        constexpr size_t kEigBatchBytes = size_t(4) << 30;
         RocblasHandle rh;
         MGPU_ROCBLAS(rocblas_create_handle(&rh.h));
         MGPU_ROCBLAS(rocblas_set_stream(rh.h, s));
         int *d_rank = nullptr, *d_info = nullptr;
         MGPU_RET(pb.arena.alloc(&d_rank, (size_t) nb));
-        MGPU_RET(pb.arena.alloc(&d_info, (size_t) ndet));
-        for (int c = 0; c < nces; c++) {
+        MGPU_RET(pb.arena.alloc(&d_info, (size_t) nb));
+        for (int c = 0; c < nces;) {
             const int b0 = c * ndet;
             const int n = M.h_nt[b0];
+            const size_t per_ces = (size_t) ndet * n * n * sizeof(double);
+            int e = c + 1;
+            while (e < nces && M.h_nt[(size_t) e * ndet] == n && (size_t) (e + 1 - c) * per_ces <= kEigBatchBytes) e++;
+            const int batch = (e - c) * ndet;
+            c = e;
             if (n == 0) continue;
             double *V = nullptr, *D = nullptr, *E = nullptr;
-            MGPU_RET(pb.arena.alloc(&V, (size_t) ndet * n * n));
-            MGPU_RET(pb.arena.alloc(&D, (size_t) ndet * n));
-            MGPU_RET(pb.arena.alloc(&E, (size_t) ndet * n));
-            MGPU_HIP(hipMemcpyAsync(V, M.d_B + M.h_boff[b0], sizeof(double) * ndet * n * n, hipMemcpyDeviceToDevice, s));
+            MGPU_RET(pb.arena.alloc(&V, (size_t) batch * n * n));
+            MGPU_RET(pb.arena.alloc(&D, (size_t) batch * n));
+            MGPU_RET(pb.arena.alloc(&E, (size_t) batch * n));
+            MGPU_HIP(hipMemcpyAsync(V, M.d_B + M.h_boff[b0], sizeof(double) * batch * n * n, hipMemcpyDeviceToDevice,
+                                    s));
             MGPU_ROCBLAS(rocsolver_dsyevd_strided_batched(rh.h, rocblas_evect_original, rocblas_fill_upper, n, V, n,
-                                                          (rocblas_stride) n * n, D, n, E, n, d_info, ndet));
-            std::vector<int> info((size_t) ndet);
-            MGPU_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * ndet, hipMemcpyDeviceToHost, s));
+                                                          (rocblas_stride) n * n, D, n, E, n, d_info, batch));
+            std::vector<int> info((size_t) batch);
+            MGPU_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
             MGPU_HIP(hipStreamSynchronize(s));
-            for (int j = 0; j < ndet; j++)
+            for (int j = 0; j < batch; j++)
                 MGPU_ARG(info[j] == 0, "eigensolve of template kernel %d failed (info %d)", b0 + j, info[j]);
-            k_eig_inv<<<ndet, 256, 0, s>>>(D, n, d_rank + b0);
+            k_eig_inv<<<batch, 256, 0, s>>>(D, n, d_rank + b0);
             MGPU_LAUNCHED();
             const unsigned tiles = (unsigned) ((n + kPT - 1) / kPT);
-            k_pinv_form<<<dim3(tiles, tiles, ndet), kPT * kPT, 0, s>>>(V, D, n, M.d_Binv + M.h_boff[b0]);
+            k_pinv_form<<<dim3(tiles, tiles, batch), kPT * kPT, 0, s>>>(V, D, n, M.d_Binv + M.h_boff[b0]);
             MGPU_LAUNCHED();
             MGPU_HIP(hipStreamSynchronize(s));
             pb.arena.release(V);
