@@ -810,12 +810,10 @@ static int apply_tm(Problem &pb, const double *d_q, const double *in_scale, doub
     hipStream_t s = pb.stream;
     pb.prof.begin(MGPU_KT_TMPL, s);
     MGPU_RET(launch_tr(pb, d_q, in_scale, M.d_a1));
-    if (M.ntot > 0) {
+    if (M.ntot > 0) {  // (with no template class k_tmpl_sub reads no amplitude)
         k_tmpl_binv<<<grid_for(M.ntot, 4), 256, 0, s>>>(M.d_Binv, M.d_boff, M.d_off, M.d_row_block, M.ntot, M.d_a1,
                                                         M.d_a2, pb.live_state);
         MGPU_LAUNCHED();
-    } else {
-        MGPU_HIP(hipMemsetAsync(M.d_a2, 0, sizeof(double), s));
     }
     if (d_q == d_y)
         k_tmpl_sub<true><<<M.nwg_samples, 256, 0, s>>>(nullptr, in_scale, d_y, pb.tp.d_start, pb.tp.d_T0, M.ndet,

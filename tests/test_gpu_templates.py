@@ -168,3 +168,18 @@ def test_irregular_sweeps_and_ces_lengths(gpu):
     dx = x - r.x
     dx[0::3] -= dx[0::3].mean()
     assert np.abs(dx).max() <= 1e-6 * np.abs(r.x).max()
+
+
+def test_template_pcg_graph_replay_is_bit_identical(gpu):
+    """The template kernels inside HIP-graph-captured iteration blocks give the
+    eager solve's iterates bit for bit."""
+    wl = W.config_templates_small(flag_frac=0.01)
+    print("case: template PCG eager vs graph", flush=True)
+    out = []
+    for graph in (0, 1):
+        with gpu.Problem(wl.pix, wl.weights, wl.block_sizes, 1, wl.invtt, opts=gpu.options(graph=graph)) as pb:
+            pb.set_templates(wl.meta)
+            sig, _, _ = pb.precond_build(wl.signal)
+            out.append(pb.pcg(sig, wl.noise, tol=1e-10, maxiter=300))
+    (x0, h0, r0), (x1, h1, r1) = out
+    assert r0.iterations == r1.iterations and np.array_equal(x0, x1) and np.array_equal(h0, h1)
