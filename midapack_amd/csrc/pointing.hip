@@ -710,26 +710,28 @@ __global__ __launch_bounds__(kPtBlock) void k_trmatvec(const int64_t *__restrict
     if (qq < qend) {
         // The slice width is uniform over the wave: every lane walks all of it,
         // its padding entries (t = -1, weights 0) adding +0.0, which leaves the
-        // sums unchanged (they start at +0.0 and so are never -0.0).  Four hits
-        // per step keep eight independent loads in flight per lane.
+        // sums unchanged (they start at +0.0 and so are never -0.0).  Sixteen
+        // hits per step keep 32 independent loads in flight per lane (on the
+        // 64 x 2^20 raster: 4 hits 0.395 ms, 8 hits 0.386 ms, 16 hits 0.380 ms).
         const int64_t b0 = base[s];
         const int width = (int) ((base[s + 1] - b0) / kSlice);
         const int *__restrict__ cte = ct + b0 + lane;
         const TW2 *__restrict__ cwe = cw12 + b0 + lane;
         int i = 0;
-        for (; i + 4 <= width; i += 4) {
-            int t[4];
-            TW2 w[4];
-            double yt[4];
+        constexpr int U = 16;
+        for (; i + U <= width; i += U) {
+            int t[U];
+            TW2 w[U];
+            double yt[U];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < U; u++) {
                 t[u] = cte[(i + u) * kSlice];
                 w[u] = cwe[(i + u) * kSlice];
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) yt[u] = t[u] >= 0 ? (double) y[t[u]] : 0.0;
+            for (int u = 0; u < U; u++) yt[u] = t[u] >= 0 ? (double) y[t[u]] : 0.0;
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < U; u++) {
                 a0 += (W0 ? (double) cw0[b0 + lane + (int64_t) (i + u) * kSlice] * yt[u] : yt[u]);
                 a1 += (double) w[u].x * yt[u];
                 a2 += (double) w[u].y * yt[u];
