@@ -37,3 +37,30 @@ def test_host_only_entry_points():
     o = G.options()
     assert o.precision == G.MGPU_FP64 and o.refresh_every == 10 and o.poll_every == 8
     assert isinstance(G.last_error(), str)
+
+
+def _params(name):
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    m = re.search(rf"\bvoid {name}\((.*?)\);", text, flags=re.S)
+    assert m, name
+    return [re.sub(r"\s+", " ", p.strip()) for p in m.group(1).split(",")]
+
+
+def test_reference_entry_point_signatures():
+    """MLmap / MTmap take the reference's parameters in its order (map.h:10-18;
+    MPI_Comm arrives as one integer-sized handle), so its ctypes argtypes
+    (mappraiser_wrapper/__init__.py:58-122) line up one to one."""
+    ml = [p.rsplit(" ", 1)[-1].lstrip("*") for p in _params("MLmap")]
+    assert ml == ["comm", "outpath", "ref", "solver", "precond", "Z_2lvl", "pointing_commflag", "tol", "maxiter",
+                  "enlFac", "ortho_alg", "bs_red", "nside", "data_size_proc", "nb_blocks_loc", "local_blocks_sizes",
+                  "Nnz", "pix", "pixweights", "signal", "noise", "lambda", "invtt"]
+    mt = _params("MTmap")
+    names = [p.rsplit(" ", 1)[-1].lstrip("*") for p in mt]
+    assert names == ["comm", "outpath", "ref", "solver", "precond", "Z_2lvl", "pointing_commflag", "npoly", "nhwp",
+                     "delta_t", "ground", "n_sss_bins", "tol", "maxiter", "enlFac", "ortho_alg", "bs_red", "nside",
+                     "sweeptstamps", "nsweeps", "az", "az_min", "az_max", "hwp_angle", "nces", "data_size_proc",
+                     "nb_blocks_loc", "local_blocks_sizes", "Nnz", "pix", "pixweights", "signal", "noise",
+                     "sampling_freq", "invtt"]
+    assert mt[18].startswith("int **") and mt[20].startswith("double **") and mt[23].startswith("double **")
+    L = G.load()
+    assert len(L.MLmap.argtypes) == len(ml) and len(L.MTmap.argtypes) == len(names)
