@@ -57,8 +57,9 @@ typedef struct mgpu_options {
                            profile = 1) */
     int precond;        /* PCG_GLS_true's precond (pcg_true.c:52, build_precond precond.c:1310-1398):
                            0 block Jacobi, 1 two-level a priori (build_Z), 2 two-level
-                           a posteriori (Lanczos_eig); two-level needs lambda > 1 and one
-                           problem per solve (not the batch / server entry points) */
+                           a posteriori (Lanczos_eig); two-level solves one problem per
+                           call (not the batch / server entry points); with lambda = 1 its
+                           products remove the block means as stbmmProd does */
     int z_2lvl;         /* Z_2lvl: deflation vectors of the two-level preconditioners
                            (0 -> 1, the one-rank communicator size, pcg_true.c:50; at most 256) */
 } mgpu_options;
