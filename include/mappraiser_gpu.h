@@ -47,11 +47,15 @@ typedef struct mgpu_options {
     int poll_every;     /* PCG: iterations launched between host convergence polls (default 8) */
     int profile;        /* 1: time every kernel class with HIP events on the handle's stream */
     int offset_mode;    /* per-block offset handling of N^-1 (MGPU_OFFSET_*), default REFERENCE */
-    int tile_variant;   /* Toeplitz tile kernel variant: 0 = autotuned (each variant of the tile
-                           length is timed on the first product, once per process, device,
-                           precision and tile length); k > 0 forces variant k (1 = the default
-                           shape) of the precision of the PCG iterations (FP32 under
-                           MGPU_MIXED); see mappraiser_gpu_tile_variants */
+    int tile_variant;   /* Toeplitz tile kernel variant of the precision of the PCG iterations
+                           (FP32 under MGPU_MIXED): 0 = the fixed default of the tile length
+                           (results depend on the inputs only), k > 0 forces variant k,
+                           MGPU_TILE_AUTOTUNE times every variant on the first product (once
+                           per process, device, precision and tile length) and keeps the
+                           fastest; variants round differently, so autotuned FP64 results
+                           may differ between processes in the last bits.  FP64 products of
+                           a mixed solve always use the FP64 default.  See
+                           mappraiser_gpu_tile_variants */
     int graph;          /* 1: capture each block of PCG iterations once into a HIP graph and
                            replay it (same kernels, bit-identical results; ignored when
                            profile = 1) */
@@ -72,6 +76,8 @@ typedef struct mgpu_options {
  *                          projection; equals T C except within lambda of block edges.
  *   MGPU_OFFSET_NONE       N^-1 = T (no offset removal).
  * lambda == 1 (white noise) never removes offsets, as in apply_weights (pcg_true.c:583-593). */
+#define MGPU_TILE_AUTOTUNE (-1)  /* mgpu_options.tile_variant: opt into timing-based variant choice */
+
 #define MGPU_OFFSET_REFERENCE 0
 #define MGPU_OFFSET_SYMMETRIC 1
 #define MGPU_OFFSET_NONE 2
@@ -95,7 +101,8 @@ typedef struct mgpu_report {
     int    lcount;      /* local index count incl. trash entries (Mat.lcount after the rebuild) */
     int    trash_pix;   /* Mat.trash_pix after the rebuild */
     int    n_cut;       /* pixels removed by the rcond <= 0.1 cut (precond.c:560) */
-    double t_setup_ms;  /* device time: pointing build + preconditioner */
+    double t_setup_ms;  /* device time: pointing build + Toeplitz plan (problem constructor) and
+                           the preconditioner build with its cut (uploads excluded) */
     double t_solve_ms;  /* device time: PCG iterations */
     double true_res_rel;/* ||b - A x||/||b - A x0|| recomputed in FP64 at exit (MGPU_MIXED), else -1 */
     int    refreshes;   /* FP64 residual recomputations performed (MGPU_MIXED) */

@@ -114,9 +114,35 @@ int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt) {
     mappraiser_gpu_options_default(&pb->opt);
     if (opt) pb->opt = *opt;
     pb->prof.on = pb->opt.profile != 0;
+    MGPU_HIP(hipEventCreate(&pb->ev0));
+    MGPU_HIP(hipEventCreate(&pb->ev1));
     MGPU_ARG(pb->opt.precision == MGPU_FP64 || pb->opt.precision == MGPU_MIXED, "unknown precision %d",
              pb->opt.precision);
     return MGPU_OK;
+}
+
+static int elapsed_ms(Problem &pb, double *ms) {
+    MGPU_HIP(hipEventSynchronize(pb.ev1));
+    float t = 0.f;
+    MGPU_HIP(hipEventElapsedTime(&t, pb.ev0, pb.ev1));
+    *ms = t;
+    return MGPU_OK;
+}
+
+int problem_setup(Problem &pb, const int *d_pix, const double *d_w, int64_t m, int nnz, const int *block_sizes,
+                  int nb, int lambda, const double *d_invtt, const double *h_invtt) {
+    MGPU_HIP(hipEventRecord(pb.ev0, pb.stream));
+    MGPU_RET(pointing_build(pb, d_pix, d_w, m, nnz));
+    MGPU_RET(toeplitz_plan(pb, block_sizes, nb, lambda, d_invtt, h_invtt));
+    MGPU_HIP(hipEventRecord(pb.ev1, pb.stream));
+    return elapsed_ms(pb, &pb.t_create_ms);
+}
+
+int precond_build_timed(Problem &pb, double *d_signal) {
+    MGPU_HIP(hipEventRecord(pb.ev0, pb.stream));
+    MGPU_RET(precond_build(pb, d_signal));
+    MGPU_HIP(hipEventRecord(pb.ev1, pb.stream));
+    return elapsed_ms(pb, &pb.t_precond_ms);
 }
 
 int check_common(int m, int nnz, int nb, const int *bs, int lambda) {
@@ -172,10 +198,9 @@ int mappraiser_gpu_problem_create(mgpu_problem **out, int m, int nnz, const int 
     double *d_w;
     MGPU_RET(upload(*pb, &d_pix, pix, (size_t) m * nnz));
     MGPU_RET(upload(*pb, &d_w, pixweights, (size_t) m * nnz));
-    MGPU_RET(pointing_build(*pb, d_pix, d_w, m, nnz));
+    MGPU_RET(problem_setup(*pb, d_pix, d_w, m, nnz, local_blocks_sizes, nb, lambda, nullptr, invtt));
     pb->arena.release(d_pix);
     pb->arena.release(d_w);
-    MGPU_RET(toeplitz_plan(*pb, local_blocks_sizes, nb, lambda, nullptr, invtt));
     *out = reinterpret_cast<mgpu_problem *>(pb.release());
     return MGPU_OK;
 }
@@ -188,8 +213,7 @@ int mappraiser_gpu_problem_create_device(mgpu_problem **out, int m, int nnz, con
     MGPU_RET(check_common(m, nnz, nb, local_blocks_sizes, lambda));
     std::unique_ptr<Problem> pb;
     MGPU_RET(new_problem(pb, opt));
-    MGPU_RET(pointing_build(*pb, d_pix, d_pixweights, m, nnz));
-    MGPU_RET(toeplitz_plan(*pb, local_blocks_sizes, nb, lambda, d_invtt, nullptr));
+    MGPU_RET(problem_setup(*pb, d_pix, d_pixweights, m, nnz, local_blocks_sizes, nb, lambda, d_invtt, nullptr));
     *out = reinterpret_cast<mgpu_problem *>(pb.release());
     return MGPU_OK;
 }
@@ -275,7 +299,7 @@ int mappraiser_gpu_precond_build(mgpu_problem *h, double *signal, double *cond, 
         d_sig = pb.d_sig;
         MGPU_HIP(hipMemcpyAsync(d_sig, signal, sizeof(double) * pb.pt.m, hipMemcpyHostToDevice, pb.stream));
     }
-    MGPU_RET(precond_build(pb, d_sig));
+    MGPU_RET(precond_build_timed(pb, d_sig));
     const int np = pb.pt.npix;
     if (signal)
         MGPU_HIP(hipMemcpyAsync(signal, d_sig, sizeof(double) * pb.pt.m, hipMemcpyDeviceToHost, pb.stream));
@@ -322,15 +346,9 @@ int mappraiser_gpu_solve_device(mgpu_problem *h, double *d_signal, const double 
     Problem &pb = *PB(h);
     MGPU_HIP(hipSetDevice(pb.device));
     MGPU_RET(ensure_work(pb));
-    MGPU_HIP(hipEventRecord(pb.ev0, pb.stream));
-    MGPU_RET(precond_build(pb, d_signal));
-    MGPU_HIP(hipEventRecord(pb.ev1, pb.stream));
-    MGPU_HIP(hipEventSynchronize(pb.ev1));
-    float setup_ms = 0.f;
-    MGPU_HIP(hipEventElapsedTime(&setup_ms, pb.ev0, pb.ev1));
+    MGPU_RET(precond_build_timed(pb, d_signal));
     mgpu_report r{};
     MGPU_RET(pcg_run(pb, d_signal, d_noise, tol, maxiter, 0, x, res_hist, &r));
-    r.t_setup_ms = setup_ms;
     const int np = pb.pt.npix;
     if (cond) MGPU_HIP(hipMemcpyAsync(cond, pb.pc.d_cond, sizeof(double) * np, hipMemcpyDeviceToHost, pb.stream));
     if (lhits) MGPU_HIP(hipMemcpyAsync(lhits, pb.pc.d_hits, sizeof(int) * np, hipMemcpyDeviceToHost, pb.stream));
@@ -422,7 +440,7 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
     MGPU_RET(pb.arena.alloc(&pb.d_noise, (size_t) m));
     MGPU_HIP(hipMemcpyAsync(pb.d_sig, signal, sizeof(double) * m, hipMemcpyHostToDevice, s));
     MGPU_HIP(hipMemcpyAsync(pb.d_noise, noise, sizeof(double) * m, hipMemcpyHostToDevice, s));
-    MGPU_RET(precond_build(pb, pb.d_sig));
+    MGPU_RET(precond_build_timed(pb, pb.d_sig));
     const int np = pb.pt.npix;
     MGPU_HIP(hipMemcpyAsync(signal, pb.d_sig, sizeof(double) * m, hipMemcpyDeviceToHost, s));
     if (cond) MGPU_HIP(hipMemcpyAsync(cond, pb.pc.d_cond, sizeof(double) * np, hipMemcpyDeviceToHost, s));

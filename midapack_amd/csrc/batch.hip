@@ -98,7 +98,6 @@ int solve_group(Group &gr, mgpu_batch_item *items, int grid_npix, bool on_dev, d
     EventPair ev;
     MGPU_HIP(hipEventCreate(&ev.a));
     MGPU_HIP(hipEventCreate(&ev.b));
-    MGPU_HIP(hipEventRecord(ev.a, s));
     int *d_pix, *d_err;
     double *d_w, *d_inv;
     MGPU_RET(P.arena.alloc(&d_pix, (size_t) (3 * M)));
@@ -139,6 +138,9 @@ int solve_group(Group &gr, mgpu_batch_item *items, int grid_npix, bool on_dev, d
     MGPU_HIP(hipStreamSynchronize(s));
     MGPU_ARG(!herr, "a pixel index is >= grid_npix (%d)", grid_npix);
 
+    // set-up time: the span of a solo solve's t_setup_ms (pointing build, Toeplitz
+    // plan, preconditioner), without the input uploads
+    MGPU_HIP(hipEventRecord(ev.a, s));
     MGPU_RET(pointing_build(P, d_pix, d_w, M, 3));
     P.arena.release(d_pix);
     P.arena.release(d_w);

@@ -180,7 +180,7 @@ struct ToeplitzPlan {
     // FP32 products (MGPU_MIXED) with a longer tile than the FP64 one, when that
     // pays off (less overlap per useful sample); log2n32 == 0: FP32 uses the FP64 tiles
     int log2n32 = 0, L32 = 0, ntiles32 = 0;
-    bool fp32_long = true;           // FP32 uses the long tiles (set by the autotuner)
+    bool fp32_long = false;          // FP32 uses the long tiles (only when the autotuner picks them)
     TileDesc *d_tiles32 = nullptr;
     float *d_coef32L = nullptr;      // nfilters * (N32/2) (a_k, b_k)
     float *d_tw32L = nullptr;
@@ -427,6 +427,9 @@ struct Problem {
     int *d_flag = nullptr;                        // PCG stop flag
     int64_t nred = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device time of the set-up steps (mgpu_report.t_setup_ms = their sum):
+    // pointing build + Toeplitz plan of the constructor, preconditioner build
+    double t_create_ms = 0.0, t_precond_ms = 0.0;
     ~Problem();
 };
 
@@ -458,12 +461,26 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
 // compute_partials) -> d_out (may not alias d_in)
 int toeplitz_apply(Problem &pb, const double *d_in, double *d_out, bool compute_partials);
 int toeplitz_apply_f32(Problem &pb, const float *d_in, float *d_out, bool compute_partials);
+// scratch of one N^-1 product: partial sums [nchunks + nb], means [nb], slice
+// sums of the symmetric mode's output mean [nb * 8]
+struct ToeplitzScratch {
+    double *partials, *mean, *edge_part;
+};
+size_t toeplitz_scratch_doubles(const ToeplitzPlan &T);
+// N^-1 (FP64, partial sums computed) with the scratch laid out in d_scratch
+// (toeplitz_scratch_doubles(T) doubles) instead of the plan's buffers
+int toeplitz_apply_scratch(Problem &pb, const double *d_in, double *d_out, double *d_scratch);
 int tile_variant_count(bool fp32, int log2n);
 // real tile length 2^k the plan of a lambda / longest block would use (0: no tiles)
 int toeplitz_tile_log2(int lambda, int64_t nmax, int forced);
 const char *tile_variant_desc(bool fp32, int log2n, int variant);
 
 int precond_build(Problem &pb, double *d_signal);
+// precond_build timed with the problem's events (t_precond_ms)
+int precond_build_timed(Problem &pb, double *d_signal);
+// pointing build + Toeplitz plan of a new problem, timed (t_create_ms)
+int problem_setup(Problem &pb, const int *d_pix, const double *d_w, int64_t m, int nnz, const int *block_sizes,
+                  int nb, int lambda, const double *d_invtt, const double *h_invtt);
 // A v = P^T N^-1 P v in FP64 (map -> map), optional fused <v, Av> partials
 int apply_A64(Problem &pb, const double *d_v, double *d_out, double *d_dot);
 // two-level preconditioner: build Z, A Z, pinv(E) for opt.precond (1 a priori,

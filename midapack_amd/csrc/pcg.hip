@@ -197,10 +197,13 @@ __global__ __launch_bounds__(kMB) void k_update_xg(double *__restrict__ x, const
 }
 
 // dots -> g2pix, polak, res; history; stop test; gamma.  One workgroup per segment.
+// verify = 1 (mixed precision, g updated in FP32 this iteration): a passing stop
+// test does not stop the segment; it raises F_NPEND, and the host queues an
+// iteration with an FP64 refresh, whose test (verify = 0) decides.
 __global__ __launch_bounds__(kRB) void k_scalars(const double *__restrict__ part, int nwg,
                                                  const int *__restrict__ wg0, double *__restrict__ S,
                                                  double *__restrict__ hist, int maxiter, int *__restrict__ state,
-                                                 int nseg, int *__restrict__ flag) {
+                                                 int nseg, int *__restrict__ flag, int verify) {
     if (stopped(flag)) return;
     const int r = blockIdx.x;
     if (state[r]) return;
@@ -223,10 +226,11 @@ __global__ __launch_bounds__(kRB) void k_scalars(const double *__restrict__ part
         if (hist && ki < maxiter) hist[(int64_t) r * maxiter + ki] = sqrt(res) / sqrt(Sr[S_RES0]);
         atomicMax(&flag[F_KMAX], ki);
         int st = 0;
-        if (res <= Sr[S_TOL2])
+        if (res <= Sr[S_TOL2] && !verify)
             st = 1;
         else if ((Sr[S_STOPK] > 0.0 && k >= Sr[S_STOPK]) || k + 1.0 >= Sr[S_MAXK])
             st = 2;
+        if (res <= Sr[S_TOL2] && verify) flag[F_NPEND] = 1;
         if (st) {
             state[r] = st;
             if (r == 0 && st == 1) flag[F_CONV0] = 1;
@@ -234,6 +238,11 @@ __global__ __launch_bounds__(kRB) void k_scalars(const double *__restrict__ part
         }
         Sr[S_GAMMA] = polak / Sr[S_G2PIXP];
     }
+}
+
+// clear one device flag (a kernel, not a memset, so that it may be captured)
+__global__ void k_clear_flag(int *__restrict__ flag, int which) {
+    if (threadIdx.x == 0) flag[which] = 0;
 }
 
 // h = h gamma + Cg
@@ -256,8 +265,10 @@ __global__ __launch_bounds__(kMB) void k_hupdate(double *__restrict__ h, const d
 __global__ __launch_bounds__(kMB) void k_refresh(const double *__restrict__ b, const double *__restrict__ Ax,
                                                  double *__restrict__ g, const int *__restrict__ wg_seg,
                                                  const int *__restrict__ wg_q0, const int *__restrict__ seg_q,
-                                                 const int *__restrict__ state, const int *__restrict__ flag) {
+                                                 const int *__restrict__ state, const int *__restrict__ flag,
+                                                 int *__restrict__ nref) {
     if (stopped(flag)) return;
+    if (nref && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(nref, 1);
     const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
     if ((state && state[sp.r]) || sp.q >= sp.qend) return;
     for (int c = 0; c < 3; c++) {
@@ -470,11 +481,9 @@ int ensure_work(Problem &pb) {
         pb.nscal = nscal;
         MGPU_RET(pb.arena.alloc(&pb.d_scal, (size_t) nscal));
     }
-    if (!pb.d_flag) {
-        MGPU_RET(pb.arena.alloc(&pb.d_flag, (size_t) F_NFLAGS));
-        MGPU_HIP(hipEventCreate(&pb.ev0));
-        MGPU_HIP(hipEventCreate(&pb.ev1));
-    }
+    if (!pb.d_flag) MGPU_RET(pb.arena.alloc(&pb.d_flag, (size_t) F_NFLAGS));
+    if (!pb.ev0) MGPU_HIP(hipEventCreate(&pb.ev0));
+    if (!pb.ev1) MGPU_HIP(hipEventCreate(&pb.ev1));
     if (pb.opt.precision == MGPU_MIXED && !pb.d_todf) {
         MGPU_RET(pb.arena.alloc(&pb.d_todf, (size_t) m));
         MGPU_RET(pb.arena.alloc(&pb.d_tod2f, (size_t) m));
@@ -619,9 +628,16 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     double *g = gA, *gp = gB;
     GraphBlock gb;
 
-    auto enqueue_iteration = [&]() -> int {
+    // force_refresh: refresh the residual in FP64 in this iteration whatever its
+    // number (mixed precision: an FP32-updated residual met the stop test)
+    auto enqueue_iteration = [&](bool force_refresh) -> int {
         ++launched;
         pb.prof.tag = launched;
+        const bool do_refresh = refresh && (force_refresh || launched % refresh == 0);
+        if (force_refresh) {
+            k_clear_flag<<<1, 64, 0, s>>>(flag, F_NPEND);
+            MGPU_LAUNCHED();
+        }
         std::swap(g, gp);  // pcg_true.c:140-142
         // q = A h with fused <h, q>
         if (mixed)
@@ -640,13 +656,13 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             MGPU_LAUNCHED();
         }
         pb.prof.end(s);
-        if (refresh && launched % refresh == 0) {
+        if (do_refresh) {
             // g = b - A x in FP64, then Cg = M g and the three dots on the refreshed residual
             pb.prof.begin(MGPU_KT_REFRESH, s);
             const bool keep = pb.prof.on;
             pb.prof.on = false;  // the FP64 operator inside is accounted as a refresh
             MGPU_RET(apply_A64(pb, x, Ax, nullptr));
-            k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, g, wseg, wq0, sq, state, flag);
+            k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, g, wseg, wq0, sq, state, flag, flag + F_NREF);
             MGPU_LAUNCHED();
             if (two) {
                 MGPU_RET(twolvl_apply(pb, g, gp, Cg, nullptr, part, false, state, flag));
@@ -658,7 +674,8 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             pb.prof.end(s);
         }
         pb.prof.begin(MGPU_KT_MAP, s);
-        k_scalars<<<nseg, kRB, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag);
+        k_scalars<<<nseg, kRB, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag,
+                                       mixed && !do_refresh ? 1 : 0);
         MGPU_LAUNCHED();
         k_hupdate<<<nwg, kMB, 0, s>>>(h, Cg, wseg, wq0, sq, S, state, flag);
         MGPU_LAUNCHED();
@@ -667,15 +684,20 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     };
 
     bool done = maxiter <= 1;  // maxiter == 1: no loop iteration (for k = 1; k < K)
+    bool pending = false;      // mixed precision: the next iteration refreshes and tests in FP64
+    bool eager = false;        // after such an iteration the g/gp parity of a captured block is lost
     while (!done) {
-        if (gb.exec) {
+        if (pending) {
+            MGPU_RET(enqueue_iteration(true));
+            eager = true;
+        } else if (gb.exec && !eager) {
             MGPU_HIP(hipGraphLaunch(gb.exec, s));
             launched += block;
-        } else if (use_graph && launched > 0) {
-            // the first block ran eagerly (tile autotuning, kernel attributes); capture the second
+        } else if (use_graph && launched > 0 && !eager) {
+            // the first block ran eagerly (kernel attributes); capture the second
             MGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int rc = MGPU_OK;
-            for (int it = 0; it < block && rc == MGPU_OK; it++) rc = enqueue_iteration();
+            for (int it = 0; it < block && rc == MGPU_OK; it++) rc = enqueue_iteration(false);
             hipGraph_t graph = nullptr;
             const hipError_t ec = hipStreamEndCapture(s, &graph);
             gb.graph = graph;
@@ -684,12 +706,13 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             MGPU_HIP(hipGraphInstantiate(&gb.exec, gb.graph, nullptr, nullptr, 0));
             MGPU_HIP(hipGraphLaunch(gb.exec, s));
         } else {
-            for (int it = 0; it < block && launched + 1 < maxiter; it++) MGPU_RET(enqueue_iteration());
+            for (int it = 0; it < block && launched + 1 < maxiter; it++) MGPU_RET(enqueue_iteration(false));
         }
         int hf[F_NFLAGS];
         MGPU_HIP(hipMemcpyAsync(hf, flag, sizeof(hf), hipMemcpyDeviceToHost, s));
         MGPU_HIP(hipStreamSynchronize(s));
         pb.prof.collect(hf[F_KMAX]);
+        pending = mixed && hf[F_NPEND] && !hf[F_STOP];
         if (hf[F_STOP] || launched + 1 >= maxiter) done = true;
     }
     pb.prof.tag = 0;
@@ -707,8 +730,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     if (h_hist) MGPU_HIP(hipMemcpy(h_hist, d_hist, sizeof(double) * (size_t) maxiter * nseg, hipMemcpyDeviceToHost));
     float ms = 0.f;
     MGPU_HIP(hipEventElapsedTime(&ms, pb.ev0, pb.ev1));
-    // refreshes performed: one per `refresh` iterations up to the last iteration that ran
-    const int refreshes = refresh ? hf[F_KMAX] / refresh : 0;
+    const int refreshes = hf[F_NREF];
 
     std::vector<double> true_rel((size_t) nseg, -1.0);
     if (mixed && (rep || segs)) {
@@ -718,7 +740,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         const int rc = apply_A64(pb, x, Ax, nullptr);
         pb.prof.on = keep;
         MGPU_RET(rc);
-        k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, qv, wseg, wq0, sq, nullptr, pb.d_flag + F_NEVER);
+        k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, qv, wseg, wq0, sq, nullptr, pb.d_flag + F_NEVER, nullptr);
         MGPU_LAUNCHED();
         k_two_norms<<<nwg, kMB, 0, s>>>(qv, bvec, wseg, wq0, sq, part);
         MGPU_LAUNCHED();
@@ -745,6 +767,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         rep->lcount = 3 * npix + 3 * pb.pt.trash;
         rep->n_cut = pb.pc.n_cut;
         rep->t_solve_ms = ms;
+        rep->t_setup_ms = pb.t_create_ms + pb.t_precond_ms;
         rep->refreshes = refreshes;
         rep->tile_variant = pb.tp.variant[mixed ? 1 : 0];
         rep->true_res_rel = true_rel[0];

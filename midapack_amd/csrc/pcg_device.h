@@ -28,8 +28,11 @@ enum {
 };
 
 // device flags: [0] every segment stopped, [1] segment 0 converged, [2] largest
-// iteration count reached, [3] number of stopped segments; [4..7] never set
-enum { F_STOP = 0, F_CONV0, F_KMAX, F_NDONE, F_NEVER = 4, F_NFLAGS = 8 };
+// iteration count reached, [3] number of stopped segments, [4] never set,
+// [5] a segment's FP32-updated residual met the stop test (mixed precision:
+// the next iteration refreshes the residual in FP64 and tests it), [6] FP64
+// residual refreshes performed
+enum { F_STOP = 0, F_CONV0, F_KMAX, F_NDONE, F_NEVER = 4, F_NPEND = 5, F_NREF = 6, F_NFLAGS = 8 };
 
 constexpr int kMB = kMapWG;  // map-domain workgroup size (pixels)
 

@@ -1177,7 +1177,8 @@ static TileUse tile_use(const ToeplitzPlan &T) {
 }
 
 template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO, int MINW>
-static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *coef, const void *tw) {
+static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *coef, const void *tw,
+                            const ToeplitzScratch &sc) {
     const ToeplitzPlan &T = pb.tp;
     const TileUse u = tile_use<R>(T);
     constexpr int NC = 1 << LOGNC, NT = NC / (1 << LOGV);
@@ -1189,7 +1190,7 @@ static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *co
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
         attr_devices |= 1u << pb.device;
     }
-    TileArgs a{d_in, d_out, u.tiles, coef, tw, T.d_mean, T.d_edge_part, T.lambda, u.L, pb.sg.d_block_seg,
+    TileArgs a{d_in, d_out, u.tiles, coef, tw, sc.mean, sc.edge_part, T.lambda, u.L, pb.sg.d_block_seg,
                pb.sg.d_block_seg ? pb.live_state : nullptr};
     hipLaunchKernelGGL(kern, dim3(u.ntiles), dim3(NT), lds, pb.stream, a);
     MGPU_LAUNCHED();
@@ -1197,13 +1198,13 @@ static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *co
 }
 
 template <typename R>
-static int launch_tiles(Problem &pb, const R *d_in, R *d_out, int variant) {
+static int launch_tiles(Problem &pb, const R *d_in, R *d_out, int variant, const ToeplitzScratch &sc) {
     const ToeplitzPlan &T = pb.tp;
     constexpr bool F32 = sizeof(R) == 4;
     const TileUse u = tile_use<R>(T);
     const void *coef = u.coef, *tw = u.tw;
 #define MGPU_TL(LOGNC, LOGV, LOGR, TWO, MINW) \
-    return launch_tiles_cfg<R, LOGNC, LOGV, LOGR, TWO, MINW>(pb, d_in, d_out, coef, tw)
+    return launch_tiles_cfg<R, LOGNC, LOGV, LOGR, TWO, MINW>(pb, d_in, d_out, coef, tw, sc)
     switch (u.log2n - 1) {  // LOGNC
         case 9: MGPU_TL(9, 3, 3, true, 1);
         case 10:
@@ -1241,25 +1242,42 @@ static int launch_tiles(Problem &pb, const R *d_in, R *d_out, int variant) {
 #undef MGPU_TL
 }
 
-// Process-wide autotuning results: (device, precision, tile length) -> variant.
+// Fixed tile variant of a (precision, tile length): the one the products use
+// unless the caller forces another or opts into autotuning.  FP64 results
+// therefore depend only on the inputs (variants differ in pass structure and
+// round differently).  The entries are the fastest variants measured on
+// MI355X for the lengths the workloads use (the variant tables above).
+static int default_variant(bool fp32, int log2n) {
+    const int lognc = log2n - 1;
+    if (!fp32 && lognc == 14) return 2;
+    if (fp32 && lognc == 14) return 4;
+    return 1;
+}
+
+// Process-wide autotuning results (opt-in, tile_variant = MGPU_TILE_AUTOTUNE):
+// (device, precision, tile lengths) -> variant.  The mutex is held while a
+// key is being timed, so concurrent first products of one key wait for the
+// result instead of timing under each other's load.
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int>, int> g_tune;
 
 // Choose the tile variant of precision R for this plan: the forced one, the
-// cached result, or the fastest of 1 warm-up + 3 timed launches of each
-// variant on the caller's buffers (d_out is overwritten by every launch; the
-// caller launches the winner afterwards).
+// fixed default, or (MGPU_TILE_AUTOTUNE) the cached result or the fastest of
+// 1 warm-up + 3 timed launches of each variant on the caller's buffers (d_out
+// is overwritten by every launch; the caller launches the winner afterwards).
 template <typename R>
-static int pick_variant(Problem &pb, const R *d_in, R *d_out) {
+static int pick_variant(Problem &pb, const R *d_in, R *d_out, const ToeplitzScratch &sc) {
     ToeplitzPlan &T = pb.tp;
     const int slot = sizeof(R) == 4 ? 1 : 0;
     if (T.variant[slot]) return MGPU_OK;
+    // FP32 products use the FP64 tile length unless autotuning picks the long set
+    if (slot == 1) T.fp32_long = false;
     const int log2n = tile_use<R>(T).log2n;
     const int nv = tile_variant_count(slot == 1, log2n);
     MGPU_ARG(nv > 0, "unsupported tile length 2^%d", log2n);
     // a forced variant applies to the precision of the PCG iterations (FP32 under
     // MGPU_MIXED); the FP64 products of a mixed solve (right-hand side, residual
-    // refreshes) are autotuned
+    // refreshes) use the fixed FP64 default
     const int main_slot = pb.opt.precision == MGPU_MIXED ? 1 : 0;
     if (pb.opt.tile_variant > 0 && slot == main_slot) {
         MGPU_ARG(pb.opt.tile_variant <= nv, "tile_variant=%d: the %s tile of length 2^%d has %d variant(s)",
@@ -1267,42 +1285,46 @@ static int pick_variant(Problem &pb, const R *d_in, R *d_out) {
         T.variant[slot] = pb.opt.tile_variant;
         return MGPU_OK;
     }
-    // FP32 with a long-tile set: the candidates are every variant of both lengths
+    if (pb.opt.tile_variant != MGPU_TILE_AUTOTUNE || slot != main_slot) {
+        MGPU_ARG(pb.opt.tile_variant >= 0 || pb.opt.tile_variant == MGPU_TILE_AUTOTUNE,
+                 "tile_variant=%d: use 0 (fixed default), k > 0 or MGPU_TILE_AUTOTUNE", pb.opt.tile_variant);
+        T.variant[slot] = std::min(default_variant(slot == 1, log2n), nv);
+        return MGPU_OK;
+    }
+    // autotuning; FP32 with a long-tile set: the candidates are every variant of both lengths
     const bool two_len = slot == 1 && T.log2n32 > 0;
+    const int nv_long = two_len ? tile_variant_count(true, T.log2n32) : 0;
     if (nv == 1 && !two_len) {
         T.variant[slot] = 1;
         return MGPU_OK;
     }
-    const int nv_short = two_len ? tile_variant_count(true, T.log2n) : 0;
-    const auto key = std::make_tuple(pb.device, slot, log2n * 100 + (two_len ? T.log2n : 0));
-    {
-        std::lock_guard<std::mutex> g(g_tune_mu);
-        auto it = g_tune.find(key);
-        if (it != g_tune.end()) {
-            T.variant[slot] = it->second % 1000;
-            if (two_len) T.fp32_long = it->second < 1000;
-            return MGPU_OK;
-        }
+    const auto key = std::make_tuple(pb.device, slot, log2n * 100 + (two_len ? T.log2n32 : 0));
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+        T.variant[slot] = it->second % 1000;
+        if (two_len) T.fp32_long = it->second >= 1000;
+        return MGPU_OK;
     }
     hipEvent_t e0, e1;
     MGPU_HIP(hipEventCreate(&e0));
     MGPU_HIP(hipEventCreate(&e1));
     int best = 1;
-    bool best_long = true;
+    bool best_long = false;
     float best_ms = 0.f;
     int rc = MGPU_OK;
-    for (int c = 1; c <= nv + nv_short && rc == MGPU_OK; c++) {
-        const bool lng = c <= nv;
-        const int v = lng ? c : c - nv;
+    for (int c = 1; c <= nv + nv_long && rc == MGPU_OK; c++) {
+        const bool lng = c > nv;
+        const int v = lng ? c - nv : c;
         if (two_len) T.fp32_long = lng;
-        rc = launch_tiles<R>(pb, d_in, d_out, v);
+        rc = launch_tiles<R>(pb, d_in, d_out, v, sc);
         if (rc != MGPU_OK) break;
         if (hipEventRecord(e0, pb.stream) != hipSuccess) {
             set_error("tile autotuning: hipEventRecord failed");
             rc = MGPU_EHIP;
             break;
         }
-        for (int k = 0; k < 3 && rc == MGPU_OK; k++) rc = launch_tiles<R>(pb, d_in, d_out, v);
+        for (int k = 0; k < 3 && rc == MGPU_OK; k++) rc = launch_tiles<R>(pb, d_in, d_out, v, sc);
         // per useful output: the two lengths have different tile counts but the same outputs
         if (rc != MGPU_OK) break;
         float ms = 0.f;
@@ -1323,13 +1345,15 @@ static int pick_variant(Problem &pb, const R *d_in, R *d_out) {
     if (rc != MGPU_OK) return rc;
     T.variant[slot] = best;
     if (two_len) T.fp32_long = best_long;
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[key] = best + (two_len && !best_long ? 1000 : 0);
+    g_tune[key] = best + (best_long ? 1000 : 0);
     return MGPU_OK;
 }
 
+// scratch of one product: per-chunk partial sums, block means, slice sums of
+// the symmetric mode's output mean (the plan's own buffers, or a caller's)
 template <typename R>
-static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_partials) {
+static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_partials,
+                            const ToeplitzScratch &sc) {
     ToeplitzPlan &T = pb.tp;
     hipStream_t s = pb.stream;
     if (T.lambda == 1) {
@@ -1337,12 +1361,12 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
             // stbmmProd with lambda 1 (the two-level products): block means removed
             // (with lambda 1, C T C = T C, so the symmetric mode is the same product)
             k_partials<R><<<(unsigned) T.nchunks, 256, 0, s>>>(d_in, T.m, T.d_start, T.d_chunk_block, T.nb,
-                                                               T.d_partials);
+                                                               sc.partials);
             MGPU_LAUNCHED();
-            k_block_mean<R><<<dim3(T.nb, 1), 256, 0, s>>>(T.d_partials, T.d_start, T.nb, T.d_mean, d_in, nullptr,
+            k_block_mean<R><<<dim3(T.nb, 1), 256, 0, s>>>(sc.partials, T.d_start, T.nb, sc.mean, d_in, nullptr,
                                                           nullptr, nullptr, 1, nullptr, nullptr, nullptr);
             MGPU_LAUNCHED();
-            k_diag_mean<R><<<grid_for(T.m, 256), 256, 0, s>>>(d_in, d_out, T.m, T.d_start, T.nb, T.d_T0, T.d_mean);
+            k_diag_mean<R><<<grid_for(T.m, 256), 256, 0, s>>>(d_in, d_out, T.m, T.d_start, T.nb, T.d_T0, sc.mean);
         } else {
             k_diag<R><<<grid_for(T.m, 256), 256, 0, s>>>(d_in, d_out, T.m, T.d_start, T.nb, T.d_T0);
         }
@@ -1353,30 +1377,30 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
         pb.prof.begin(MGPU_KT_MEAN, s);
         if (compute_partials) {
             k_partials<R><<<(unsigned) T.nchunks, 256, 0, s>>>(d_in, T.m, T.d_start, T.d_chunk_block, T.nb,
-                                                               T.d_partials);
+                                                               sc.partials);
             MGPU_LAUNCHED();
         }
         const bool sym = T.offset_mode == MGPU_OFFSET_SYMMETRIC;
         k_block_mean<R><<<dim3(T.nb, sym ? kEdgeSlices : 1), 256, 0, s>>>(
-                T.d_partials, T.d_start, T.nb, T.d_mean, d_in, T.d_tail, T.d_block_filter, T.d_block_kind, T.lambda,
-                sym ? T.d_edge_part : nullptr, pb.sg.d_block_seg, pb.sg.d_block_seg ? pb.live_state : nullptr);
+                sc.partials, T.d_start, T.nb, sc.mean, d_in, T.d_tail, T.d_block_filter, T.d_block_kind, T.lambda,
+                sym ? sc.edge_part : nullptr, pb.sg.d_block_seg, pb.sg.d_block_seg ? pb.live_state : nullptr);
         MGPU_LAUNCHED();
         pb.prof.end(s);
     }
     if (T.ntiles) {
-        MGPU_RET(pick_variant<R>(pb, d_in, d_out));
+        MGPU_RET(pick_variant<R>(pb, d_in, d_out, sc));
         pb.prof.begin(MGPU_KT_TILES, s);
-        MGPU_RET(launch_tiles<R>(pb, d_in, d_out, T.variant[sizeof(R) == 4 ? 1 : 0]));
+        MGPU_RET(launch_tiles<R>(pb, d_in, d_out, T.variant[sizeof(R) == 4 ? 1 : 0], sc));
         pb.prof.end(s);
     }
     if (!T.h_direct_blocks.empty()) {
         k_direct<R><<<dim3(64, (unsigned) T.h_direct_blocks.size()), 256, 0, s>>>(
-                d_in, d_out, T.d_start, T.d_direct_blocks, T.d_mean, T.d_edge_part, T.d_direct_invtt, T.lambda);
+                d_in, d_out, T.d_start, T.d_direct_blocks, sc.mean, sc.edge_part, T.d_direct_invtt, T.lambda);
         MGPU_LAUNCHED();
     }
     if (!T.h_short_blocks.empty()) {
         k_mean_only<R><<<dim3(16, (unsigned) T.h_short_blocks.size()), 256, 0, s>>>(d_in, d_out, T.d_start,
-                                                                                  T.d_short_blocks, T.d_mean);
+                                                                                  T.d_short_blocks, sc.mean);
         MGPU_LAUNCHED();
     }
     if (T.covered < T.m) {
@@ -1386,12 +1410,40 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
     return MGPU_OK;
 }
 
+static ToeplitzScratch plan_scratch(const ToeplitzPlan &T) { return {T.d_partials, T.d_mean, T.d_edge_part}; }
+
 int toeplitz_apply(Problem &pb, const double *d_in, double *d_out, bool compute_partials) {
-    return toeplitz_apply_t<double>(pb, d_in, d_out, compute_partials);
+    return toeplitz_apply_t<double>(pb, d_in, d_out, compute_partials, plan_scratch(pb.tp));
 }
 
 int toeplitz_apply_f32(Problem &pb, const float *d_in, float *d_out, bool compute_partials) {
-    return toeplitz_apply_t<float>(pb, d_in, d_out, compute_partials);
+    return toeplitz_apply_t<float>(pb, d_in, d_out, compute_partials, plan_scratch(pb.tp));
+}
+
+size_t toeplitz_scratch_doubles(const ToeplitzPlan &T) {
+    return (size_t) (T.nchunks + T.nb) + (size_t) T.nb + (size_t) T.nb * kEdgeSlices;
+}
+
+__global__ void k_fill_zero(double *__restrict__ p, int64_t n) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+int toeplitz_apply_scratch(Problem &pb, const double *d_in, double *d_out, double *d_scratch) {
+    const ToeplitzPlan &T = pb.tp;
+    ToeplitzScratch sc;
+    sc.partials = d_scratch;
+    sc.mean = sc.partials + (T.nchunks + T.nb);
+    sc.edge_part = sc.mean + T.nb;
+    // means and output-mean slices that the product reads but no kernel of this
+    // mode writes must be zero (the array keeps the previous call's values):
+    // cleared with a kernel so that a captured graph holds no memset node
+    const int64_t nz = (int64_t) T.nb * (1 + kEdgeSlices);
+    if (T.lambda > 1 && T.offset_mode != MGPU_OFFSET_SYMMETRIC) {
+        k_fill_zero<<<grid_for(nz, 256), 256, 0, pb.stream>>>(sc.mean, nz);
+        MGPU_LAUNCHED();
+    }
+    return toeplitz_apply_t<double>(pb, d_in, d_out, true, sc);
 }
 
 }  // namespace mgpu

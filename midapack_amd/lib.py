@@ -22,6 +22,7 @@ MGPU_MIXED = 1
 MGPU_OFFSET_REFERENCE = 0
 MGPU_OFFSET_SYMMETRIC = 1
 MGPU_OFFSET_NONE = 2
+MGPU_TILE_AUTOTUNE = -1
 MGPU_APOD_CHEBWIN = 0
 MGPU_APOD_GAUSSIAN = 1
 

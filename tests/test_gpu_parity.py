@@ -324,14 +324,24 @@ def test_mixed_tile_variants_map_error(gpu, case):
     assert err <= 1e-6
 
 
-def test_tile_autotune_picks_a_variant(gpu):
+def test_tile_variant_default_is_fixed(gpu):
+    """Without tile_variant the products use the fixed default of the tile length
+    (FP64 results depend on the inputs only); MGPU_TILE_AUTOTUNE picks one of the
+    variants by timing."""
     wl = W.make_workload("autotune", 2, 1 << 17, 64, 64, 8192, 3, block_len=1 << 17)
-    print("case: autotuned tile variant, lambda 8192", flush=True)
+    print("case: default and autotuned tile variant, lambda 8192", flush=True)
+    fixed = {gpu.MGPU_FP64: 2, gpu.MGPU_MIXED: 4}
     for prec in (gpu.MGPU_FP64, gpu.MGPU_MIXED):
         g = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
                         maxiter=50, opts=gpu.options(precision=prec, offset_mode=gpu.MGPU_OFFSET_SYMMETRIC))
+        assert g.report["tile_variant"] == fixed[prec]
+        a = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
+                        maxiter=50, opts=gpu.options(precision=prec, offset_mode=gpu.MGPU_OFFSET_SYMMETRIC,
+                                                     tile_variant=gpu.MGPU_TILE_AUTOTUNE))
         nv = len(gpu.tile_variants(prec, 15))
-        assert 1 <= g.report["tile_variant"] <= nv
+        assert 1 <= a.report["tile_variant"] <= nv
+        if prec == gpu.MGPU_FP64 and a.report["tile_variant"] == fixed[prec]:
+            assert np.array_equal(a.x, g.x)
 
 
 def test_mixed_precision_map_error(gpu):
