@@ -1,19 +1,24 @@
 #!/usr/bin/env python3
 """Time-to-map of the GLS map-making solve on one MI355X.
 
-A step is one complete solve of the workload from raw inputs already resident
-in HBM (MLmap's arrays: global pixel indices, weights, signal, noise, invtt):
-pointing build (MatInit), block-Jacobi preconditioner with the degenerate-pixel
-cut, and PCG to a relative residual of 1e-8.  Default workload: 64 detectors x
-2^20 samples on a 1024^2 grid, lambda = 8192, 2^17-sample Toeplitz blocks,
-FP64 (one rank's share).
+A step is one complete drop-in solve of the workload from the caller's host
+arrays (MLmap's arrays: global pixel indices, weights, signal, noise, invtt),
+exactly as MLmap hands them over: upload to HBM, pointing build (MatInit),
+Toeplitz plan, block-Jacobi preconditioner with the degenerate-pixel cut, PCG
+to a relative residual of 1e-8, the map back in host memory and the
+reference's side effects written into the caller's pix / weights / signal.
+Default workload: 64 detectors x 2^20 samples on a 1024^2 grid, lambda = 8192,
+2^17-sample Toeplitz blocks, FP64 (one rank's share).  Every step gets fresh
+copies of the input arrays (the solve rewrites pix, weights and signal in
+place); the copies are made outside the timed region.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 
-Prints one JSON line.  The roofline entry is for the dominant kernel (the
-Toeplitz FFT tiles of N^-1): algorithmic bytes per launch over its average
-launch duration, measured with HIP events on the library's stream during the
-timed steps.
+Prints one JSON line.  Beside `value` it reports the same solve with the
+inputs already resident in HBM (`device_resident`).  The roofline entry is for
+the dominant kernel (the Toeplitz FFT tiles of N^-1): algorithmic bytes per
+launch over its average launch duration, measured with HIP events on the
+library's stream during the device-resident solves.
 """
 from __future__ import annotations
 
@@ -81,8 +86,8 @@ def main():
                     help="per-block offset handling of N^-1 for the timed solves")
     ap.add_argument("--no-reference-probe", action="store_true")
     ap.add_argument("--c4-real", type=int, default=32, help="batched Monte Carlo realizations (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=12, help="requests per serving rate (0: skip)")
-    ap.add_argument("--serve-rates", default="1,2,5,10", help="Poisson arrival rates (req/s)")
+    ap.add_argument("--serve-requests", type=int, default=50, help="requests per serving rate (0: skip)")
+    ap.add_argument("--serve-rates", default="1,2,5,10,20,50", help="Poisson arrival rates (req/s)")
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded requests cycled by the load")
     ap.add_argument("--no-noise-model", action="store_true", help="skip the invtt-from-noise section")
     ap.add_argument("--no-templates", action="store_true", help="skip the template map-making (MTmap) section")
@@ -140,29 +145,43 @@ def main():
         pb.close()
         return x, hist, rep, kt
 
-    for i in range(args.warmup):
-        _, _, rep, _ = solve(G.MGPU_FP64)
-        log(f"warmup {i}: {rep.iterations} iterations, converged={rep.converged}")
+    # ---- the headline: drop-in solves from the caller's host arrays ---------
+    hp, hw, hs = np.empty_like(wl.pix), np.empty_like(wl.weights), np.empty_like(wl.signal)
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reps, kts = [], []
-    x64 = None
+    def from_host(precision, profile=False, offset_mode=offset, maxiter=args.maxiter):
+        """One drop-in solve on fresh copies of the inputs (made before the clock
+        starts); returns (result, seconds)."""
+        np.copyto(hp, wl.pix)
+        np.copyto(hw, wl.weights)
+        np.copyto(hs, wl.signal)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = G.pcg_gls(hp, hw, hs, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=args.tol, maxiter=maxiter,
+                      opts=G.options(precision=precision, profile=profile, offset_mode=offset_mode), inplace=True)
+        torch.cuda.synchronize()
+        return g, time.perf_counter() - t0
+
+    for i in range(args.warmup):
+        g, dt = from_host(G.MGPU_FP64)
+        log(f"warmup {i}: {dt:.3f} s, {g.report['iterations']} iterations, converged={g.report['converged']}")
+    G.dropin_kernel_times(reset=True)
+    times, reps = [], []
+    g = None
     for i in range(args.steps):
-        x64, hist, rep, kt = solve(G.MGPU_FP64)
-        reps.append(rep.as_dict())
-        kts.append(kt)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    per_map = elapsed / args.steps
-    log(f"FP64: {per_map:.3f} s per map, iterations {reps[-1]['iterations']}, converged {reps[-1]['converged']}")
+        g, dt = from_host(G.MGPU_FP64, profile=True)
+        times.append(dt)
+        reps.append(g.report)
+    x64 = g.x
+    kt_tot = G.dropin_kernel_times(reset=True)
+    per_map = float(np.mean(times))
+    log(f"FP64 from host arrays: {per_map:.3f} s per map (min {min(times):.3f}), iterations "
+        f"{reps[-1]['iterations']}, converged {reps[-1]['converged']}")
 
     n_map = reps[-1]["n_map"]
     nbytes = algorithmic_bytes(wl.m, n_map, 8, 8)
     kernels = {}
     for k in G.KERNEL_CLASSES:
-        ms = sum(kt[k][0] for kt in kts)
-        cnt = sum(kt[k][1] for kt in kts)
+        ms, cnt = kt_tot[k]
         if cnt == 0:
             continue
         avg = ms / cnt
@@ -184,6 +203,17 @@ def main():
                           "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": round(tf / FP64_VECTOR_PEAK_TFLOPS, 4), "flops_per_launch": flops}
 
+    # ---- the same solve with the inputs resident in HBM ---------------------
+    solve(G.MGPU_FP64, profile=False)
+    ndev = max(1, min(args.steps, 3))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(ndev):
+        _, _, rep_d, _ = solve(G.MGPU_FP64, profile=False)
+    torch.cuda.synchronize()
+    per_map_dev = (time.perf_counter() - t0) / ndev
+    log(f"FP64 inputs resident in HBM: {per_map_dev:.3f} s per map")
+
     out = {
         "metric": "time-to-map at rel. residual 1e-8 (s)",
         "value": round(per_map, 4),
@@ -199,14 +229,19 @@ def main():
         "data": "synthetic (seeded raster scan, 1/f noise, analytic invtt)",
         "config": {"workload": wl.name, "samples": wl.m, "pixels_seen": int(reps[-1]["n_map"] // 3 + reps[-1]["n_cut"]),
                    "lambda": wl.lam, "blocks": int(wl.block_sizes.size), "tol": args.tol,
+                   "inputs": "host arrays (pageable numpy), side effects written back; per-step copies untimed",
                    "n_inv": {"symmetric": "C T C (symmetric per-block offset projection)",
                              "reference": "T C (reference stbmmProd mean removal)",
                              "none": "T"}[args.offset]},
         "iterations": reps[-1]["iterations"],
         "converged": bool(reps[-1]["converged"]),
         "n_cut": reps[-1]["n_cut"],
+        "step_s": [round(t, 4) for t in times],
         "setup_ms": round(float(np.mean([r["t_setup_ms"] for r in reps])), 2),
         "solve_ms": round(float(np.mean([r["t_solve_ms"] for r in reps])), 2),
+        "device_resident": {"time_to_map_s": round(per_map_dev, 4), "steps": ndev,
+                            "iterations": int(rep_d.iterations),
+                            "what": "same solve with pix, weights, signal, noise, invtt already in HBM"},
         "kernels": kernels,
         "roofline": roofline,
         "roofline_flops": roofline_flops,
@@ -214,40 +249,52 @@ def main():
 
     if not args.no_mixed:
         try:
-            xm, hm, repm, ktm = solve(G.MGPU_MIXED)
-            torch.cuda.synchronize()
-            tm0 = time.perf_counter()
-            nm = max(1, args.steps)
+            nm = max(1, min(args.steps, 5))
+            from_host(G.MGPU_MIXED)
+            tms = []
             for _ in range(nm):
-                xm, hm, repm, ktm = solve(G.MGPU_MIXED)
+                gm, dt = from_host(G.MGPU_MIXED)
+                tms.append(dt)
+            tm = float(np.mean(tms))
+            solve(G.MGPU_MIXED, profile=False)
             torch.cuda.synchronize()
-            tm = (time.perf_counter() - tm0) / nm
-            err = float(np.max(np.abs(xm - x64)) / np.max(np.abs(x64)))
+            t0 = time.perf_counter()
+            for _ in range(ndev):
+                xm, hm, repm, _ = solve(G.MGPU_MIXED, profile=False)
+            torch.cuda.synchronize()
+            tmd = (time.perf_counter() - t0) / ndev
+            err = float(np.max(np.abs(gm.x - x64)) / np.max(np.abs(x64)))
             out["mixed"] = {"time_to_map_s": round(tm, 4), "speedup_vs_fp64": round(per_map / tm, 3),
-                            "iterations": repm.iterations, "converged": bool(repm.converged),
-                            "refreshes": repm.refreshes, "true_res_rel": repm.true_res_rel,
+                            "device_resident_s": round(tmd, 4),
+                            "device_resident_speedup_vs_fp64": round(per_map_dev / tmd, 3),
+                            "iterations": gm.report["iterations"], "converged": bool(gm.report["converged"]),
+                            "refreshes": gm.report["refreshes"], "true_res_rel": gm.report["true_res_rel"],
                             "map_rel_err_vs_fp64": err}
-            log(f"mixed: {tm:.3f} s per map, {repm.iterations} iterations, map error vs FP64 {err:.2e}")
+            log(f"mixed: {tm:.3f} s per map from host, {tmd:.3f} s resident, {gm.report['iterations']} iterations, "
+                f"map error vs FP64 {err:.2e}")
         except Exception as e:  # reported, not hidden
             out["mixed"] = {"error": str(e)}
 
     if not args.no_reference_probe and args.offset != "reference":
-        # the reference's one-sided mean removal on the same workload: residual floor
+        # the reference's one-sided mean removal (T C) on the same workload: its
+        # residual floor, then a timed solve stopped where the residual is lowest
         _, hr, rr, _ = solve(G.MGPU_FP64, profile=False, offset_mode=G.MGPU_OFFSET_REFERENCE, maxiter=800)
+        kmin = int(np.nanargmin(hr))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hr2, rr2, _ = solve(G.MGPU_FP64, profile=False, offset_mode=G.MGPU_OFFSET_REFERENCE,
+                               maxiter=kmin + 1)
+        torch.cuda.synchronize()
+        tref = time.perf_counter() - t0
         out["reference_offset_probe"] = {"iterations": rr.iterations, "converged": bool(rr.converged),
                                          "min_rel_residual": float(np.nanmin(hr)),
-                                         "at_iteration": int(np.nanargmin(hr)),
-                                         "final_rel_residual": float(hr[-1])}
-        log(f"reference offsets: converged={rr.converged}, min residual {np.nanmin(hr):.2e}")
-
-    # the same solve from host arrays (the drop-in call: host -> HBM transfers and the
-    # side effects on the caller's arrays included; the copies it modifies are made first)
-    hp, hw, hs = wl.pix.astype(np.int32, copy=True), wl.weights.copy(), wl.signal.copy()
-    t0 = time.perf_counter()
-    gh = G.pcg_gls(hp, hw, hs, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=args.tol,
-                   maxiter=args.maxiter, opts=G.options(offset_mode=offset), inplace=True)
-    out["time_to_map_from_host_s"] = round(time.perf_counter() - t0, 4)
-    log(f"from host arrays: {out['time_to_map_from_host_s']} s ({gh.report['iterations']} iterations)")
+                                         "at_iteration": kmin,
+                                         "final_rel_residual": float(hr[-1]),
+                                         "time_to_min_residual_s": round(tref, 4),
+                                         "what": "T C (stbmmProd's offsets), inputs in HBM, stopped at the "
+                                                 "iteration of its lowest residual"}
+        log(f"reference offsets: converged={rr.converged}, min residual {np.nanmin(hr):.2e} at {kmin}, "
+            f"{tref:.3f} s to reach it")
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args, reps[-1]["iterations"])
@@ -257,7 +304,7 @@ def main():
         out["c1_full_solve"] = c1_section(G, W, args)
     if not args.no_noise_model:
         out["noise_model_c2"] = noise_section(G, wl, args)
-    del d_pix, d_w, d_sig0, d_sig, d_noise, d_inv, wl, gh
+    del d_pix, d_w, d_sig0, d_sig, d_noise, d_inv, wl, hp, hw, hs
     torch.cuda.empty_cache()
     if not args.no_templates:
         out["templates_c6"] = templates_section(G, W, args)
@@ -455,23 +502,28 @@ def serve_section(G, W, args):
 
             th = threading.Thread(target=feed)
             th.start()
-            lat, errs = [], []
+            lat, qs, errs = [], [], []
             for i in range(n):
                 while tickets[i] is None:
                     time.sleep(0.001)
                 j, tk = tickets[i]
-                r, l_ms, _ = srv.wait(tk)
+                r, l_ms, q_ms = srv.wait(tk)
                 lat.append(l_ms)
-                log(f"serve rate {rate}: request {i} done in {l_ms:.0f} ms")
+                qs.append(q_ms)
                 a = ref[j].x
                 errs.append(float(np.max(np.abs(r.x - a)) / np.max(np.abs(a))))
+                if not r.report["converged"]:
+                    errs[-1] = float("inf")
+                if i % 10 == 9:
+                    log(f"serve rate {rate}: {i + 1} requests answered")
             th.join()
             wall = time.perf_counter() - t0
             st = srv.stats()
-        ent = {"rate": rate, "p50_ms": round(float(np.percentile(lat, 50)), 1),
-               "p99_ms": round(float(np.percentile(lat, 99)), 1), "maps_per_s": round(n / wall, 3),
-               "max_map_rel_err_vs_fp64": max(errs), "batched_groups": st["groups"],
-               "peak_concurrent_solves": st["peak_running"], "peak_hbm_gb": round(st["peak_hbm_gb"], 2)}
+        ent = {"rate": rate, "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 1),
+               "p99_ms": round(float(np.percentile(lat, 99)), 1), "mean_queue_ms": round(float(np.mean(qs)), 1),
+               "maps_per_s": round(n / wall, 3), "max_map_rel_err_vs_fp64": max(errs),
+               "batched_groups": st["groups"], "peak_concurrent_solves": st["peak_running"],
+               "peak_hbm_gb": round(st["peak_hbm_gb"], 2)}
         log(f"serve rate {rate}: p50 {ent['p50_ms']} ms, p99 {ent['p99_ms']} ms, {ent['maps_per_s']} maps/s")
         out["rates"].append(ent)
     return out
@@ -532,12 +584,61 @@ def cpu_baseline(wl, args, gpu_iters, threads=1):
     scale = args.ndet / nd
     est = (setup + gpu_iters * per_iter) * scale
     log(f"cpu oracle ({threads} thread(s)) slice {nd}/{args.ndet} det: setup {setup:.2f}s, {per_iter:.2f}s/iteration")
-    return {"value": round(est, 2), "unit": "s", "cores": threads, "kind": "port",
-            "sample": f"first {nd} of {args.ndet} detectors ({n} samples): setup {setup:.2f} s + "
-                      f"{per_iter:.3f} s/iteration measured, scaled x{scale:g} to the full workload at the "
-                      f"GPU's {gpu_iters} iterations" + (" (OpenMP build)" if omp else ""),
-            "cpu": cpu_model()}
+    out = {"value": round(est, 2), "unit": "s", "cores": threads, "kind": "port",
+           "sample": f"first {nd} of {args.ndet} detectors ({n} samples): setup {setup:.2f} s + "
+                     f"{per_iter:.3f} s/iteration measured, scaled x{scale:g} to the full workload at the "
+                     f"GPU's {gpu_iters} iterations" + (" (OpenMP build)" if omp else ""),
+           "cpu": cpu_model()}
+    if not omp:
+        cal = fft_calibration()
+        out["fft_calibration"] = cal
+        out["sample"] += (f"; the oracle's FFT takes {cal['ratio_c2']:.2f}x the time of scipy.fft (pocketfft, "
+                          f"1 thread) at the C2 tile size")
+    return out
+
+
+def fft_calibration():
+    """The oracle's complex FFT (what its real tile transforms run on, at half
+    the real tile length) against scipy.fft's on one thread, at the tile sizes
+    of the reference's rule for C1 (bs 4096) and C2 (bs 32768): the factor by
+    which the oracle's FFT may overstate the CPU baseline's N^-1 time."""
+    import scipy.fft as sfft
+    from oracle import oracle as O
+
+    res = {}
+    for tag, n in (("c1", 2048), ("c2", 16384)):
+        z = np.random.default_rng(n).standard_normal(n) + 1j * np.random.default_rng(n + 1).standard_normal(n)
+        reps = max(20, (1 << 22) // n)
+
+        def best(fn):
+            fn()
+            t = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    fn()
+                t.append((time.perf_counter() - t0) / reps)
+            return min(t)
+
+        to = best(lambda: O.fft(z))
+        ts = best(lambda: sfft.fft(z, workers=1))
+        err = float(np.max(np.abs(O.fft(z) - sfft.fft(z))) / np.max(np.abs(sfft.fft(z))))
+        res[f"oracle_us_{tag}"] = round(to * 1e6, 2)
+        res[f"scipy_us_{tag}"] = round(ts * 1e6, 2)
+        res[f"ratio_{tag}"] = round(to / ts, 3)
+        res[f"max_rel_diff_{tag}"] = err
+    log(f"FFT calibration: oracle/scipy = {res['ratio_c1']} (n=2048), {res['ratio_c2']} (n=16384)")
+    return res
 
 
 if __name__ == "__main__":
     main()
+    # every helper thread has been joined; release the library's device and
+    # pinned host memory, then leave at once (no lingering interpreter teardown)
+    from midapack_amd import lib as _G
+
+    if _G._LIB is not None:
+        _G.finalize()
+    sys.stdout.flush()
+    sys.stderr.flush()
+    os._exit(0)

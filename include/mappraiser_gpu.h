@@ -178,8 +178,10 @@ int mappraiser_gpu_solve_device(mgpu_problem *pb, double *d_signal, const double
 
 /* Kernel timing of a handle created with opt.profile = 1: accumulated device
  * milliseconds and launch counts per kernel class (MGPU_KT_*), measured with
- * HIP events on the handle's stream; reset != 0 clears them afterwards.  No
- * reference counterpart (the reference prints MPI_Wtime splits,
+ * HIP events on the handle's stream; reset != 0 clears them afterwards.  With
+ * pb == NULL: the totals over every drop-in solve (mappraiser_gpu_pcg_gls,
+ * MLmap, MTmap) run with opt.profile = 1 in this process.  No reference
+ * counterpart (the reference prints MPI_Wtime splits,
  * pcg_true.c:27,98-99,178-179). */
 int mappraiser_gpu_kernel_times(mgpu_problem *pb, double *ms, long long *counts, int reset);
 
@@ -210,14 +212,17 @@ int mappraiser_gpu_noise_invtt(const double *noise, int nb, const int *local_blo
 /* ---- batched solves ----
  * One item is one independent GLS problem (a Monte Carlo noise realization, a
  * detector subset, a serving request) with its own pointing, signal, noise,
- * Toeplitz blocks (their sizes must sum to m) and lambda.  Items of equal
- * lambda and equal Toeplitz tile length (the one each would use alone) are
- * solved together as one segmented problem: every kernel of a PCG iteration
- * is launched once for all of them, each item keeps its own PCG scalars and
- * stop test, and converged items are frozen while the others go on.  Each
- * item's map, iteration count and residuals are bit-identical to its solo
- * solve with the same options (tile variants are autotuned once per process).
- * Groups are solved one after the other.  No reference counterpart: the reference solves one
+ * Toeplitz blocks (their sizes must sum to m) and lambda.  All items are
+ * solved together as one segmented problem, whatever their bands and sizes:
+ * every kernel of a PCG iteration is launched once for all of them (the
+ * Toeplitz tiles once per (lambda, tile length) class, each block keeping the
+ * tile length its item would use alone), each item keeps its own PCG scalars
+ * and stop test, and converged items are frozen while the others go on.
+ * Each item's map, iteration count and residuals are bit-identical to its
+ * solo solve with the same options (with MGPU_TILE_AUTOTUNE: when both runs
+ * pick the same tile variants).  Items with diagonal noise (lambda = 1), and
+ * items whose band is too wide for a tile, form problems of their own, solved
+ * after the others.  No reference counterpart: the reference solves one
  * problem per MLmap call (mappraiser.c:34-232). */
 typedef struct mgpu_batch_item {
     /* inputs: host memory, or device memory when inputs_on_device != 0 */
@@ -247,13 +252,17 @@ int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int grid_npix, 
  * server_wait returns for their ticket).  A dispatcher admits the oldest
  * queued request as soon as a slot is free and its estimated device footprint
  * fits what the running solves leave of the HBM budget, together with every
- * later queued request of the same solve key (lambda, tile length, grid,
- * tol, maxiter) that fits, up to max_batch; each admitted group runs as one
- * mappraiser_gpu_pcg_batch on its own stream and host thread, so solves
+ * later queued request that fits, up to max_batch, whatever their sizes,
+ * bands, grids, tolerances and iteration caps; each admitted group runs as
+ * one batched solve (as mappraiser_gpu_pcg_batch, each request with its own
+ * grid, tol and maxiter) on its own stream and host thread, so solves
  * admitted at different times overlap on the device (opt->graph is ignored:
- * no graph is captured while other threads allocate).  Results are
- * bit-identical to the requests' solo solves.  hbm_budget_gb <= 0 uses 80% of
- * the free device memory.  No reference counterpart. */
+ * no graph is captured while other threads allocate).  A request is answered
+ * as soon as its own map has converged, while the rest of its group iterates
+ * on (its report then carries no solve time and, in MGPU_MIXED, no FP64 true
+ * residual).  Results are bit-identical to the requests' solo solves.
+ * hbm_budget_gb <= 0 uses 80% of the free device memory.  No reference
+ * counterpart. */
 typedef struct mgpu_server mgpu_server;
 
 int mappraiser_gpu_server_create(mgpu_server **out, double hbm_budget_gb, int max_slots, int max_batch,

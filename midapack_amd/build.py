@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmappraiser_gpu.so")
-SOURCES = ["api.hip", "pointing.hip", "toeplitz.hip", "precond.hip", "pcg.hip", "batch.hip", "server.hip", "noise.hip", "twolvl.hip", "mlmap.hip", "templates.hip"]
+SOURCES = ["api.hip", "pointing.hip", "toeplitz.hip", "precond.hip", "pcg.hip", "batch.hip", "server.hip", "noise.hip", "twolvl.hip", "mlmap.hip", "templates.hip", "hostio.hip"]
 ARCH = os.environ.get("MGPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -59,7 +59,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         raise RuntimeError("hipcc failed")
     tmp = LIB + ".tmp"
     rc, out = _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-                    "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+                    "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lrocprofiler-sdk-roctx",
+                    "-Wl,-rpath,/opt/rocm/lib"])
     if rc:
         print(out, file=sys.stderr)
         raise RuntimeError("link failed")

@@ -18,6 +18,10 @@
 namespace mgpu {
 
 static thread_local char g_err[1024] = "";
+// kernel-class times of the drop-in solves run with opt.profile (process-wide)
+static std::mutex g_kt_mu;
+static double g_kt_ms[MGPU_NKT] = {0};
+static long long g_kt_cnt[MGPU_NKT] = {0};
 static int g_device = 0;
 
 void set_error(const char *fmt, ...) {
@@ -100,8 +104,7 @@ Problem::~Problem() {
 template <typename T>
 static int upload(Problem &pb, T **d, const T *h, size_t n) {
     MGPU_RET(pb.arena.alloc(d, n));
-    MGPU_HIP(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, pb.stream));
-    return MGPU_OK;
+    return h2d_staged(pb.device, *d, h, n * sizeof(T), pb.stream);
 }
 
 int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt) {
@@ -139,6 +142,7 @@ int problem_setup(Problem &pb, const int *d_pix, const double *d_w, int64_t m, i
 }
 
 int precond_build_timed(Problem &pb, double *d_signal) {
+    TimerRange tr("Preconditioner computation time");  // pcg_true.c:48-56
     MGPU_HIP(hipEventRecord(pb.ev0, pb.stream));
     MGPU_RET(precond_build(pb, d_signal));
     MGPU_HIP(hipEventRecord(pb.ev1, pb.stream));
@@ -172,6 +176,7 @@ int mappraiser_gpu_init(int device) {
 
 int mappraiser_gpu_finalize(void) {
     dev_cache_trim(-1);
+    staging_free();
     return MGPU_OK;
 }
 
@@ -358,7 +363,18 @@ int mappraiser_gpu_solve_device(mgpu_problem *h, double *d_signal, const double 
 }
 
 int mappraiser_gpu_kernel_times(mgpu_problem *h, double *ms, long long *counts, int reset) {
-    MGPU_ARG(h, "NULL handle");
+    if (!h) {  // totals of the drop-in solves run with opt.profile
+        std::lock_guard<std::mutex> g(g_kt_mu);
+        for (int i = 0; i < MGPU_NKT; i++) {
+            if (ms) ms[i] = g_kt_ms[i];
+            if (counts) counts[i] = g_kt_cnt[i];
+            if (reset) {
+                g_kt_ms[i] = 0.0;
+                g_kt_cnt[i] = 0;
+            }
+        }
+        return MGPU_OK;
+    }
     Problem &pb = *PB(h);
     MGPU_HIP(hipSetDevice(pb.device));
     MGPU_HIP(hipStreamSynchronize(pb.stream));
@@ -427,50 +443,138 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
                  const double *noise, int nb, const int *local_blocks_sizes, int lambda, const double *invtt,
                  double tol, int maxiter, double *x, double *cond, int *lhits, int *lindices,
                  const mgpu_options *opt, mgpu_report *rep, const mgpu_templates *tm) {
-    mgpu_problem *h = nullptr;
+    g_err[0] = 0;
     MGPU_ARG(pix && pixweights && signal && noise && x, "NULL argument");
-    MGPU_RET(mappraiser_gpu_problem_create(&h, m, nnz, pix, pixweights, nb, local_blocks_sizes, lambda, invtt, opt));
-    std::unique_ptr<Problem> guard(PB(h));
+    MGPU_RET(check_common(m, nnz, nb, local_blocks_sizes, lambda));
+    std::unique_ptr<Problem> guard;
+    MGPU_RET(new_problem(guard, opt));
     Problem &pb = *guard;
     hipStream_t s = pb.stream;
+    const int dev = pb.device;
+    const size_t mm = (size_t) m;
+    int *d_pix;
+    double *d_w;
+    MGPU_RET(pb.arena.alloc(&d_pix, mm * nnz));
+    MGPU_RET(pb.arena.alloc(&d_w, mm * nnz));
+    MGPU_RET(pb.arena.alloc(&pb.d_sig, mm));
+    MGPU_RET(pb.arena.alloc(&pb.d_noise, mm));
+    {
+        TimerRange tr("Initializing pointing matrix");  // mappraiser.c:92-103 (with the Toeplitz set-up)
+        MGPU_RET(h2d_staged(dev, d_pix, pix, sizeof(int) * mm * nnz, s));
+        MGPU_RET(h2d_staged(dev, d_w, pixweights, sizeof(double) * mm * nnz, s));
+        // signal and noise go up on a stream of their own (another host thread)
+        // while the device builds the pointing tables and the Toeplitz plan; the
+        // preconditioner build cuts the device signal, which the solve then uses
+        hipStream_t s_up = nullptr;
+        hipEvent_t e_up = nullptr;
+        MGPU_HIP(hipStreamCreateWithFlags(&s_up, hipStreamNonBlocking));
+        struct UpGuard {
+            hipStream_t &st;
+            hipEvent_t &ev;
+            ~UpGuard() {
+                if (ev) (void) hipEventDestroy(ev);
+                if (st) (void) hipStreamDestroy(st);
+            }
+        } ug{s_up, e_up};
+        MGPU_HIP(hipEventCreateWithFlags(&e_up, hipEventDisableTiming));
+        int rc_up = MGPU_OK;
+        std::string err_up;
+        std::thread up([&] {
+            if (hipSetDevice(dev) != hipSuccess) {
+                rc_up = MGPU_EHIP;
+                err_up = "hipSetDevice failed in the upload thread";
+                return;
+            }
+            rc_up = h2d_staged(dev, pb.d_sig, signal, sizeof(double) * mm, s_up);
+            if (rc_up == MGPU_OK) rc_up = h2d_staged(dev, pb.d_noise, noise, sizeof(double) * mm, s_up);
+            if (rc_up == MGPU_OK && hipEventRecord(e_up, s_up) != hipSuccess) rc_up = MGPU_EHIP;
+            if (rc_up != MGPU_OK) err_up = mappraiser_gpu_last_error();
+        });
+        const int rc = problem_setup(pb, d_pix, d_w, m, nnz, local_blocks_sizes, nb, lambda, nullptr, invtt);
+        up.join();
+        MGPU_RET(rc);
+        if (rc_up != MGPU_OK) {
+            set_error("%s", err_up.c_str());
+            return rc_up;
+        }
+        MGPU_HIP(hipStreamWaitEvent(s, e_up, 0));
+        MGPU_HIP(hipStreamSynchronize(s_up));
+    }
+    pb.arena.release(d_pix);
+    pb.arena.release(d_w);
+    TimerRange total("Total PCG time");  // mappraiser.c:208-231
     if (tm) MGPU_RET(templates_build(pb, tm));
-    // signal and noise go to the device once: the preconditioner build cuts the
-    // device signal, which the solve then uses as it is
-    MGPU_RET(pb.arena.alloc(&pb.d_sig, (size_t) m));
-    MGPU_RET(pb.arena.alloc(&pb.d_noise, (size_t) m));
-    MGPU_HIP(hipMemcpyAsync(pb.d_sig, signal, sizeof(double) * m, hipMemcpyHostToDevice, s));
-    MGPU_HIP(hipMemcpyAsync(pb.d_noise, noise, sizeof(double) * m, hipMemcpyHostToDevice, s));
     MGPU_RET(precond_build_timed(pb, pb.d_sig));
     const int np = pb.pt.npix;
-    MGPU_HIP(hipMemcpyAsync(signal, pb.d_sig, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    const int trash = pb.pt.trash;
+    const int lcount = 3 * np + 3 * trash;
     if (cond) MGPU_HIP(hipMemcpyAsync(cond, pb.pc.d_cond, sizeof(double) * np, hipMemcpyDeviceToHost, s));
     if (lhits) MGPU_HIP(hipMemcpyAsync(lhits, pb.pc.d_hits, sizeof(int) * np, hipMemcpyDeviceToHost, s));
-    MGPU_HIP(hipStreamSynchronize(s));
+    if (lindices) {
+        int *d_lind;
+        MGPU_RET(pb.arena.alloc(&d_lind, (size_t) lcount));
+        MGPU_RET(pointing_lindices(pb, d_lind));
+        MGPU_HIP(hipMemcpyAsync(lindices, d_lind, sizeof(int) * lcount, hipMemcpyDeviceToHost, s));
+        MGPU_HIP(hipStreamSynchronize(s));
+        pb.arena.release(d_lind);
+    }
+    // Side effects of the reference on the caller's arrays, delivered by another
+    // host thread while the device solves: local indices of the rebuilt matrix
+    // into pix; weights and signal of cut samples (valid before, trash rows
+    // now) zeroed (precond.c:601-616).
+    int *d_loc;
+    MGPU_RET(pb.arena.alloc(&d_loc, mm * 3));
+    MGPU_RET(pointing_local_indices(pb, d_loc));
+    hipEvent_t e_loc = nullptr;
+    MGPU_HIP(hipEventCreateWithFlags(&e_loc, hipEventDisableTiming));
+    struct EvGuard {
+        hipEvent_t e;
+        ~EvGuard() { (void) hipEventDestroy(e); }
+    } eg{e_loc};
+    MGPU_HIP(hipEventRecord(e_loc, s));
+    const StageConsumer apply_side_effects = [&](size_t off, const void *chunk, size_t bytes) {
+        const int *loc = static_cast<const int *>(chunk);
+        const int64_t t0 = (int64_t) (off / (3 * sizeof(int))), nt = (int64_t) (bytes / (3 * sizeof(int)));
+        for (int64_t i = 0; i < nt; i++) {
+            const int64_t t = t0 + i;
+            if (trash && loc[3 * i] == 0 && pix[3 * t] >= 0) {
+                for (int c = 0; c < 3; c++) pixweights[3 * t + c] = 0.0;
+                signal[t] = 0.0;
+            }
+        }
+        std::memcpy(pix + 3 * t0, loc, bytes);
+    };
+    int rc_dn = MGPU_OK;
+    std::string err_dn;
+    std::thread down([&] {
+        if (hipSetDevice(dev) != hipSuccess) {
+            rc_dn = MGPU_EHIP;
+            err_dn = "hipSetDevice failed in the side-effect thread";
+            return;
+        }
+        rc_dn = d2h_staged(dev, nullptr, d_loc, sizeof(int) * mm * 3, e_loc, &apply_side_effects);
+        if (rc_dn != MGPU_OK) err_dn = mappraiser_gpu_last_error();
+    });
     std::vector<double> hist(maxiter > 0 ? maxiter : 1);
     mgpu_report r{};
-    MGPU_RET(pcg_run(pb, pb.d_sig, pb.d_noise, tol, maxiter, 0, x, hist.data(), &r));
-    // side effects of the reference on the caller's pointing: local indices of
-    // the rebuilt matrix; weights of cut samples zeroed (host loops split over threads)
-    const int64_t mn = (int64_t) m * 3;
-    std::vector<int> loc((size_t) mn);
-    int lcount = 0, trash = 0;
-    MGPU_RET(mappraiser_gpu_local_shape(h, &lcount, &trash, lindices, loc.data()));
-    const int nth = (int) std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (int k = 0; k < nth; k++)
-        th.emplace_back([&, k] {
-            const int64_t t0 = (int64_t) m * k / nth, t1 = (int64_t) m * (k + 1) / nth;
-            for (int64_t t = t0; t < t1; t++) {
-                const bool was_valid = pix[3 * t] >= 0;
-                const bool now_trash = trash && loc[3 * t] == 0;
-                if (was_valid && now_trash)
-                    for (int c = 0; c < 3; c++) pixweights[3 * t + c] = 0.0;
-            }
-            std::memcpy(pix + 3 * t0, loc.data() + 3 * t0, sizeof(int) * 3 * (size_t) (t1 - t0));
-        });
-    for (auto &t : th) t.join();
+    const int rc = pcg_run(pb, pb.d_sig, pb.d_noise, tol, maxiter, 0, x, hist.data(), &r);
+    down.join();
+    MGPU_RET(rc);
+    if (rc_dn != MGPU_OK) {
+        set_error("%s", err_dn.c_str());
+        return rc_dn;
+    }
     r.lcount = lcount;
     r.trash_pix = trash;
+    if (pb.prof.on) {
+        MGPU_HIP(hipStreamSynchronize(s));
+        pb.prof.collect();
+        std::lock_guard<std::mutex> g(g_kt_mu);
+        for (int i = 0; i < MGPU_NKT; i++) {
+            g_kt_ms[i] += pb.prof.ms[i];
+            g_kt_cnt[i] += pb.prof.cnt[i];
+        }
+    }
     if (outpath && ref) {
         char fn[4096];
         std::snprintf(fn, sizeof(fn), "%s/pcg_residuals_%s.dat", outpath, ref);
@@ -482,7 +586,6 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
         }
     }
     if (rep) *rep = r;
-    (void) pb;
     return MGPU_OK;
 }
 

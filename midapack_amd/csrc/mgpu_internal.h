@@ -18,13 +18,26 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <memory>
 #include <vector>
 
 #include "../../include/mappraiser_gpu.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 namespace mgpu {
+
+// roctx range over a scope, named after the reference's timer of the same span
+// (MPI_Wtime pairs in mappraiser.c / pcg_true.c), so that a rocprofv3
+// --marker-trace shows the reference's phases on the device timeline
+struct TimerRange {
+    explicit TimerRange(const char *name) { roctxRangePush(name); }
+    ~TimerRange() { roctxRangePop(); }
+    TimerRange(const TimerRange &) = delete;
+    TimerRange &operator=(const TimerRange &) = delete;
+};
 
 void set_error(const char *fmt, ...);
 
@@ -68,6 +81,20 @@ void set_error(const char *fmt, ...);
 int dev_alloc(void **p, size_t bytes, int device, size_t *cls);
 void dev_free(void *p, size_t cls, int device);
 void dev_cache_trim(int device);  // device < 0: every device
+
+// Copies between the caller's pageable host arrays and the device through a
+// process-wide pool of pinned staging buffers (hostio.hip).  h2d_staged: when
+// it returns, the host array may be reused, and work queued on s afterwards
+// sees the data.  d2h_staged: the copies start once the event `after` has
+// completed; when it returns the data has been delivered, either copied to h or, with
+// a consumer, handed to consume(offset, chunk, bytes) chunk by chunk (called
+// from several threads at once on disjoint chunks; offsets are multiples of
+// 12 MiB).
+using StageConsumer = std::function<void(size_t off, const void *chunk, size_t bytes)>;
+int h2d_staged(int device, void *d, const void *h, size_t bytes, hipStream_t s);
+int d2h_staged(int device, void *h, const void *d, size_t bytes, hipEvent_t after,
+               const StageConsumer *consume = nullptr);
+void staging_free();
 
 // buffers owned by a problem; returned to the cache in its destructor (after
 // the problem's stream has drained) or by release(), which first waits for
@@ -143,40 +170,24 @@ struct TileDesc {
 
 constexpr int kChunk = 1024;  // samples per partial-sum chunk (P epilogue / partials kernel)
 
-struct ToeplitzPlan {
+// One class of FFT tiles: the blocks of one band lambda that use one tile
+// length (a solo problem has one class; a batch of problems with different
+// bands or block-length profiles has one per (lambda, tile length)).
+struct TileClass {
     int lambda = 0;
-    int nb = 0;
-    int64_t m = 0;
-    int64_t covered = 0;             // sum of block sizes (samples past it are left unchanged)
-    std::vector<int64_t> h_start;    // nb+1 block starts
-    int64_t *d_start = nullptr;      // nb+1
-    double *d_T0 = nullptr;          // nb, T_b[0]
-    // FFT tiles
     int log2n = 0;                   // real tile length N = 2^log2n
     int L = 0;                       // useful outputs per tile, N - 2(lambda-1)
     int ntiles = 0;
-    TileDesc *d_tiles = nullptr;
+    TileDesc *d_tiles = nullptr;     // TileDesc.filter indexes this class's coefficient sets
     int nfilters = 0;
-    std::vector<int> h_block_filter; // nb
     double *d_coef64 = nullptr;      // nfilters * NC * 2  (a_k, b_k) interleaved, includes 1/NC
     float *d_coef32 = nullptr;
     double *d_tw64 = nullptr;        // NC complex: exp(-2 pi i m / NC)
     float *d_tw32 = nullptr;
-    // blocks shorter than lambda-1 are only mean-subtracted (stmm_main:912-913)
-    std::vector<int> h_short_blocks;
-    int *d_short_blocks = nullptr;
-    // bands too wide for an LDS tile: direct product over these blocks
-    std::vector<int> h_direct_blocks;
-    int *d_direct_blocks = nullptr;
-    double *d_direct_invtt = nullptr;
-    // partial sums for the per-block mean (toeplitz_wizard.c:86-95)
-    int64_t nchunks = 0;
-    double *d_partials = nullptr;    // nchunks + nb
-    double *d_mean = nullptr;        // nb: input mean per block
-    // MGPU_OFFSET_SYMMETRIC: mean of T (v - mean) per block, from the block-edge terms
-    int offset_mode = MGPU_OFFSET_REFERENCE;
-    int *d_chunk_block = nullptr;    // nchunks: first block overlapping chunk c (partial-sum epilogues)
     int variant[2] = {0, 0};         // tile kernel variant used for FP64 [0] and FP32 [1] (0 = not chosen)
+    // N = 32768 only: the coefficients in the lane order of the wave-local tile kernel
+    double *d_coefW64 = nullptr;
+    float *d_coefW32 = nullptr;
     // FP32 products (MGPU_MIXED) with a longer tile than the FP64 one, when that
     // pays off (less overlap per useful sample); log2n32 == 0: FP32 uses the FP64 tiles
     int log2n32 = 0, L32 = 0, ntiles32 = 0;
@@ -184,10 +195,44 @@ struct ToeplitzPlan {
     TileDesc *d_tiles32 = nullptr;
     float *d_coef32L = nullptr;      // nfilters * (N32/2) (a_k, b_k)
     float *d_tw32L = nullptr;
+};
+
+struct ToeplitzPlan {
+    int lambda = 0;                  // the band of every block (the largest one when bands differ)
+    bool mixed_lambda = false;       // blocks carry different bands (batched problems)
+    int nb = 0;
+    int64_t m = 0;
+    int64_t covered = 0;             // sum of block sizes (samples past it are left unchanged)
+    std::vector<int64_t> h_start;    // nb+1 block starts
+    int64_t *d_start = nullptr;      // nb+1
+    double *d_T0 = nullptr;          // nb, T_b[0]
+    int *d_block_lambda = nullptr;   // nb, band of each block
+    // FFT tiles, one set per class
+    std::vector<TileClass> cls;
+    int nfilters = 0;                // distinct filters over all classes
+    std::vector<int> h_block_filter; // nb: global filter of each block
+    // blocks shorter than lambda-1 are only mean-subtracted (stmm_main:912-913)
+    std::vector<int> h_short_blocks;
+    int *d_short_blocks = nullptr;
+    // bands too wide for an LDS tile: direct product over these blocks
+    std::vector<int> h_direct_blocks;
+    int *d_direct_blocks = nullptr;
+    double *d_direct_invtt = nullptr; // rows of every block, row b at d_inv_off[b]
+    int64_t *d_inv_off = nullptr;     // nb: offset of block b's band in d_direct_invtt
+    // partial sums for the per-block mean (toeplitz_wizard.c:86-95)
+    int64_t nchunks = 0;
+    double *d_partials = nullptr;    // nchunks + nb
+    double *d_mean = nullptr;        // nb: input mean per block
+    // MGPU_OFFSET_SYMMETRIC: mean of T (v - mean) per block, from the block-edge terms
+    int offset_mode = MGPU_OFFSET_REFERENCE;
+    int *d_chunk_block = nullptr;    // nchunks: first block overlapping chunk c (partial-sum epilogues)
     double *d_edge_part = nullptr;   // nb * 8 slice sums of the symmetric mode's output mean
-    double *d_tail = nullptr;        // nfilters * lambda: tail_f[j] = sum_{k>j} T_f[k]
+    double *d_tail = nullptr;        // tails of every filter: tail_f[j] = sum_{k>j} T_f[k], j < lambda_f
+    int64_t *d_tail_off = nullptr;   // nfilters: offset of filter f's tail in d_tail
     int *d_block_filter = nullptr;   // nb
     int *d_block_kind = nullptr;     // nb: 0 Toeplitz product, 1 short block (identity)
+    // tile variant of the products, for reports: class 0's (0 = none chosen)
+    int variant(int slot) const { return cls.empty() ? 0 : cls[0].variant[slot]; }
 };
 
 // ---------------------------------------------------------------------------
@@ -310,7 +355,7 @@ constexpr int kMapWG = 256;
 
 struct Segments {
     int nseg = 1;
-    int64_t stride = 0;       // segment r owns global pixels [r*stride, (r+1)*stride); 0: one segment
+    std::vector<int64_t> h_pix0;  // nseg+1: segment r owns global pixels [h_pix0[r], h_pix0[r+1]); empty: one segment
     int npix = -1;            // pixel count the tables describe (-1: not built)
     std::vector<int> h_q;     // nseg+1 first local pixel of each segment
     std::vector<int> h_wg0;   // nseg+1 first workgroup of each segment
@@ -455,8 +500,14 @@ int trmatvec_nparts(const Problem &pb);
 // smallest valid local pixel of every P chunk (INT32_MAX: none), on the host
 int pointing_chunk_keys(Problem &pb, std::vector<int> &key);
 
+// Block definitions and the Toeplitz set-up (defineBlocks_avg, tpltz_init).
+// invtt holds the band of every block, block b's band at the running sum of
+// the earlier blocks' bands (b * lambda when every band is lambda long).
+// block_lambda (optional): the band of each block (default: lambda for all);
+// block_log2n (optional): the FP64 tile length of each block (default: the
+// length choose_log2n picks for its band and longest block of that band).
 int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const double *d_invtt,
-                  const double *h_invtt);
+                  const double *h_invtt, const int *block_lambda = nullptr, const int *block_log2n = nullptr);
 // N^-1 on d_in (its per-block partials must already be in tp.d_partials unless
 // compute_partials) -> d_out (may not alias d_in)
 int toeplitz_apply(Problem &pb, const double *d_in, double *d_out, bool compute_partials);
@@ -514,12 +565,27 @@ struct SegReport {
     double res0 = 0.0, res = 0.0, true_res_rel = -1.0;
     int q0 = 0, q1 = 0;  // local pixel range of the segment
 };
-// h_hist: maxiter entries per segment (segment-major), may be NULL
+// h_hist: maxiter entries per segment (segment-major), may be NULL.  seg_tol /
+// seg_maxiter (optional, one entry per segment) replace tol / maxiter per
+// segment; maxiter is then their largest value.  seg_done (optional) is called
+// on the calling thread once per segment, at the first poll that finds the
+// segment stopped (its map d_x[3 q0 .. 3 q1) final, the stream idle), with its
+// report (true_res_rel: -1, the FP64 true residual comes after the loop).
+using SegDone = std::function<int(int seg, const SegReport &rep, const double *d_x)>;
 int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double tol, int maxiter,
             int stop_after, double *h_x, double *h_hist, mgpu_report *rep,
-            std::vector<SegReport> *segs = nullptr);
+            std::vector<SegReport> *segs = nullptr, const std::vector<double> *seg_tol = nullptr,
+            const std::vector<int> *seg_maxiter = nullptr, const SegDone *seg_done = nullptr);
 
 int ensure_work(Problem &pb);
+// batched solve (batch.hip) of items[all[k]], each with its grid size,
+// tolerance and iteration cap; done (optional) is called with the item index
+// as soon as that item's results are in place (the others may still iterate);
+// check_item validates one item
+int pcg_batch_impl(mgpu_batch_item *items, const std::vector<int> &all, const std::vector<int> &grid_of,
+                   const std::vector<double> &tol_of, const std::vector<int> &maxiter_of, bool on_dev,
+                   const mgpu_options *opt, const std::function<void(int)> *done);
+int check_item(const mgpu_batch_item &it, int i);
 // invtt rows [nb*lambda] (device) from the noise timestream (device), see mappraiser_gpu_noise_invtt
 int noise_invtt(Problem &pb, const double *d_noise, int nb, const int *block_sizes, int lambda, double fsamp,
                 int window_kind, double *d_invtt, double *h_params);

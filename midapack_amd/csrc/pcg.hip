@@ -197,13 +197,16 @@ __global__ __launch_bounds__(kMB) void k_update_xg(double *__restrict__ x, const
 }
 
 // dots -> g2pix, polak, res; history; stop test; gamma.  One workgroup per segment.
-// verify = 1 (mixed precision, g updated in FP32 this iteration): a passing stop
-// test does not stop the segment; it raises F_NPEND, and the host queues an
-// iteration with an FP64 refresh, whose test (verify = 0) decides.
+// kind (IT_*): where a segment's g was updated in FP32 this iteration (mixed
+// precision, the segment not refreshed), a passing stop test does not stop it:
+// it sets S_PEND (counted in F_NPEND) and the first iteration of the next block
+// refreshes that segment's residual in FP64, whose test decides.  Whether a
+// segment refreshes depends on its own state and the iteration number only,
+// so a segment runs the same iterations alone and in a batch.
 __global__ __launch_bounds__(kRB) void k_scalars(const double *__restrict__ part, int nwg,
                                                  const int *__restrict__ wg0, double *__restrict__ S,
                                                  double *__restrict__ hist, int maxiter, int *__restrict__ state,
-                                                 int nseg, int *__restrict__ flag, int verify) {
+                                                 int nseg, int *__restrict__ flag, int kind) {
     if (stopped(flag)) return;
     const int r = blockIdx.x;
     if (state[r]) return;
@@ -225,12 +228,21 @@ __global__ __launch_bounds__(kRB) void k_scalars(const double *__restrict__ part
         const int ki = (int) k;
         if (hist && ki < maxiter) hist[(int64_t) r * maxiter + ki] = sqrt(res) / sqrt(Sr[S_RES0]);
         atomicMax(&flag[F_KMAX], ki);
+        const bool pend = Sr[S_PEND] != 0.0;
+        const bool refreshed = kind == IT_REFRESH_ALL || (kind == IT_REFRESH_PENDING && pend);
+        const bool verify = kind != IT_FP64 && !refreshed;
+        bool pend_now = pend && !refreshed;
         int st = 0;
         if (res <= Sr[S_TOL2] && !verify)
             st = 1;
         else if ((Sr[S_STOPK] > 0.0 && k >= Sr[S_STOPK]) || k + 1.0 >= Sr[S_MAXK])
             st = 2;
-        if (res <= Sr[S_TOL2] && verify) flag[F_NPEND] = 1;
+        if (res <= Sr[S_TOL2] && verify) pend_now = true;
+        if (st) pend_now = false;
+        if (pend_now != pend) {
+            Sr[S_PEND] = pend_now ? 1.0 : 0.0;
+            atomicAdd(&flag[F_NPEND], pend_now ? 1 : -1);
+        }
         if (st) {
             state[r] = st;
             if (r == 0 && st == 1) flag[F_CONV0] = 1;
@@ -238,11 +250,6 @@ __global__ __launch_bounds__(kRB) void k_scalars(const double *__restrict__ part
         }
         Sr[S_GAMMA] = polak / Sr[S_G2PIXP];
     }
-}
-
-// clear one device flag (a kernel, not a memset, so that it may be captured)
-__global__ void k_clear_flag(int *__restrict__ flag, int which) {
-    if (threadIdx.x == 0) flag[which] = 0;
 }
 
 // h = h gamma + Cg
@@ -260,17 +267,23 @@ __global__ __launch_bounds__(kMB) void k_hupdate(double *__restrict__ h, const d
     }
 }
 
+// segment r skips the refresh of an IT_REFRESH_PENDING iteration unless it is pending
+__device__ __forceinline__ bool skip_refresh(const double *__restrict__ S, int kind, int r) {
+    return kind == IT_REFRESH_PENDING && S[(int64_t) r * S_NSCAL + S_PEND] == 0.0;
+}
+
 // FP64 refresh of the residual: g = b - Ax (mixed precision); state == nullptr
 // computes every segment (final true residual)
 __global__ __launch_bounds__(kMB) void k_refresh(const double *__restrict__ b, const double *__restrict__ Ax,
                                                  double *__restrict__ g, const int *__restrict__ wg_seg,
                                                  const int *__restrict__ wg_q0, const int *__restrict__ seg_q,
                                                  const int *__restrict__ state, const int *__restrict__ flag,
-                                                 int *__restrict__ nref) {
+                                                 int *__restrict__ nref, const double *__restrict__ S, int kind) {
     if (stopped(flag)) return;
     if (nref && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(nref, 1);
     const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
     if ((state && state[sp.r]) || sp.q >= sp.qend) return;
+    if (S && skip_refresh(S, kind, sp.r)) return;
     for (int c = 0; c < 3; c++) {
         const int i = 3 * sp.q + c;
         g[i] = b[i] - Ax[i];
@@ -282,10 +295,12 @@ __global__ __launch_bounds__(kMB) void k_refresh_three(const double *__restrict_
                                                        const double *__restrict__ gp, double *__restrict__ Cg,
                                                        const int *__restrict__ wg_seg, const int *__restrict__ wg_q0,
                                                        const int *__restrict__ seg_q, const int *__restrict__ state,
-                                                       double *__restrict__ part, const int *__restrict__ flag) {
+                                                       double *__restrict__ part, const int *__restrict__ flag,
+                                                       const double *__restrict__ S, int kind) {
     if (stopped(flag)) return;
     const SegPix sp = seg_pixel(wg_seg, wg_q0, seg_q);
-    if (state[sp.r]) return;
+    // segments not refreshed keep the partials of k_update (same g, same values)
+    if (state[sp.r] || skip_refresh(S, kind, sp.r)) return;
     __shared__ double red[kMB / 64];
     const int q = sp.q;
     double a = 0.0, b = 0.0, c = 0.0;
@@ -349,15 +364,17 @@ __global__ __launch_bounds__(kRB) void k_seg_norms(const double *__restrict__ pa
     }
 }
 
-// first local pixel of every segment: lower bound of r * stride in the sorted pixel ids
-__global__ void k_seg_bounds(const int *__restrict__ pixg, int npix, int64_t stride, int nseg, int *__restrict__ out) {
+// first local pixel of every segment: lower bound of its first global pixel
+// (pix0[r]) in the sorted pixel ids
+__global__ void k_seg_bounds(const int *__restrict__ pixg, int npix, const int64_t *__restrict__ pix0, int nseg,
+                             int *__restrict__ out) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r > nseg) return;
     if (r == nseg) {
         out[r] = npix;
         return;
     }
-    const int64_t key = (int64_t) r * stride;
+    const int64_t key = pix0[r];
     int lo = 0, hi = npix;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -377,18 +394,22 @@ int segments_ensure(Problem &pb) {
     const int npix = pb.pt.npix;
     if (G.npix == npix && G.d_wg_seg) return MGPU_OK;
     hipStream_t s = pb.stream;
-    const int nseg = G.stride > 0 ? G.nseg : 1;
+    const int nseg = G.h_pix0.size() > 1 ? (int) G.h_pix0.size() - 1 : 1;
     G.h_q.assign((size_t) nseg + 1, 0);
     if (nseg == 1) {
         G.h_q[1] = npix;
     } else {
         int *d_b;
+        int64_t *d_k;
         MGPU_RET(pb.arena.alloc(&d_b, (size_t) nseg + 1));
-        k_seg_bounds<<<grid_for(nseg + 1, 64), 64, 0, s>>>(pb.pt.d_pixg, npix, G.stride, nseg, d_b);
+        MGPU_RET(pb.arena.alloc(&d_k, (size_t) nseg));
+        MGPU_HIP(hipMemcpyAsync(d_k, G.h_pix0.data(), sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
+        k_seg_bounds<<<grid_for(nseg + 1, 64), 64, 0, s>>>(pb.pt.d_pixg, npix, d_k, nseg, d_b);
         MGPU_LAUNCHED();
         MGPU_HIP(hipMemcpyAsync(G.h_q.data(), d_b, sizeof(int) * (nseg + 1), hipMemcpyDeviceToHost, s));
         MGPU_HIP(hipStreamSynchronize(s));
         pb.arena.release(d_b);
+        pb.arena.release(d_k);
     }
     G.h_wg0.assign((size_t) nseg + 1, 0);
     std::vector<int> wseg, wq0;
@@ -532,7 +553,8 @@ struct GraphBlock {
 };
 
 int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double tol, int maxiter, int stop_after,
-            double *h_x, double *h_hist, mgpu_report *rep, std::vector<SegReport> *segs) {
+            double *h_x, double *h_hist, mgpu_report *rep, std::vector<SegReport> *segs,
+            const std::vector<double> *seg_tol, const std::vector<int> *seg_maxiter, const SegDone *seg_done) {
     MGPU_ARG(pb.pc.built, "pcg before precond_build");
     MGPU_ARG(maxiter >= 1, "maxiter=%d must be >= 1", maxiter);
     MGPU_RET(segments_ensure(pb));
@@ -569,16 +591,22 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     }
 
     // scalars of every segment: tol, stop_after, maxiter
+    MGPU_ARG(!seg_tol || (int) seg_tol->size() == nseg, "%zu tolerances for %d segments", seg_tol->size(), nseg);
+    MGPU_ARG(!seg_maxiter || (int) seg_maxiter->size() == nseg, "%zu iteration caps for %d segments",
+             seg_maxiter->size(), nseg);
     std::vector<double> init((size_t) S_NSCAL * nseg, 0.0);
     for (int r = 0; r < nseg; r++) {
-        init[(size_t) r * S_NSCAL + S_TOL] = tol;
+        const int mr = seg_maxiter ? (*seg_maxiter)[r] : maxiter;
+        MGPU_ARG(mr >= 1 && mr <= maxiter, "segment %d: maxiter %d outside [1, %d]", r, mr, maxiter);
+        init[(size_t) r * S_NSCAL + S_TOL] = seg_tol ? (*seg_tol)[r] : tol;
         init[(size_t) r * S_NSCAL + S_STOPK] = stop_after > 0 ? (double) stop_after : 0.0;
-        init[(size_t) r * S_NSCAL + S_MAXK] = (double) maxiter;
+        init[(size_t) r * S_NSCAL + S_MAXK] = (double) mr;
     }
     MGPU_HIP(hipMemcpyAsync(S, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, s));
     MGPU_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
     MGPU_HIP(hipMemsetAsync(flag, 0, sizeof(int) * F_NFLAGS, s));
     MGPU_HIP(hipEventRecord(pb.ev0, s));
+    std::unique_ptr<TimerRange> phase(new TimerRange("PCG initial residual"));  // pcg_true.c:75-98
 
     // g0 = P^T N^-1 (s + n) (FP64 in both modes: it is b, kept for refreshes)
     // the template part of N^-1 (PCG_GLS_templates) from here on; the two-level
@@ -617,27 +645,25 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     const int refresh = mixed ? (pb.opt.refresh_every > 0 ? pb.opt.refresh_every : 10) : 0;
     // Iterations are queued in blocks between host polls.  A graph block must
     // repeat exactly: an even length (g/gp swap parity) and, in mixed
-    // precision, a multiple of the refresh period.
+    // precision, a multiple of the refresh period; eager blocks have the same length.
+    // The block length is the same with and without graphs (polls, and so the
+    // iterations that refresh pending segments, fall on the same numbers)
     const bool use_graph = pb.opt.graph != 0 && !pb.prof.on;
-    int block = poll;
-    if (use_graph) {
-        const int period = refresh ? (refresh % 2 ? 2 * refresh : refresh) : 2;
-        block = std::max(1, (poll + period - 1) / period) * period;
-    }
+    const int period = refresh ? (refresh % 2 ? 2 * refresh : refresh) : 2;
+    const int block = std::max(1, (poll + period - 1) / period) * period;
     int launched = 0;  // iterations queued (k = 1 .. launched)
     double *g = gA, *gp = gB;
     GraphBlock gb;
 
-    // force_refresh: refresh the residual in FP64 in this iteration whatever its
-    // number (mixed precision: an FP32-updated residual met the stop test)
-    auto enqueue_iteration = [&](bool force_refresh) -> int {
+    // refresh_pending: the segments whose FP32-updated residual met the stop test
+    // (S_PEND) refresh their residual in FP64 in this iteration whatever its
+    // number (every live segment does when the number is a multiple of refresh)
+    auto enqueue_iteration = [&](bool refresh_pending) -> int {
         ++launched;
         pb.prof.tag = launched;
-        const bool do_refresh = refresh && (force_refresh || launched % refresh == 0);
-        if (force_refresh) {
-            k_clear_flag<<<1, 64, 0, s>>>(flag, F_NPEND);
-            MGPU_LAUNCHED();
-        }
+        const bool all = refresh && launched % refresh == 0;
+        const bool do_refresh = all || (refresh && refresh_pending);
+        const int kind = !mixed ? IT_FP64 : all ? IT_REFRESH_ALL : do_refresh ? IT_REFRESH_PENDING : IT_MIXED;
         std::swap(g, gp);  // pcg_true.c:140-142
         // q = A h with fused <h, q>
         if (mixed)
@@ -662,20 +688,21 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             const bool keep = pb.prof.on;
             pb.prof.on = false;  // the FP64 operator inside is accounted as a refresh
             MGPU_RET(apply_A64(pb, x, Ax, nullptr));
-            k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, g, wseg, wq0, sq, state, flag, flag + F_NREF);
+            k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, g, wseg, wq0, sq, state, flag, flag + F_NREF, S, kind);
             MGPU_LAUNCHED();
             if (two) {
+                // segments not refreshed get the same Cg and partials again (same g, gp)
                 MGPU_RET(twolvl_apply(pb, g, gp, Cg, nullptr, part, false, state, flag));
             } else {
-                k_refresh_three<<<nwg, kMB, 0, s>>>(pb.pc.d_inv, g, gp, Cg, wseg, wq0, sq, state, part, flag);
+                k_refresh_three<<<nwg, kMB, 0, s>>>(pb.pc.d_inv, g, gp, Cg, wseg, wq0, sq, state, part, flag, S,
+                                                     kind);
                 MGPU_LAUNCHED();
             }
             pb.prof.on = keep;
             pb.prof.end(s);
         }
         pb.prof.begin(MGPU_KT_MAP, s);
-        k_scalars<<<nseg, kRB, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag,
-                                       mixed && !do_refresh ? 1 : 0);
+        k_scalars<<<nseg, kRB, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag, kind);
         MGPU_LAUNCHED();
         k_hupdate<<<nwg, kMB, 0, s>>>(h, Cg, wseg, wq0, sq, S, state, flag);
         MGPU_LAUNCHED();
@@ -683,17 +710,53 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         return MGPU_OK;
     };
 
+    phase.reset();  // ranges nest: the initial-residual range ends before the solve range starts
+    phase.reset(new TimerRange("PCG solve time"));  // pcg_true.c:132-204
+    auto seg_iterations = [&](int r, const std::vector<double> &hs, const std::vector<int> &hstate) {
+        // reference: k at break, or K when the loop ran out (pcg_true.c:224)
+        const int kr = (int) hs[(size_t) r * S_NSCAL + S_K];
+        return hstate[r] == 1 || (stop_after > 0 && kr >= stop_after) ? kr
+                                                                       : (seg_maxiter ? (*seg_maxiter)[r] : maxiter);
+    };
+    auto seg_report = [&](int r, const std::vector<double> &hs, const std::vector<int> &hstate) {
+        SegReport o;
+        o.iterations = seg_iterations(r, hs, hstate);
+        o.converged = hstate[r] == 1;
+        o.res0 = hs[(size_t) r * S_NSCAL + S_RES0];
+        o.res = hs[(size_t) r * S_NSCAL + S_RES];
+        o.q0 = G.h_q[r];
+        o.q1 = G.h_q[r + 1];
+        return o;
+    };
+    // segments already handed to seg_done
+    std::vector<char> reported((size_t) nseg, 0);
+    auto report_stopped = [&]() -> int {
+        std::vector<double> hs((size_t) S_NSCAL * nseg);
+        std::vector<int> hstate((size_t) nseg);
+        MGPU_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
+        MGPU_HIP(hipMemcpyAsync(hstate.data(), state, sizeof(int) * nseg, hipMemcpyDeviceToHost, s));
+        MGPU_HIP(hipStreamSynchronize(s));
+        for (int r = 0; r < nseg; r++)
+            if (hstate[r] && !reported[r]) {
+                reported[r] = 1;
+                MGPU_RET((*seg_done)(r, seg_report(r, hs, hstate), x));
+            }
+        return MGPU_OK;
+    };
+
     bool done = maxiter <= 1;  // maxiter == 1: no loop iteration (for k = 1; k < K)
-    bool pending = false;      // mixed precision: the next iteration refreshes and tests in FP64
-    bool eager = false;        // after such an iteration the g/gp parity of a captured block is lost
+    bool pending = false;      // mixed precision: some segment awaits its FP64 refresh
+    int ndone_seen = 0;        // stopped segments at the last report
     while (!done) {
         if (pending) {
-            MGPU_RET(enqueue_iteration(true));
-            eager = true;
-        } else if (gb.exec && !eager) {
+            // an eager block whose first iteration refreshes the pending segments;
+            // blocks keep their length, so polls (and refreshes) fall on the same
+            // iteration numbers whatever the other segments of a batch do
+            for (int it = 0; it < block && launched + 1 < maxiter; it++) MGPU_RET(enqueue_iteration(it == 0));
+        } else if (gb.exec) {
             MGPU_HIP(hipGraphLaunch(gb.exec, s));
             launched += block;
-        } else if (use_graph && launched > 0 && !eager) {
+        } else if (use_graph && launched > 0) {
             // the first block ran eagerly (kernel attributes); capture the second
             MGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int rc = MGPU_OK;
@@ -712,11 +775,16 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         MGPU_HIP(hipMemcpyAsync(hf, flag, sizeof(hf), hipMemcpyDeviceToHost, s));
         MGPU_HIP(hipStreamSynchronize(s));
         pb.prof.collect(hf[F_KMAX]);
-        pending = mixed && hf[F_NPEND] && !hf[F_STOP];
+        pending = mixed && hf[F_NPEND] > 0 && !hf[F_STOP];
         if (hf[F_STOP] || launched + 1 >= maxiter) done = true;
+        if (seg_done && nseg > 1 && hf[F_NDONE] > ndone_seen && !done) {
+            ndone_seen = hf[F_NDONE];
+            MGPU_RET(report_stopped());
+        }
     }
     pb.prof.tag = 0;
     pb.live_state = nullptr;
+    phase.reset();
     MGPU_HIP(hipEventRecord(pb.ev1, s));
     // results
     int hf[F_NFLAGS] = {0};
@@ -740,7 +808,8 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         const int rc = apply_A64(pb, x, Ax, nullptr);
         pb.prof.on = keep;
         MGPU_RET(rc);
-        k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, qv, wseg, wq0, sq, nullptr, pb.d_flag + F_NEVER, nullptr);
+        k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, qv, wseg, wq0, sq, nullptr, pb.d_flag + F_NEVER, nullptr, nullptr,
+                                      IT_REFRESH_ALL);
         MGPU_LAUNCHED();
         k_two_norms<<<nwg, kMB, 0, s>>>(qv, bvec, wseg, wq0, sq, part);
         MGPU_LAUNCHED();
@@ -752,14 +821,9 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             true_rel[r] = std::sqrt(hs[(size_t) r * S_NSCAL + S_TRUE_R2]) /
                           std::sqrt(hs[(size_t) r * S_NSCAL + S_TRUE_B2]);
     }
-    auto seg_iterations = [&](int r) {
-        // reference: k at break, or K when the loop ran out (pcg_true.c:224)
-        const int kr = (int) hs[(size_t) r * S_NSCAL + S_K];
-        return hstate[r] == 1 || (stop_after > 0 && kr >= stop_after) ? kr : maxiter;
-    };
     if (rep) {
         rep->converged = hstate[0] == 1;
-        rep->iterations = seg_iterations(0);
+        rep->iterations = seg_iterations(0, hs, hstate);
         rep->res0 = hs[S_RES0];
         rep->res = hs[S_RES];
         rep->n_map = n;
@@ -769,7 +833,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         rep->t_solve_ms = ms;
         rep->t_setup_ms = pb.t_create_ms + pb.t_precond_ms;
         rep->refreshes = refreshes;
-        rep->tile_variant = pb.tp.variant[mixed ? 1 : 0];
+        rep->tile_variant = pb.tp.variant(mixed ? 1 : 0);
         rep->true_res_rel = true_rel[0];
         rep->t_2lvl_ms = two ? pb.tl.build_ms : 0.0;
     }
@@ -777,15 +841,19 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         segs->assign((size_t) nseg, SegReport{});
         for (int r = 0; r < nseg; r++) {
             SegReport &o = (*segs)[r];
-            o.iterations = seg_iterations(r);
-            o.converged = hstate[r] == 1;
-            o.res0 = hs[(size_t) r * S_NSCAL + S_RES0];
-            o.res = hs[(size_t) r * S_NSCAL + S_RES];
+            o = seg_report(r, hs, hstate);
             o.true_res_rel = true_rel[r];
-            o.q0 = G.h_q[r];
-            o.q1 = G.h_q[r + 1];
         }
     }
+    // segments that stopped at the last poll (or never stopped)
+    if (seg_done)
+        for (int r = 0; r < nseg; r++)
+            if (!reported[r]) {
+                reported[r] = 1;
+                SegReport o = seg_report(r, hs, hstate);
+                o.true_res_rel = true_rel[r];
+                MGPU_RET((*seg_done)(r, o, x));
+            }
     if (d_hist) pb.arena.release(d_hist);
     return MGPU_OK;
 }

@@ -24,15 +24,23 @@ enum {
     S_MAXK,
     S_TRUE_R2,  // ||b - A x||^2 of the returned x (mixed precision)
     S_TRUE_B2,  // ||b||^2
+    S_PEND,     // mixed precision: 1 while an FP32-updated residual of this segment
+                // met the stop test and awaits its FP64 refresh
     S_NSCAL = 16
 };
 
 // device flags: [0] every segment stopped, [1] segment 0 converged, [2] largest
 // iteration count reached, [3] number of stopped segments, [4] never set,
-// [5] a segment's FP32-updated residual met the stop test (mixed precision:
-// the next iteration refreshes the residual in FP64 and tests it), [6] FP64
-// residual refreshes performed
+// [5] number of segments whose FP32-updated residual met the stop test
+// (mixed precision: S_PEND set; the first iteration of the next block
+// refreshes their residual in FP64 and tests it), [6] FP64 residual refreshes
+// performed
 enum { F_STOP = 0, F_CONV0, F_KMAX, F_NDONE, F_NEVER = 4, F_NPEND = 5, F_NREF = 6, F_NFLAGS = 8 };
+
+// iteration kinds of k_scalars / k_refresh: FP64 solve; mixed precision without
+// a refresh; every live segment refreshed in FP64 (every refresh_every
+// iterations); only the segments with S_PEND set refreshed
+enum { IT_FP64 = 0, IT_MIXED = 1, IT_REFRESH_ALL = 2, IT_REFRESH_PENDING = 3 };
 
 constexpr int kMB = kMapWG;  // map-domain workgroup size (pixels)
 

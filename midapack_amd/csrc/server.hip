@@ -5,13 +5,15 @@
 // returns) enter a FIFO queue.  A dispatcher thread admits the queue head as
 // soon as a slot is free and its estimated device footprint fits the HBM
 // budget that running solves leave; with it, it admits every later queued
-// request that shares its solve key (lambda, Toeplitz tile length, grid,
-// tolerance, iteration cap) and still fits, up to max_batch.  An admitted
-// group runs as one segmented batch solve (batch.hip) on its own stream and
-// host thread, so solves admitted at different times overlap on the device;
-// when it finishes, its memory and slot return to the pool and the queue is
-// scanned again.  Each result is bit-identical to the request's solo solve:
-// batching changes neither the kernels nor the summation order of an item.
+// request that still fits, up to max_batch, whatever their sizes, bands,
+// grids, tolerances and iteration caps.  An admitted group runs as one
+// segmented batch solve (batch.hip) on its own stream and host thread, so
+// solves admitted at different times overlap on the device; each request is
+// answered as soon as its own segment has stopped (the rest of its group
+// iterates on), and when the group finishes its memory and slot return to the
+// pool and the queue is scanned again.  Each result is bit-identical to the
+// request's solo solve: batching changes neither the kernels nor the
+// summation order of an item.
 #include "mgpu_internal.h"
 
 #include <algorithm>
@@ -34,7 +36,6 @@ struct Request {
     double tol = 1e-8;
     int maxiter = 500;
     int grid = 0;
-    int lambda = 0, tile_log2 = 0;
     int64_t bytes = 0;
     Clock::time_point t_submit, t_start, t_done;
     bool done = false;
@@ -57,6 +58,7 @@ struct Server {
     std::list<Worker> workers;
     int64_t budget = 0, used = 0;
     int max_slots = 4, max_batch = 16, running = 0;
+    int device = 0;  // the device current when the server was created
     long long next_ticket = 1;
     bool stop = false;
     mgpu_options opt{};
@@ -81,22 +83,50 @@ static int64_t request_bytes(const mgpu_batch_item &it, int grid, bool mixed) {
 }
 
 void Server::run_group(std::vector<Request *> grp, Worker *self) {
-    std::vector<mgpu_batch_item> items(grp.size());
-    for (size_t i = 0; i < grp.size(); i++) items[i] = *grp[i]->item;
+    const int n = (int) grp.size();
+    std::vector<mgpu_batch_item> items((size_t) n);
+    std::vector<int> all((size_t) n), grid((size_t) n), mx((size_t) n);
+    std::vector<double> tl((size_t) n);
+    for (int i = 0; i < n; i++) {
+        items[i] = *grp[i]->item;
+        all[i] = i;
+        grid[i] = grp[i]->grid;
+        tl[i] = grp[i]->tol;
+        mx[i] = grp[i]->maxiter;
+    }
     const Clock::time_point t0 = Clock::now();
     for (Request *r : grp) r->t_start = t0;
-    int rc = mappraiser_gpu_pcg_batch((int) items.size(), items.data(), grp[0]->grid, 0, grp[0]->tol,
-                                      grp[0]->maxiter, &opt);
+    // a request is answered as soon as its own segment has stopped
+    const std::function<void(int)> done = [&](int i) {
+        const Clock::time_point t = Clock::now();
+        std::lock_guard<std::mutex> g(mu);
+        Request *r = grp[i];
+        r->item->rep = items[i].rep;
+        r->status = MGPU_OK;
+        r->t_done = t;
+        r->done = true;
+        cv.notify_all();
+    };
+    int rc = MGPU_OK;
+    if (hipSetDevice(device) != hipSuccess) {
+        rc = MGPU_EHIP;
+        set_error("hipSetDevice(%d) failed in a server worker", device);
+    } else {
+        for (int i = 0; i < n && rc == MGPU_OK; i++) rc = check_item(items[i], i);
+        if (rc == MGPU_OK) rc = pcg_batch_impl(items.data(), all, grid, tl, mx, false, &opt, &done);
+    }
     std::string err = rc == MGPU_OK ? std::string() : std::string(mappraiser_gpu_last_error());
     const Clock::time_point t1 = Clock::now();
     std::lock_guard<std::mutex> g(mu);
-    for (size_t i = 0; i < grp.size(); i++) {
+    for (int i = 0; i < n; i++) {
         Request *r = grp[i];
-        r->item->rep = items[i].rep;
-        r->status = rc;
-        r->err = err;
-        r->t_done = t1;
-        r->done = true;
+        if (!r->done) {  // failed solves answer every request of the group still waiting
+            r->item->rep = items[i].rep;
+            r->status = rc;
+            r->err = err;
+            r->t_done = t1;
+            r->done = true;
+        }
         used -= r->bytes;
     }
     running--;
@@ -122,7 +152,7 @@ void Server::run_dispatcher() {
             cv.wait(lk);
             continue;
         }
-        // admit the head and every later request with the same solve key that fits
+        // admit the head and every later request that fits
         std::vector<Request *> grp;
         Request *head = pending.front();
         pending.pop_front();
@@ -130,9 +160,7 @@ void Server::run_dispatcher() {
         used += head->bytes;
         for (auto it = pending.begin(); it != pending.end() && (int) grp.size() < max_batch;) {
             Request *r = *it;
-            const bool same = r->lambda == head->lambda && r->tile_log2 == head->tile_log2 &&
-                              r->grid == head->grid && r->tol == head->tol && r->maxiter == head->maxiter;
-            if (same && used + r->bytes <= budget) {
+            if (used + r->bytes <= budget) {
                 grp.push_back(r);
                 used += r->bytes;
                 it = pending.erase(it);
@@ -179,6 +207,7 @@ int mappraiser_gpu_server_create(mgpu_server **out, double hbm_budget_gb, int ma
     }
     s->max_slots = max_slots;
     s->max_batch = max_batch;
+    MGPU_HIP(hipGetDevice(&s->device));
     Server *raw = s.get();
     s->dispatcher = std::thread(&Server::run_dispatcher, raw);
     *out = reinterpret_cast<mgpu_server *>(s.release());
@@ -191,15 +220,11 @@ int mappraiser_gpu_server_submit(mgpu_server *h, mgpu_batch_item *item, int grid
     MGPU_ARG(grid_npix > 0 && maxiter >= 1, "grid_npix=%d / maxiter=%d", grid_npix, maxiter);
     MGPU_ARG(item->local_blocks_sizes && item->nb > 0 && item->m > 0, "invalid request");
     Server &s = *reinterpret_cast<Server *>(h);
-    int64_t nmax = 0;
-    for (int b = 0; b < item->nb; b++) nmax = std::max<int64_t>(nmax, item->local_blocks_sizes[b]);
     std::unique_ptr<Request> r(new Request());
     r->item = item;
     r->tol = tol;
     r->maxiter = maxiter;
     r->grid = grid_npix;
-    r->lambda = item->lambda;
-    r->tile_log2 = toeplitz_tile_log2(item->lambda, nmax, s.opt.tile_log2);
     r->bytes = request_bytes(*item, grid_npix, s.opt.precision == MGPU_MIXED);
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(s.mu);

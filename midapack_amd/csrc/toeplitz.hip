@@ -548,6 +548,9 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV, MINW) void k_tiles(TileArgs a
     const int lo = a.lambda - 1, hi = a.lambda - 1 + a.L;
     if (interior) {
         // tile sample i is an output iff (unsigned) (i - lo) < L; it lies inside the block
+        // Whether a slot (b, r) holds outputs is decided for the whole workgroup
+        // (its samples span [2(bNT + r NC/RAD0), +2NT)): inside the output
+        // range every lane stores both values without a per-lane test
         R *__restrict__ zt = zb + rel0;
         const unsigned L = (unsigned) a.L;
 #pragma unroll
@@ -556,8 +559,14 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV, MINW) void k_tiles(TileArgs a
 #pragma unroll
             for (int r = 0; r < RAD0; r++) {
                 const int i0 = 2 * (j + r * (NC / RAD0));
-                if ((unsigned) (i0 - lo) < L) zt[i0] = re[b * RAD0 + r] - mean_out;
-                if ((unsigned) (i0 + 1 - lo) < L) zt[i0 + 1] = -im[b * RAD0 + r] - mean_out;
+                const int s_lo = 2 * (b * NT + r * (NC / RAD0)), s_hi = s_lo + 2 * NT;  // uniform
+                if (s_lo >= lo && s_hi <= hi) {
+                    zt[i0] = re[b * RAD0 + r] - mean_out;
+                    zt[i0 + 1] = -im[b * RAD0 + r] - mean_out;
+                } else if (s_hi > lo && s_lo < hi) {
+                    if ((unsigned) (i0 - lo) < L) zt[i0] = re[b * RAD0 + r] - mean_out;
+                    if ((unsigned) (i0 + 1 - lo) < L) zt[i0 + 1] = -im[b * RAD0 + r] - mean_out;
+                }
             }
         }
     } else {
@@ -576,6 +585,294 @@ __global__ __launch_bounds__((1 << LOGNC) >> LOGV, MINW) void k_tiles(TileArgs a
         }
     }
     (void) NP;
+}
+
+// ---------------------------------------------------------------------------
+// N = 32768 tiles with wave-local inner passes (k_tiles_wl)
+// ---------------------------------------------------------------------------
+// NC = 16384 complex points on 512 lanes of 32 values.  Forward transform:
+//   A   lane t: 32-point DFT over r of z[t + 512 r], then twiddle w_NC^(t k1)
+//   E1  workgroup exchange: wave w gathers Y[k1][t] (all t) for its four k1
+//   B1  lane (k1, t1): 32-point DFT over t2 of Y[k1][t1 + 16 t2], then twiddle
+//       w_512^(t1 k2a)
+//   E2  exchange inside the wave: the lane gathers its two (k1, k2a) columns
+//   B2  16-point DFT over t1: X[k1 + 32 k2a + 1024 k2b], k2b = 0..15
+// A lane's two columns are each other's mirror (X[k], X[NC - k]), so the
+// real-FFT untangling and the spectral step run in registers as in k_tiles.
+// The inverse runs B2, E2, B1, E1, A backwards (twiddles before the DFTs,
+// conj trick).  Only E1 synchronises the workgroup: between the two E1
+// exchanges of a tile the waves run independently, so one wave's LDS traffic
+// overlaps another's arithmetic on the same SIMD.  The spectral coefficients
+// are read from a copy permuted into the lanes' order (k_coef_wl): coalesced.
+constexpr int kWlNC = 16384, kWlNT = 512;
+constexpr int kWlPlaneStride = 33;                // E1: element (t, k1) at t * 33 + k1
+constexpr int kWlRow = 17, kWlCol = 32 * kWlRow;  // E2: (kidx, k2a, t1) at kidx * 544 + k2a * 17 + t1
+constexpr int kWlRegion = 4 * kWlCol;             // elements of one wave's E2 region
+constexpr int kWlLdsElems = 8 * kWlRegion;        // >= 512 * 33, the E1 plane
+
+// the four k1 of wave w (wave 0: 0, 16, 1, 31; wave j: 2j, 32-2j, 2j+1, 31-2j)
+__device__ __forceinline__ int wl_k1(int w, int kidx) {
+    if (w == 0) return kidx == 0 ? 0 : kidx == 1 ? 16 : kidx == 2 ? 1 : 31;
+    const int a = 2 * w, b = 2 * w + 1;
+    return kidx == 0 ? a : kidx == 1 ? 32 - a : kidx == 2 ? b : 32 - b;
+}
+
+// the two mirrored columns (kidx in the wave's k1 list, k2a) of lane l of wave w;
+// lane 63 of wave 0 holds the two self-mirrored columns (0, 0) and (0, 16)
+__device__ __forceinline__ void wl_pair(int w, int l, int &kp, int &ap, int &kq, int &aq) {
+    if (w > 0) {
+        const int h = l >> 5, c = l & 31;
+        kp = 2 * h;          // k1 = a (h = 0) or b (h = 1)
+        kq = 2 * h + 1;      // 32 - a or 32 - b
+        ap = c;
+        aq = 31 - c;
+    } else if (l < 32) {     // k1 = 1 and its mirror 31
+        kp = 2;
+        kq = 3;
+        ap = l;
+        aq = 31 - l;
+    } else if (l < 48) {     // k1 = 16 with itself
+        kp = kq = 1;
+        ap = l - 32;
+        aq = 31 - (l - 32);
+    } else if (l < 63) {     // k1 = 0 with itself
+        kp = kq = 0;
+        ap = l - 47;
+        aq = 32 - (l - 47);
+    } else {
+        kp = kq = 0;
+        ap = 0;
+        aq = 16;
+    }
+}
+
+// spectral index k of the i-th coefficient of lane tid (i < 16: first column,
+// k2b = i; i >= 16: second column, k2b = i - 16)
+__device__ __forceinline__ int wl_coef_k(int tid, int i) {
+    int kp, ap, kq, aq;
+    wl_pair(tid >> 6, tid & 63, kp, ap, kq, aq);
+    const int w = tid >> 6;
+    return i < 16 ? wl_k1(w, kp) + 32 * ap + 1024 * i : wl_k1(w, kq) + 32 * aq + 1024 * (i - 16);
+}
+
+// coefficient tables of every filter in the lanes' order: entry i * 512 + tid
+__global__ void k_coef_wl(const double *__restrict__ c64, int nf, double *__restrict__ w64, float *__restrict__ w32) {
+    const int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t) nf * kWlNC) return;
+    const int f = (int) (idx / kWlNC), j = (int) (idx % kWlNC);
+    const int i = j / kWlNT, tid = j % kWlNT;
+    const int64_t src = ((int64_t) f * kWlNC + wl_coef_k(tid, i)) * 2;
+    w64[2 * idx] = c64[src];
+    w64[2 * idx + 1] = c64[src + 1];
+    w32[2 * idx] = (float) c64[src];
+    w32[2 * idx + 1] = (float) c64[src + 1];
+}
+
+// v[k] *= w^k for k = 1..N-1, w = tw[e] (recurrence, as in pass_compute)
+template <typename R, int N>
+__device__ __forceinline__ void wl_twiddle(R *re, R *im, Cx<R> w1) {
+    R wr = w1.re, wi = w1.im;
+#pragma unroll
+    for (int k = 1; k < N; k++) {
+        const R xr = re[k], xi = im[k];
+        re[k] = fma(xr, wr, -(xi * wi));
+        im[k] = fma(xr, wi, xi * wr);
+        if (k + 1 < N) {
+            const R nr = fma(wr, w1.re, -(wi * w1.im));
+            const R ni = fma(wr, w1.im, wi * w1.re);
+            wr = nr;
+            wi = ni;
+        }
+    }
+}
+
+// keep the compiler from moving LDS accesses across a point inside a wave
+__device__ __forceinline__ void wl_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename R, int MINW>
+__global__ __launch_bounds__(kWlNT, MINW) void k_tiles_wl(TileArgs a) {
+    constexpr int NC = kWlNC, NT = kWlNT;
+    constexpr bool SCHED_FENCE = sizeof(R) == 8;
+    extern __shared__ unsigned char smem_raw[];
+    R *lds = reinterpret_cast<R *>(smem_raw);
+
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const TileDesc d = a.tiles[blockIdx.x];
+    if (a.state) {  // the tile's segment has stopped (uniform over the workgroup)
+        const int sg = a.block_seg[d.block];
+        if (sg >= 0 && a.state[sg]) return;
+    }
+    const R *__restrict__ y = reinterpret_cast<const R *>(a.y);
+    R *__restrict__ z = reinterpret_cast<R *>(a.z);
+    const Cx<R> *__restrict__ tw = reinterpret_cast<const Cx<R> *>(a.tw);
+    const Cx<R> *__restrict__ cw = reinterpret_cast<const Cx<R> *>(a.coef) + (int64_t) d.filter * NC + t;
+    const R mean = (R) a.mean[d.block];
+    const R mean_out = (R) mean_out_of(a.edge_part, d.block, d.blk_hi - d.blk_lo);
+    const int64_t in0 = d.out_lo - (a.lambda - 1);
+    const R *__restrict__ yb = y + d.blk_lo;
+    R *__restrict__ zb = z + d.blk_lo;
+    const int nblk = (int) (d.blk_hi - d.blk_lo);
+    const int rel0 = (int) (in0 - d.blk_lo);
+    const bool interior = rel0 >= 0 && rel0 + 2 * NC <= nblk;
+
+    // lane roles: (kidx, t1) in B1 (its two columns in B2 are set before E2)
+    const int kidx = l >> 4, t1 = l & 15;
+    R *__restrict__ reg = lds + w * kWlRegion;
+
+    R re[32], im[32];
+    // ---- A: z[n] = x[2n] + i x[2n+1], n = t + 512 r ----
+    if (interior) {
+        const R *__restrict__ yt = yb + rel0;
+#pragma unroll
+        for (int r = 0; r < 32; r++) {
+            const int n = t + r * NT;
+            re[r] = yt[2 * n] - mean;
+            im[r] = yt[2 * n + 1] - mean;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 32; r++) {
+            const int n = t + r * NT;
+            const int i0 = rel0 + 2 * n, i1 = i0 + 1;
+            const R v0 = yb[(unsigned) min(max(i0, 0), nblk - 1)];
+            const R v1 = yb[(unsigned) min(max(i1, 0), nblk - 1)];
+            re[r] = ((unsigned) i0 < (unsigned) nblk) ? v0 - mean : (R) 0;
+            im[r] = ((unsigned) i1 < (unsigned) nblk) ? v1 - mean : (R) 0;
+        }
+    }
+    if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+    dft<R, 32>(re, im);
+    if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+    wl_twiddle<R, 32>(re, im, tw[t]);
+    // ---- E1 ----
+    const int k1 = wl_k1(w, kidx);
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        R *v = p ? im : re;
+#pragma unroll
+        for (int q = 0; q < 32; q++) lds[t * kWlPlaneStride + q] = v[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 32; q++) v[q] = lds[(t1 + 16 * q) * kWlPlaneStride + k1];
+        __syncthreads();
+    }
+    // ---- B1 ----
+    dft<R, 32>(re, im);
+    if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+    wl_twiddle<R, 32>(re, im, tw[32 * t1]);
+    // ---- E2 (inside the wave) ----
+    int kp, ap, kq, aq;
+    wl_pair(w, l, kp, ap, kq, aq);
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        R *v = p ? im : re;
+#pragma unroll
+        for (int q = 0; q < 32; q++) reg[kidx * kWlCol + q * kWlRow + t1] = v[q];
+        wl_wave_sync();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            v[q] = reg[kp * kWlCol + ap * kWlRow + q];
+            v[16 + q] = reg[kq * kWlCol + aq * kWlRow + q];
+        }
+        wl_wave_sync();
+    }
+    // ---- B2 and the spectral step ----
+    dft<R, 16>(re, im);
+    dft<R, 16>(re + 16, im + 16);
+    if (w != 0 || l != 63) {
+        // P[k2b] (k) mirrors Q[15 - k2b] (NC - k); coefficients 4 pairs at a time
+#pragma unroll
+        for (int g = 0; g < 16; g += 4) {
+            Cx<R> cp[4], cq[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                cp[u] = cw[(g + u) * NT];
+                cq[u] = cw[(16 + 15 - (g + u)) * NT];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int b = g + u;
+                spectral_pair<R>(re[b], im[b], re[16 + 15 - b], im[16 + 15 - b], cp[u], cq[u]);
+            }
+            if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+        // (0, 0): k = 1024 k2b, k2b <-> 16 - k2b; (0, 16): k = 512 + 1024 k2b, k2b <-> 15 - k2b
+        spectral_self<R>(re[0], im[0], cw[0]);
+        spectral_self<R>(re[8], im[8], cw[8 * NT]);
+#pragma unroll
+        for (int b = 1; b < 8; b++)
+            spectral_pair<R>(re[b], im[b], re[16 - b], im[16 - b], cw[b * NT], cw[(16 - b) * NT]);
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+            spectral_pair<R>(re[16 + b], im[16 + b], re[31 - b], im[31 - b], cw[(16 + b) * NT], cw[(31 - b) * NT]);
+    }
+    // ---- inverse B2: 16-point DFTs over k2b ----
+    dft<R, 16>(re, im);
+    dft<R, 16>(re + 16, im + 16);
+    // ---- inverse E2 ----
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        R *v = p ? im : re;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            reg[kp * kWlCol + ap * kWlRow + q] = v[q];
+            reg[kq * kWlCol + aq * kWlRow + q] = v[16 + q];
+        }
+        wl_wave_sync();
+#pragma unroll
+        for (int q = 0; q < 32; q++) v[q] = reg[kidx * kWlCol + q * kWlRow + t1];
+        wl_wave_sync();
+    }
+    // ---- inverse B1: twiddle, 32-point DFT over k2a ----
+    wl_twiddle<R, 32>(re, im, tw[32 * t1]);
+    if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+    dft<R, 32>(re, im);
+    // ---- inverse E1 (every wave has finished reading its E2 region) ----
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        R *v = p ? im : re;
+#pragma unroll
+        for (int q = 0; q < 32; q++) lds[(t1 + 16 * q) * kWlPlaneStride + k1] = v[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 32; q++) v[q] = lds[t * kWlPlaneStride + q];
+        if (p == 0) __syncthreads();
+    }
+    // ---- inverse A: twiddle, 32-point DFT over k1 -> conj(y) at n = t + 512 r ----
+    wl_twiddle<R, 32>(re, im, tw[t]);
+    if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+    dft<R, 32>(re, im);
+    const int lo = a.lambda - 1, hi = a.lambda - 1 + a.L;
+    if (interior) {
+        R *__restrict__ zt = zb + rel0;
+        const unsigned L = (unsigned) a.L;
+#pragma unroll
+        for (int r = 0; r < 32; r++) {
+            const int i0 = 2 * (t + r * NT);
+            const int s_lo = 2 * r * NT, s_hi = s_lo + 2 * NT;  // samples of slot r (uniform)
+            if (s_lo >= lo && s_hi <= hi) {
+                zt[i0] = re[r] - mean_out;
+                zt[i0 + 1] = -im[r] - mean_out;
+            } else if (s_hi > lo && s_lo < hi) {
+                if ((unsigned) (i0 - lo) < L) zt[i0] = re[r] - mean_out;
+                if ((unsigned) (i0 + 1 - lo) < L) zt[i0 + 1] = -im[r] - mean_out;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 32; r++) {
+            const int i0 = 2 * (t + r * NT);
+            const int o0 = rel0 + i0;
+            if (i0 >= lo && i0 < hi && o0 < nblk) zb[(unsigned) o0] = re[r] - mean_out;
+            if (i0 + 1 >= lo && i0 + 1 < hi && o0 + 1 < nblk) zb[(unsigned) (o0 + 1)] = -im[r] - mean_out;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -646,8 +943,10 @@ __global__ __launch_bounds__(256) void k_block_mean(const double *__restrict__ p
                                                     const int64_t *__restrict__ start, int nb,
                                                     double *__restrict__ mean, const R *__restrict__ v,
                                                     const double *__restrict__ tail,
+                                                    const int64_t *__restrict__ tail_off,
                                                     const int *__restrict__ block_filter,
-                                                    const int *__restrict__ block_kind, int lambda,
+                                                    const int *__restrict__ block_kind,
+                                                    const int *__restrict__ block_lambda,
                                                     double *__restrict__ edge_part,
                                                     const int *__restrict__ block_seg, const int *__restrict__ state) {
     __shared__ double red[256];
@@ -662,8 +961,8 @@ __global__ __launch_bounds__(256) void k_block_mean(const double *__restrict__ p
     if (!edge_part) return;
     double part = 0.0;
     if (block_kind[b] == 0) {  // short blocks: identity product, output mean 0
-        const int n = (int) (hi - lo), L1 = lambda - 1;
-        const double *tl = tail + (int64_t) block_filter[b] * lambda;
+        const int n = (int) (hi - lo), L1 = block_lambda[b] - 1;
+        const double *tl = tail + tail_off[block_filter[b]];
         const int a_end = min(L1, n);
         const int b_beg = max(n - L1, a_end);
         const int tot_e = a_end + (n - b_beg);
@@ -734,11 +1033,13 @@ __global__ __launch_bounds__(256) void k_diag_mean(const R *__restrict__ y, R *_
 template <typename R>
 __global__ void k_direct(const R *__restrict__ y, R *__restrict__ z, const int64_t *__restrict__ start,
                          const int *__restrict__ blocks, const double *__restrict__ mean,
-                         const double *__restrict__ edge_part, const double *__restrict__ invtt, int lambda) {
+                         const double *__restrict__ edge_part, const double *__restrict__ invtt,
+                         const int64_t *__restrict__ inv_off, const int *__restrict__ block_lambda) {
     const int b = blocks[blockIdx.y];
     const int64_t lo = start[b], hi = start[b + 1];
     const double mu = mean[b], mo = mean_out_of(edge_part, b, hi - lo);
-    const double *T = invtt + (int64_t) b * lambda;
+    const double *T = invtt + inv_off[b];
+    const int lambda = block_lambda[b];
     for (int64_t i = lo + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < hi;
          i += (int64_t) gridDim.x * blockDim.x) {
         const int64_t j0 = max(lo, i - (lambda - 1)), j1 = min(hi, i + lambda);
@@ -751,12 +1052,12 @@ __global__ void k_direct(const R *__restrict__ y, R *__restrict__ z, const int64
 // ---------------------------------------------------------------------------
 // filter spectra: H = FFT_N(T_circ) (toeplitz.c:441-470), then (a_k, b_k)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_circ_fft(const double *__restrict__ invtt, const int *__restrict__ src_block,
+__global__ __launch_bounds__(1024) void k_circ_fft(const double *__restrict__ invtt, const int64_t *__restrict__ src_off,
                                                     int lambda, int log2n, double2 *__restrict__ buf) {
     const int f = blockIdx.x;
     const int N = 1 << log2n;
     double2 *a = buf + (int64_t) f * N;
-    const double *T = invtt + (int64_t) src_block[f] * lambda;
+    const double *T = invtt + src_off[f];
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         int r = (int) (__builtin_bitreverse32((unsigned) i) >> (32 - log2n));
         double v = 0.0;
@@ -860,38 +1161,82 @@ static int choose_log2n(int lambda, int64_t nmax, bool fp32, int forced) {
     return best;
 }
 
+// spectral tables of one tile length for the filters whose bands start at
+// d_inv + src_off[i] (band lambda): FFT of the circulant embedding, then the
+// (a_k, b_k) coefficients in both precisions and the twiddles
+static int class_spectra(Problem &pb, const double *d_inv, const std::vector<int64_t> &src_off, int lambda, int l2,
+                         double *c64, float *c32, double *t64, float *t32) {
+    hipStream_t s = pb.stream;
+    const int nf = (int) src_off.size();
+    const int N = 1 << l2, NC = N / 2;
+    int64_t *d_src;
+    double2 *d_buf;
+    MGPU_RET(pb.arena.alloc(&d_src, src_off.size()));
+    MGPU_HIP(hipMemcpyAsync(d_src, src_off.data(), sizeof(int64_t) * src_off.size(), hipMemcpyHostToDevice, s));
+    MGPU_RET(pb.arena.alloc(&d_buf, (size_t) nf * N));
+    k_circ_fft<<<nf, 1024, 0, s>>>(d_inv, d_src, lambda, l2, d_buf);
+    MGPU_LAUNCHED();
+    double *c64_tmp = nullptr;
+    if (!c64) MGPU_RET(pb.arena.alloc(&c64_tmp, (size_t) nf * NC * 2));
+    k_coef<<<grid_for((int64_t) nf * NC, 256), 256, 0, s>>>(d_buf, l2, nf, c64 ? c64 : c64_tmp, c32);
+    MGPU_LAUNCHED();
+    double *t64_tmp = nullptr;
+    if (!t64) MGPU_RET(pb.arena.alloc(&t64_tmp, (size_t) NC * 2));
+    k_twiddles<<<grid_for(NC, 256), 256, 0, s>>>(NC, t64 ? t64 : t64_tmp, t32);
+    MGPU_LAUNCHED();
+    MGPU_HIP(hipStreamSynchronize(s));
+    for (void *p : {(void *) d_buf, (void *) d_src, (void *) c64_tmp, (void *) t64_tmp})
+        if (p) pb.arena.release(p);
+    return MGPU_OK;
+}
+
 int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const double *d_invtt,
-                  const double *h_invtt) {
+                  const double *h_invtt, const int *block_lambda, const int *block_log2n) {
     ToeplitzPlan &T = pb.tp;
     hipStream_t s = pb.stream;
     MGPU_ARG(nb > 0, "nb_blocks=%d must be positive", nb);
     MGPU_ARG(lambda >= 1, "lambda=%d must be >= 1", lambda);
-    T.lambda = lambda;
+    // band of every block and the offset of its band in invtt
+    std::vector<int> lam((size_t) nb, lambda);
+    if (block_lambda)
+        for (int b = 0; b < nb; b++) lam[b] = block_lambda[b];
+    std::vector<int64_t> inv_off((size_t) nb + 1, 0);
+    int lmax = 0, lmin = INT32_MAX;
+    for (int b = 0; b < nb; b++) {
+        MGPU_ARG(lam[b] >= 1, "block %d: lambda=%d must be >= 1", b, lam[b]);
+        inv_off[b + 1] = inv_off[b] + lam[b];
+        lmax = std::max(lmax, lam[b]);
+        lmin = std::min(lmin, lam[b]);
+    }
+    MGPU_ARG(lmin == lmax || lmin > 1, "bands of 1 (diagonal noise) cannot be mixed with wider bands");
+    T.lambda = lmax;
+    T.mixed_lambda = lmin != lmax;
     T.nb = nb;
     T.m = pb.pt.m;
     T.h_start.assign(nb + 1, 0);
-    int64_t nmax = 0;
     for (int b = 0; b < nb; b++) {
         MGPU_ARG(block_sizes[b] > 0, "local_blocks_sizes[%d]=%d must be positive", b, block_sizes[b]);
         T.h_start[b + 1] = T.h_start[b] + block_sizes[b];
-        nmax = std::max<int64_t>(nmax, block_sizes[b]);
     }
     T.covered = T.h_start[nb];
     MGPU_ARG(T.covered <= T.m, "sum of local_blocks_sizes (%lld) exceeds m (%lld)", (long long) T.covered,
              (long long) T.m);
     MGPU_RET(pb.arena.alloc(&T.d_start, (size_t) nb + 1));
     MGPU_HIP(hipMemcpyAsync(T.d_start, T.h_start.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, s));
+    MGPU_RET(pb.arena.alloc(&T.d_block_lambda, (size_t) nb));
+    MGPU_HIP(hipMemcpyAsync(T.d_block_lambda, lam.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
     // host copy of invtt (dedupe filters, T0)
+    const int64_t ninv = inv_off[nb];
     std::vector<double> hinv;
     const double *inv = h_invtt;
     if (!inv) {
-        hinv.resize((size_t) nb * lambda);
+        hinv.resize((size_t) ninv);
         MGPU_HIP(hipMemcpyAsync(hinv.data(), d_invtt, sizeof(double) * hinv.size(), hipMemcpyDeviceToHost, s));
         MGPU_HIP(hipStreamSynchronize(s));
         inv = hinv.data();
     }
     std::vector<double> T0(nb);
-    for (int b = 0; b < nb; b++) T0[b] = inv[(int64_t) b * lambda];
+    for (int b = 0; b < nb; b++) T0[b] = inv[inv_off[b]];
     MGPU_RET(pb.arena.alloc(&T.d_T0, (size_t) nb));
     MGPU_HIP(hipMemcpyAsync(T.d_T0, T0.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
 
@@ -917,32 +1262,24 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
     T.offset_mode = pb.opt.offset_mode;
     MGPU_ARG(T.offset_mode >= MGPU_OFFSET_REFERENCE && T.offset_mode <= MGPU_OFFSET_NONE, "unknown offset_mode %d",
              T.offset_mode);
-    if (lambda == 1) {
+    if (lmax == 1) {
         MGPU_HIP(hipStreamSynchronize(s));
         return MGPU_OK;
     }
 
     const bool fp32 = pb.opt.precision == MGPU_MIXED;
-    // FP64 tiles use the FP64 limit even in mixed mode (the FP64 residual refresh needs them)
-    // a forced length longer than the FP64 limit applies to the FP32 tiles only
-    int l2 = choose_log2n(lambda, nmax, false, std::min(pb.opt.tile_log2, max_log2n(false)));
-    T.h_short_blocks.clear();
-    std::vector<int> direct_blocks;
-    if (l2 < 0 || l2 < min_log2n() || l2 > max_log2n(false)) {
-        l2 = -1;
-    }
-    T.log2n = l2;
-    // dedupe filters by content
-    std::vector<int> filter_src;
+    // dedupe filters by band and content
+    std::vector<int> filter_src;  // first block of every filter
     T.h_block_filter.assign(nb, -1);
     {
         std::unordered_map<uint64_t, std::vector<int>> by_hash;
         for (int b = 0; b < nb; b++) {
-            const double *Tb = inv + (int64_t) b * lambda;
-            // hash of <= 256 evenly spaced values (the candidates are confirmed by memcmp)
-            uint64_t h = 1469598103934665603ull;
-            const int step = lambda > 256 ? lambda / 256 : 1;
-            for (int i = 0; i < lambda; i += step) {
+            const double *Tb = inv + inv_off[b];
+            const int lb = lam[b];
+            // hash of the band and <= 256 evenly spaced values (candidates are confirmed by memcmp)
+            uint64_t h = 1469598103934665603ull ^ (uint64_t) lb;
+            const int step = lb > 256 ? lb / 256 : 1;
+            for (int i = 0; i < lb; i += step) {
                 uint64_t w;
                 std::memcpy(&w, Tb + i, sizeof(w));
                 h = (h ^ w) * 1099511628211ull;
@@ -951,7 +1288,8 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
             int fid = -1;
             auto &cand = by_hash[h];
             for (int f : cand)
-                if (std::memcmp(inv + (int64_t) filter_src[f] * lambda, Tb, sizeof(double) * lambda) == 0) {
+                if (lam[filter_src[f]] == lb &&
+                    std::memcmp(inv + inv_off[filter_src[f]], Tb, sizeof(double) * lb) == 0) {
                     fid = f;
                     break;
                 }
@@ -963,132 +1301,146 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
             T.h_block_filter[b] = fid;
         }
     }
+    T.nfilters = (int) filter_src.size();
     // block-edge tails for the symmetric offset projection: tail_f[j] = sum_{k>j} T_f[k]
     {
-        const int nf = (int) filter_src.size();
-        std::vector<double> tails((size_t) nf * lambda, 0.0);
+        const int nf = T.nfilters;
+        std::vector<int64_t> toff((size_t) nf + 1, 0);
+        for (int f = 0; f < nf; f++) toff[f + 1] = toff[f] + lam[filter_src[f]];
+        std::vector<double> tails((size_t) toff[nf], 0.0);
         for (int f = 0; f < nf; f++) {
-            const double *Tf = inv + (int64_t) filter_src[f] * lambda;
-            double *tl = tails.data() + (int64_t) f * lambda;
+            const int lf = lam[filter_src[f]];
+            const double *Tf = inv + inv_off[filter_src[f]];
+            double *tl = tails.data() + toff[f];
             double acc = 0.0;
-            for (int j = lambda - 1; j >= 0; j--) {
+            for (int j = lf - 1; j >= 0; j--) {
                 tl[j] = acc;
                 acc += Tf[j];
             }
         }
         std::vector<int> kind(nb);
-        for (int b = 0; b < nb; b++) kind[b] = block_sizes[b] < lambda - 1 ? 1 : 0;
+        for (int b = 0; b < nb; b++) kind[b] = block_sizes[b] < lam[b] - 1 ? 1 : 0;
         MGPU_RET(pb.arena.alloc(&T.d_tail, tails.size()));
+        MGPU_RET(pb.arena.alloc(&T.d_tail_off, (size_t) nf));
         MGPU_RET(pb.arena.alloc(&T.d_block_filter, (size_t) nb));
         MGPU_RET(pb.arena.alloc(&T.d_block_kind, (size_t) nb));
         MGPU_HIP(hipMemcpyAsync(T.d_tail, tails.data(), sizeof(double) * tails.size(), hipMemcpyHostToDevice, s));
+        MGPU_HIP(hipMemcpyAsync(T.d_tail_off, toff.data(), sizeof(int64_t) * nf, hipMemcpyHostToDevice, s));
         MGPU_HIP(hipMemcpyAsync(T.d_block_filter, T.h_block_filter.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
         MGPU_HIP(hipMemcpyAsync(T.d_block_kind, kind.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
         MGPU_HIP(hipStreamSynchronize(s));
     }
-    std::vector<TileDesc> tiles;
-    if (l2 > 0) {
-        const int N = 1 << l2;
-        T.L = N - 2 * (lambda - 1);
-        for (int b = 0; b < nb; b++) {
-            const int64_t n = block_sizes[b];
-            if (n < lambda - 1) {
-                T.h_short_blocks.push_back(b);
-                continue;
-            }
-            const int64_t nt = (n + T.L - 1) / T.L;
-            for (int64_t k = 0; k < nt; k++)
-                tiles.push_back(TileDesc{T.h_start[b], T.h_start[b + 1], T.h_start[b] + k * T.L, b,
-                                         T.h_block_filter[b]});
-        }
-    } else {
-        for (int b = 0; b < nb; b++) {
-            if (block_sizes[b] < lambda - 1)
-                T.h_short_blocks.push_back(b);
-            else
-                direct_blocks.push_back(b);
-        }
+    // tile length of every block: the caller's, or the choice for its band and
+    // the longest block of that band; blocks without a tile length take the direct product
+    std::map<int, int64_t> nmax_of;
+    for (int b = 0; b < nb; b++) nmax_of[lam[b]] = std::max<int64_t>(nmax_of[lam[b]], block_sizes[b]);
+    std::vector<int> l2b((size_t) nb, -1);
+    for (int b = 0; b < nb; b++) {
+        int l2 = block_log2n ? block_log2n[b]
+                             : choose_log2n(lam[b], nmax_of[lam[b]], false, std::min(pb.opt.tile_log2, max_log2n(false)));
+        if (l2 < min_log2n() || l2 > max_log2n(false) || (1 << l2) - 2 * (lam[b] - 1) < 1) l2 = -1;
+        l2b[b] = l2;
     }
-    T.ntiles = (int) tiles.size();
-    if (T.ntiles) {
-        MGPU_RET(pb.arena.alloc(&T.d_tiles, tiles.size()));
-        tiles = xcd_order(tiles);
-        MGPU_HIP(hipMemcpyAsync(T.d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice, s));
-        T.nfilters = (int) filter_src.size();
-        const int N = 1 << l2, NC = N / 2;
-        int *d_src;
-        double2 *d_buf;
-        MGPU_RET(pb.arena.alloc(&d_src, filter_src.size()));
-        MGPU_HIP(hipMemcpyAsync(d_src, filter_src.data(), sizeof(int) * filter_src.size(), hipMemcpyHostToDevice, s));
-        MGPU_RET(pb.arena.alloc(&d_buf, (size_t) T.nfilters * N));
-        // the spectra need invtt on the device
-        const double *d_inv = d_invtt;
-        double *d_inv_tmp = nullptr;
-        if (!d_inv) {
-            MGPU_RET(pb.arena.alloc(&d_inv_tmp, (size_t) nb * lambda));
-            MGPU_HIP(hipMemcpyAsync(d_inv_tmp, inv, sizeof(double) * (size_t) nb * lambda, hipMemcpyHostToDevice, s));
-            d_inv = d_inv_tmp;
+    // classes: (band, tile length) in increasing order; blocks shorter than their
+    // band are only mean-subtracted
+    T.h_short_blocks.clear();
+    std::vector<int> direct_blocks;
+    std::map<std::pair<int, int>, std::vector<int>> class_blocks;
+    for (int b = 0; b < nb; b++) {
+        if (block_sizes[b] < lam[b] - 1)
+            T.h_short_blocks.push_back(b);
+        else if (l2b[b] < 0)
+            direct_blocks.push_back(b);
+        else
+            class_blocks[std::make_pair(lam[b], l2b[b])].push_back(b);
+    }
+    MGPU_ARG(direct_blocks.empty() || !T.mixed_lambda,
+             "bands wider than a tile (direct product) cannot be batched with other bands");
+    // the spectra need invtt on the device
+    const double *d_inv = d_invtt;
+    double *d_inv_tmp = nullptr;
+    if (!d_inv && !class_blocks.empty()) {
+        MGPU_RET(pb.arena.alloc(&d_inv_tmp, (size_t) ninv));
+        MGPU_HIP(hipMemcpyAsync(d_inv_tmp, inv, sizeof(double) * (size_t) ninv, hipMemcpyHostToDevice, s));
+        d_inv = d_inv_tmp;
+    }
+    T.cls.clear();
+    for (auto &kv : class_blocks) {
+        T.cls.emplace_back();
+        TileClass &C = T.cls.back();
+        C.lambda = kv.first.first;
+        C.log2n = kv.first.second;
+        const std::vector<int> &bl = kv.second;
+        const int N = 1 << C.log2n, NC = N / 2;
+        C.L = N - 2 * (C.lambda - 1);
+        // the class's filters, numbered in order of first use
+        std::unordered_map<int, int> local;
+        std::vector<int64_t> src_off;
+        int64_t nmax_c = 0;
+        for (int b : bl) {
+            nmax_c = std::max<int64_t>(nmax_c, block_sizes[b]);
+            const int f = T.h_block_filter[b];
+            if (!local.count(f)) {
+                local[f] = (int) src_off.size();
+                src_off.push_back(inv_off[filter_src[f]]);
+            }
         }
-        k_circ_fft<<<T.nfilters, 1024, 0, s>>>(d_inv, d_src, lambda, l2, d_buf);
-        MGPU_LAUNCHED();
-        MGPU_RET(pb.arena.alloc(&T.d_coef64, (size_t) T.nfilters * NC * 2));
-        MGPU_RET(pb.arena.alloc(&T.d_coef32, (size_t) T.nfilters * NC * 2));
-        k_coef<<<grid_for((int64_t) T.nfilters * NC, 256), 256, 0, s>>>(d_buf, l2, T.nfilters, T.d_coef64,
-                                                                          T.d_coef32);
-        MGPU_LAUNCHED();
-        MGPU_RET(pb.arena.alloc(&T.d_tw64, (size_t) NC * 2));
-        MGPU_RET(pb.arena.alloc(&T.d_tw32, (size_t) NC * 2));
-        k_twiddles<<<grid_for(NC, 256), 256, 0, s>>>(NC, T.d_tw64, T.d_tw32);
-        MGPU_LAUNCHED();
+        C.nfilters = (int) src_off.size();
+        std::vector<TileDesc> tiles;
+        for (int b : bl) {
+            const int64_t nt = (block_sizes[b] + C.L - 1) / C.L;
+            for (int64_t k = 0; k < nt; k++)
+                tiles.push_back(TileDesc{T.h_start[b], T.h_start[b + 1], T.h_start[b] + k * C.L, b,
+                                         local[T.h_block_filter[b]]});
+        }
+        C.ntiles = (int) tiles.size();
+        MGPU_RET(pb.arena.alloc(&C.d_tiles, tiles.size()));
+        tiles = xcd_order(tiles);
+        MGPU_HIP(hipMemcpyAsync(C.d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice, s));
+        MGPU_RET(pb.arena.alloc(&C.d_coef64, (size_t) C.nfilters * NC * 2));
+        MGPU_RET(pb.arena.alloc(&C.d_coef32, (size_t) C.nfilters * NC * 2));
+        MGPU_RET(pb.arena.alloc(&C.d_tw64, (size_t) NC * 2));
+        MGPU_RET(pb.arena.alloc(&C.d_tw32, (size_t) NC * 2));
+        MGPU_RET(class_spectra(pb, d_inv, src_off, C.lambda, C.log2n, C.d_coef64, C.d_coef32, C.d_tw64, C.d_tw32));
+        if (NC == kWlNC) {  // the wave-local tile kernel reads the coefficients in its lanes' order
+            MGPU_RET(pb.arena.alloc(&C.d_coefW64, (size_t) C.nfilters * NC * 2));
+            MGPU_RET(pb.arena.alloc(&C.d_coefW32, (size_t) C.nfilters * NC * 2));
+            k_coef_wl<<<grid_for((int64_t) C.nfilters * NC, 256), 256, 0, s>>>(C.d_coef64, C.nfilters, C.d_coefW64,
+                                                                                 C.d_coefW32);
+            MGPU_LAUNCHED();
+        }
         // FP32 products of a mixed solve: a longer tile when it costs fewer FFT
         // operations per useful sample (FP32 halves the registers and LDS a tile needs)
-        const int l2f = fp32 ? choose_log2n(lambda, nmax, true, pb.opt.tile_log2) : -1;
-        if (l2f > l2 && l2f <= max_log2n(true)) {
+        const int l2f = fp32 ? choose_log2n(C.lambda, nmax_c, true, pb.opt.tile_log2) : -1;
+        if (l2f > C.log2n && l2f <= max_log2n(true)) {
             const int Nf = 1 << l2f, NCf = Nf / 2;
-            T.log2n32 = l2f;
-            T.L32 = Nf - 2 * (lambda - 1);
+            C.log2n32 = l2f;
+            C.L32 = Nf - 2 * (C.lambda - 1);
             std::vector<TileDesc> tf;
-            for (int b = 0; b < nb; b++) {
-                const int64_t n = block_sizes[b];
-                if (n < lambda - 1) continue;
-                const int64_t nt = (n + T.L32 - 1) / T.L32;
+            for (int b : bl) {
+                const int64_t nt = (block_sizes[b] + C.L32 - 1) / C.L32;
                 for (int64_t k = 0; k < nt; k++)
-                    tf.push_back(TileDesc{T.h_start[b], T.h_start[b + 1], T.h_start[b] + k * T.L32, b,
-                                          T.h_block_filter[b]});
+                    tf.push_back(TileDesc{T.h_start[b], T.h_start[b + 1], T.h_start[b] + k * C.L32, b,
+                                          local[T.h_block_filter[b]]});
             }
-            T.ntiles32 = (int) tf.size();
-            MGPU_RET(pb.arena.alloc(&T.d_tiles32, tf.size()));
+            C.ntiles32 = (int) tf.size();
+            MGPU_RET(pb.arena.alloc(&C.d_tiles32, tf.size()));
             tf = xcd_order(tf);
-            MGPU_HIP(hipMemcpyAsync(T.d_tiles32, tf.data(), sizeof(TileDesc) * tf.size(), hipMemcpyHostToDevice, s));
-            double2 *d_bf;
-            double *d_c64;
-            double *d_t64;
-            MGPU_RET(pb.arena.alloc(&d_bf, (size_t) T.nfilters * Nf));
-            MGPU_RET(pb.arena.alloc(&d_c64, (size_t) T.nfilters * NCf * 2));
-            MGPU_RET(pb.arena.alloc(&d_t64, (size_t) NCf * 2));
-            MGPU_RET(pb.arena.alloc(&T.d_coef32L, (size_t) T.nfilters * NCf * 2));
-            MGPU_RET(pb.arena.alloc(&T.d_tw32L, (size_t) NCf * 2));
-            k_circ_fft<<<T.nfilters, 1024, 0, s>>>(d_inv, d_src, lambda, l2f, d_bf);
-            MGPU_LAUNCHED();
-            k_coef<<<grid_for((int64_t) T.nfilters * NCf, 256), 256, 0, s>>>(d_bf, l2f, T.nfilters, d_c64,
-                                                                               T.d_coef32L);
-            MGPU_LAUNCHED();
-            k_twiddles<<<grid_for(NCf, 256), 256, 0, s>>>(NCf, d_t64, T.d_tw32L);
-            MGPU_LAUNCHED();
-            MGPU_HIP(hipStreamSynchronize(s));
-            for (void *p : {(void *) d_bf, (void *) d_c64, (void *) d_t64}) pb.arena.release(p);
+            MGPU_HIP(hipMemcpyAsync(C.d_tiles32, tf.data(), sizeof(TileDesc) * tf.size(), hipMemcpyHostToDevice, s));
+            MGPU_RET(pb.arena.alloc(&C.d_coef32L, (size_t) C.nfilters * NCf * 2));
+            MGPU_RET(pb.arena.alloc(&C.d_tw32L, (size_t) NCf * 2));
+            MGPU_RET(class_spectra(pb, d_inv, src_off, C.lambda, l2f, nullptr, C.d_coef32L, nullptr, C.d_tw32L));
         }
-        MGPU_HIP(hipStreamSynchronize(s));
-        pb.arena.release(d_buf);
-        pb.arena.release(d_src);
-        if (d_inv_tmp) pb.arena.release(d_inv_tmp);
     }
+    if (d_inv_tmp) pb.arena.release(d_inv_tmp);
     if (!direct_blocks.empty()) {
         // keep invtt on device for the direct product
         double *d_inv_keep;
-        MGPU_RET(pb.arena.alloc(&d_inv_keep, (size_t) nb * lambda));
-        MGPU_HIP(hipMemcpyAsync(d_inv_keep, inv, sizeof(double) * (size_t) nb * lambda, hipMemcpyHostToDevice, s));
+        MGPU_RET(pb.arena.alloc(&d_inv_keep, (size_t) ninv));
+        MGPU_HIP(hipMemcpyAsync(d_inv_keep, inv, sizeof(double) * (size_t) ninv, hipMemcpyHostToDevice, s));
         T.d_direct_invtt = d_inv_keep;
+        MGPU_RET(pb.arena.alloc(&T.d_inv_off, (size_t) nb));
+        MGPU_HIP(hipMemcpyAsync(T.d_inv_off, inv_off.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, s));
         T.h_direct_blocks = direct_blocks;
         MGPU_RET(pb.arena.alloc(&T.d_direct_blocks, direct_blocks.size()));
         MGPU_HIP(hipMemcpyAsync(T.d_direct_blocks, direct_blocks.data(), sizeof(int) * direct_blocks.size(),
@@ -1115,13 +1467,16 @@ struct TileShape {
     int logv, logr;
     bool two;
     int minw;
+    bool wave_local;  // k_tiles_wl (N = 32768): inner passes exchange inside the wave
 };
 
-static const TileShape kShapes64_9[] = {{3, 3, true, 1}};
-static const TileShape kShapes64_mid[] = {{4, 4, true, 1}, {3, 3, true, 1}};
-static const TileShape kShapes64_14[] = {{5, 4, false, 1}, {5, 5, false, 1}, {4, 4, false, 1}};
-static const TileShape kShapes32_14[] = {{5, 4, true, 1}, {5, 5, true, 1}, {4, 4, true, 1}, {5, 4, false, 4}};
-static const TileShape kShapes32_15[] = {{5, 4, false, 1}, {6, 5, false, 1}};
+static const TileShape kShapes64_9[] = {{3, 3, true, 1, false}};
+static const TileShape kShapes64_mid[] = {{4, 4, true, 1, false}, {3, 3, true, 1, false}};
+static const TileShape kShapes64_14[] = {{5, 4, false, 1, false}, {5, 5, false, 1, false}, {4, 4, false, 1, false},
+                                         {5, 5, false, 1, true}};
+static const TileShape kShapes32_14[] = {{5, 4, true, 1, false}, {5, 5, true, 1, false}, {4, 4, true, 1, false},
+                                         {5, 4, false, 4, false}, {5, 5, false, 4, true}};
+static const TileShape kShapes32_15[] = {{5, 4, false, 1, false}, {6, 5, false, 1, false}};
 
 static const TileShape *tile_shapes(bool fp32, int log2n, int *n) {
     const int lognc = log2n - 1;
@@ -1156,9 +1511,9 @@ const char *tile_variant_desc(bool fp32, int log2n, int variant) {
     const TileShape &v = sh[variant - 1];
     const int nc = 1 << (log2n - 1);
     std::snprintf(buf, sizeof(buf),
-                  "%s N=%d: %d complex values per lane, %d lanes, radix <= %d, %s LDS exchange, >= %d waves/SIMD",
+                  "%s N=%d: %d complex values per lane, %d lanes, radix <= %d, %s LDS exchange%s, >= %d waves/SIMD",
                   fp32 ? "fp32" : "fp64", 1 << log2n, 1 << v.logv, nc >> v.logv, 1 << v.logr,
-                  v.two ? "two-plane" : "one-plane", v.minw);
+                  v.two ? "two-plane" : "one-plane", v.wave_local ? " (inner passes inside the wave)" : "", v.minw);
     return buf;
 }
 
@@ -1169,18 +1524,17 @@ struct TileUse {
     const void *coef, *tw;
 };
 template <typename R>
-static TileUse tile_use(const ToeplitzPlan &T) {
-    if (sizeof(R) == 4 && T.log2n32 > 0 && T.fp32_long)
-        return {T.log2n32, T.L32, T.ntiles32, T.d_tiles32, T.d_coef32L, T.d_tw32L};
-    if (sizeof(R) == 4) return {T.log2n, T.L, T.ntiles, T.d_tiles, T.d_coef32, T.d_tw32};
-    return {T.log2n, T.L, T.ntiles, T.d_tiles, T.d_coef64, T.d_tw64};
+static TileUse tile_use(const TileClass &C) {
+    if (sizeof(R) == 4 && C.log2n32 > 0 && C.fp32_long)
+        return {C.log2n32, C.L32, C.ntiles32, C.d_tiles32, C.d_coef32L, C.d_tw32L};
+    if (sizeof(R) == 4) return {C.log2n, C.L, C.ntiles, C.d_tiles, C.d_coef32, C.d_tw32};
+    return {C.log2n, C.L, C.ntiles, C.d_tiles, C.d_coef64, C.d_tw64};
 }
 
 template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO, int MINW>
-static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *coef, const void *tw,
-                            const ToeplitzScratch &sc) {
-    const ToeplitzPlan &T = pb.tp;
-    const TileUse u = tile_use<R>(T);
+static int launch_tiles_cfg(Problem &pb, const TileClass &C, const R *d_in, R *d_out, const void *coef,
+                            const void *tw, const ToeplitzScratch &sc) {
+    const TileUse u = tile_use<R>(C);
     constexpr int NC = 1 << LOGNC, NT = NC / (1 << LOGV);
     constexpr size_t lds = sizeof(R) * (size_t) lds_elems(NC) * (TWO ? 2 : 1);
     static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
@@ -1190,21 +1544,41 @@ static int launch_tiles_cfg(Problem &pb, const R *d_in, R *d_out, const void *co
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
         attr_devices |= 1u << pb.device;
     }
-    TileArgs a{d_in, d_out, u.tiles, coef, tw, sc.mean, sc.edge_part, T.lambda, u.L, pb.sg.d_block_seg,
+    TileArgs a{d_in, d_out, u.tiles, coef, tw, sc.mean, sc.edge_part, C.lambda, u.L, pb.sg.d_block_seg,
                pb.sg.d_block_seg ? pb.live_state : nullptr};
     hipLaunchKernelGGL(kern, dim3(u.ntiles), dim3(NT), lds, pb.stream, a);
     MGPU_LAUNCHED();
     return MGPU_OK;
 }
 
+template <typename R, int MINW>
+static int launch_tiles_wl(Problem &pb, const TileClass &C, const R *d_in, R *d_out, const ToeplitzScratch &sc) {
+    const void *coef = sizeof(R) == 4 ? (const void *) C.d_coefW32 : (const void *) C.d_coefW64;
+    const void *tw = sizeof(R) == 4 ? (const void *) C.d_tw32 : (const void *) C.d_tw64;
+    MGPU_ARG(coef && C.log2n == 15 && !(sizeof(R) == 4 && C.fp32_long), "wave-local tiles need N = 32768");
+    constexpr size_t lds = sizeof(R) * (size_t) kWlLdsElems;
+    static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
+    auto kern = k_tiles_wl<R, MINW>;
+    static unsigned attr_devices = 0;  // devices on which the LDS limit was raised
+    if (pb.device < 32 && !(attr_devices & (1u << pb.device))) {
+        MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+        attr_devices |= 1u << pb.device;
+    }
+    TileArgs a{d_in, d_out, C.d_tiles, coef, tw, sc.mean, sc.edge_part, C.lambda, C.L, pb.sg.d_block_seg,
+               pb.sg.d_block_seg ? pb.live_state : nullptr};
+    hipLaunchKernelGGL(kern, dim3(C.ntiles), dim3(kWlNT), lds, pb.stream, a);
+    MGPU_LAUNCHED();
+    return MGPU_OK;
+}
+
 template <typename R>
-static int launch_tiles(Problem &pb, const R *d_in, R *d_out, int variant, const ToeplitzScratch &sc) {
-    const ToeplitzPlan &T = pb.tp;
+static int launch_tiles(Problem &pb, const TileClass &C, const R *d_in, R *d_out, int variant,
+                        const ToeplitzScratch &sc) {
     constexpr bool F32 = sizeof(R) == 4;
-    const TileUse u = tile_use<R>(T);
+    const TileUse u = tile_use<R>(C);
     const void *coef = u.coef, *tw = u.tw;
 #define MGPU_TL(LOGNC, LOGV, LOGR, TWO, MINW) \
-    return launch_tiles_cfg<R, LOGNC, LOGV, LOGR, TWO, MINW>(pb, d_in, d_out, coef, tw, sc)
+    return launch_tiles_cfg<R, LOGNC, LOGV, LOGR, TWO, MINW>(pb, C, d_in, d_out, coef, tw, sc)
     switch (u.log2n - 1) {  // LOGNC
         case 9: MGPU_TL(9, 3, 3, true, 1);
         case 10:
@@ -1224,10 +1598,12 @@ static int launch_tiles(Problem &pb, const R *d_in, R *d_out, int variant, const
                 if (variant == 2) MGPU_TL(14, 5, 5, true, 1);
                 if (variant == 3) MGPU_TL(14, 4, 4, true, 1);
                 if (variant == 4) MGPU_TL(14, 5, 4, false, 4);
+                if (variant == 5) return launch_tiles_wl<R, 4>(pb, C, d_in, d_out, sc);
                 MGPU_TL(14, 5, 4, true, 1);
             } else {
                 if (variant == 2) MGPU_TL(14, 5, 5, false, 1);
                 if (variant == 3) MGPU_TL(14, 4, 4, false, 1);
+                if (variant == 4) return launch_tiles_wl<R, 1>(pb, C, d_in, d_out, sc);
                 MGPU_TL(14, 5, 4, false, 1);
             }
         case 15:
@@ -1250,7 +1626,7 @@ static int launch_tiles(Problem &pb, const R *d_in, R *d_out, int variant, const
 static int default_variant(bool fp32, int log2n) {
     const int lognc = log2n - 1;
     if (!fp32 && lognc == 14) return 2;
-    if (fp32 && lognc == 14) return 4;
+    if (fp32 && lognc == 14) return 5;
     return 1;
 }
 
@@ -1261,13 +1637,12 @@ static int default_variant(bool fp32, int log2n) {
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int>, int> g_tune;
 
-// Choose the tile variant of precision R for this plan: the forced one, the
+// Choose the tile variant of precision R for a tile class: the forced one, the
 // fixed default, or (MGPU_TILE_AUTOTUNE) the cached result or the fastest of
 // 1 warm-up + 3 timed launches of each variant on the caller's buffers (d_out
 // is overwritten by every launch; the caller launches the winner afterwards).
 template <typename R>
-static int pick_variant(Problem &pb, const R *d_in, R *d_out, const ToeplitzScratch &sc) {
-    ToeplitzPlan &T = pb.tp;
+static int pick_variant(Problem &pb, TileClass &T, const R *d_in, R *d_out, const ToeplitzScratch &sc) {
     const int slot = sizeof(R) == 4 ? 1 : 0;
     if (T.variant[slot]) return MGPU_OK;
     // FP32 products use the FP64 tile length unless autotuning picks the long set
@@ -1317,14 +1692,14 @@ static int pick_variant(Problem &pb, const R *d_in, R *d_out, const ToeplitzScra
         const bool lng = c > nv;
         const int v = lng ? c - nv : c;
         if (two_len) T.fp32_long = lng;
-        rc = launch_tiles<R>(pb, d_in, d_out, v, sc);
+        rc = launch_tiles<R>(pb, T, d_in, d_out, v, sc);
         if (rc != MGPU_OK) break;
         if (hipEventRecord(e0, pb.stream) != hipSuccess) {
             set_error("tile autotuning: hipEventRecord failed");
             rc = MGPU_EHIP;
             break;
         }
-        for (int k = 0; k < 3 && rc == MGPU_OK; k++) rc = launch_tiles<R>(pb, d_in, d_out, v, sc);
+        for (int k = 0; k < 3 && rc == MGPU_OK; k++) rc = launch_tiles<R>(pb, T, d_in, d_out, v, sc);
         // per useful output: the two lengths have different tile counts but the same outputs
         if (rc != MGPU_OK) break;
         float ms = 0.f;
@@ -1364,7 +1739,7 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
                                                                sc.partials);
             MGPU_LAUNCHED();
             k_block_mean<R><<<dim3(T.nb, 1), 256, 0, s>>>(sc.partials, T.d_start, T.nb, sc.mean, d_in, nullptr,
-                                                          nullptr, nullptr, 1, nullptr, nullptr, nullptr);
+                                                          nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
             MGPU_LAUNCHED();
             k_diag_mean<R><<<grid_for(T.m, 256), 256, 0, s>>>(d_in, d_out, T.m, T.d_start, T.nb, T.d_T0, sc.mean);
         } else {
@@ -1382,20 +1757,23 @@ static int toeplitz_apply_t(Problem &pb, const R *d_in, R *d_out, bool compute_p
         }
         const bool sym = T.offset_mode == MGPU_OFFSET_SYMMETRIC;
         k_block_mean<R><<<dim3(T.nb, sym ? kEdgeSlices : 1), 256, 0, s>>>(
-                sc.partials, T.d_start, T.nb, sc.mean, d_in, T.d_tail, T.d_block_filter, T.d_block_kind, T.lambda,
+                sc.partials, T.d_start, T.nb, sc.mean, d_in, T.d_tail, T.d_tail_off, T.d_block_filter, T.d_block_kind,
+                T.d_block_lambda,
                 sym ? sc.edge_part : nullptr, pb.sg.d_block_seg, pb.sg.d_block_seg ? pb.live_state : nullptr);
         MGPU_LAUNCHED();
         pb.prof.end(s);
     }
-    if (T.ntiles) {
-        MGPU_RET(pick_variant<R>(pb, d_in, d_out, sc));
+    // one launch per tile class (band and tile length), in one profiled span
+    for (TileClass &C : T.cls) MGPU_RET(pick_variant<R>(pb, C, d_in, d_out, sc));
+    if (!T.cls.empty()) {
         pb.prof.begin(MGPU_KT_TILES, s);
-        MGPU_RET(launch_tiles<R>(pb, d_in, d_out, T.variant[sizeof(R) == 4 ? 1 : 0], sc));
+        for (TileClass &C : T.cls) MGPU_RET(launch_tiles<R>(pb, C, d_in, d_out, C.variant[sizeof(R) == 4 ? 1 : 0], sc));
         pb.prof.end(s);
     }
     if (!T.h_direct_blocks.empty()) {
         k_direct<R><<<dim3(64, (unsigned) T.h_direct_blocks.size()), 256, 0, s>>>(
-                d_in, d_out, T.d_start, T.d_direct_blocks, sc.mean, sc.edge_part, T.d_direct_invtt, T.lambda);
+                d_in, d_out, T.d_start, T.d_direct_blocks, sc.mean, sc.edge_part, T.d_direct_invtt, T.d_inv_off,
+                T.d_block_lambda);
         MGPU_LAUNCHED();
     }
     if (!T.h_short_blocks.empty()) {

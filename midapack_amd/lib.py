@@ -321,6 +321,11 @@ def tile_variants(precision: int, tile_log2: int) -> list[str]:
     return out
 
 
+def finalize():
+    """Release the library's cached device blocks and pinned staging buffers."""
+    _check(load().mappraiser_gpu_finalize(), "finalize")
+
+
 def init(device: int = 0):
     _check(load().mappraiser_gpu_init(int(device)), "mappraiser_gpu_init")
 
@@ -493,6 +498,15 @@ class Problem:
                                          hist.ctypes.data if hist is not None else None, ct.byref(rep)), "pcg")
         nh = min(rep.iterations + 1, maxiter)
         return x[:n], (hist[:nh] if hist is not None else None), rep
+
+
+def dropin_kernel_times(reset=False):
+    """{class: (total ms, launches)} over the drop-in solves (pcg_gls, MLmap, MTmap)
+    run with options(profile=True) in this process."""
+    ms = np.zeros(len(KERNEL_CLASSES), np.float64)
+    cnt = np.zeros(len(KERNEL_CLASSES), np.int64)
+    _check(load().mappraiser_gpu_kernel_times(None, ms.ctypes.data, cnt.ctypes.data, int(reset)), "kernel_times")
+    return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
 
 @dataclass

@@ -330,7 +330,7 @@ def test_tile_variant_default_is_fixed(gpu):
     variants by timing."""
     wl = W.make_workload("autotune", 2, 1 << 17, 64, 64, 8192, 3, block_len=1 << 17)
     print("case: default and autotuned tile variant, lambda 8192", flush=True)
-    fixed = {gpu.MGPU_FP64: 2, gpu.MGPU_MIXED: 4}
+    fixed = {gpu.MGPU_FP64: 2, gpu.MGPU_MIXED: 5}
     for prec in (gpu.MGPU_FP64, gpu.MGPU_MIXED):
         g = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8,
                         maxiter=50, opts=gpu.options(precision=prec, offset_mode=gpu.MGPU_OFFSET_SYMMETRIC))
