@@ -523,7 +523,7 @@ def serve_section(G, W, args):
                "p99_ms": round(float(np.percentile(lat, 99)), 1), "mean_queue_ms": round(float(np.mean(qs)), 1),
                "maps_per_s": round(n / wall, 3), "max_map_rel_err_vs_fp64": max(errs),
                "batched_groups": st["groups"], "peak_concurrent_solves": st["peak_running"],
-               "peak_hbm_gb": round(st["peak_hbm_gb"], 2)}
+               "peak_hbm_gb": round(st["peak_hbm_gb"], 2), "requeued": st["requeued"]}
         log(f"serve rate {rate}: p50 {ent['p50_ms']} ms, p99 {ent['p99_ms']} ms, {ent['maps_per_s']} maps/s")
         out["rates"].append(ent)
     return out
@@ -641,4 +641,6 @@ if __name__ == "__main__":
         _G.finalize()
     sys.stdout.flush()
     sys.stderr.flush()
-    os._exit(0)
+    # under a profiler (its library preloaded) the normal exit runs its finalisation
+    if not any(k.startswith("ROCPROFILER") for k in os.environ) and "rocprofiler" not in os.environ.get("LD_PRELOAD", ""):
+        os._exit(0)

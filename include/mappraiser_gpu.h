@@ -108,6 +108,9 @@ typedef struct mgpu_report {
     int    refreshes;   /* FP64 residual recomputations performed (MGPU_MIXED) */
     int    tile_variant;  /* tile kernel variant used by the solve's Toeplitz product (0: none) */
     double t_2lvl_ms;   /* device time building the two-level space Z, A Z, pinv(E) (precond 1, 2) */
+    long long dev_peak_bytes; /* device memory the solve held at its high-water mark (inputs staged on
+                                 the device, pointing tables, Toeplitz plan, preconditioner, work
+                                 vectors); for a batch item, that of the whole batch it ran in */
 } mgpu_report;
 
 /* ---- library ---- */
@@ -261,8 +264,13 @@ int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int grid_npix, 
  * as soon as its own map has converged, while the rest of its group iterates
  * on (its report then carries no solve time and, in MGPU_MIXED, no FP64 true
  * residual).  Results are bit-identical to the requests' solo solves.
- * hbm_budget_gb <= 0 uses 80% of the free device memory.  No reference
- * counterpart. */
+ * hbm_budget_gb <= 0 uses 80% of the free device memory.  Footprints are
+ * estimated per sample and per grid pixel and scaled by the largest ratio of
+ * measured to estimated bytes seen in finished groups.  A group whose device
+ * allocations fail while other solves run, or that merged several requests,
+ * returns to the queue head; its requests then run one at a time on an idle
+ * device (a failure there is reported).  A merged group stays within 2^30
+ * samples and 2^28 grid pixels.  No reference counterpart. */
 typedef struct mgpu_server mgpu_server;
 
 int mappraiser_gpu_server_create(mgpu_server **out, double hbm_budget_gb, int max_slots, int max_batch,
@@ -271,8 +279,10 @@ int mappraiser_gpu_server_submit(mgpu_server *s, mgpu_batch_item *item, int grid
                                  long long *ticket);
 /* blocks until the request is solved; latency = done - submitted, queue = admitted - submitted */
 int mappraiser_gpu_server_wait(mgpu_server *s, long long ticket, double *latency_ms, double *queue_ms);
+/* admitted: admissions (a requeued request counts again); groups: batched solves started;
+ * peak_gb: largest budget charge; requeued: requests sent back after an allocation failure */
 int mappraiser_gpu_server_stats(mgpu_server *s, long long *admitted, long long *groups, double *peak_gb,
-                                int *peak_running);
+                                int *peak_running, long long *requeued);
 /* finishes every queued and running request, then frees the server */
 int mappraiser_gpu_server_destroy(mgpu_server *s);
 

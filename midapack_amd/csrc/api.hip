@@ -49,10 +49,13 @@ int dev_alloc(void **p, size_t bytes, int device, size_t *cls) {
     const size_t c = size_class(bytes);
     *cls = c;
     {
+        // smallest cached block of at least c bytes, if no more than 1/8 larger
+        // (blocks keep their own size class)
         std::lock_guard<std::mutex> g(g_cache_mu);
         auto &fl = g_cache[device];
-        auto it = fl.find(c);
-        if (it != fl.end()) {
+        auto it = fl.lower_bound(c);
+        if (it != fl.end() && it->first <= c + c / 8) {
+            *cls = it->first;
             *p = it->second;
             fl.erase(it);
             return MGPU_OK;
@@ -114,6 +117,7 @@ int new_problem(std::unique_ptr<Problem> &pb, const mgpu_options *opt) {
     MGPU_HIP(hipSetDevice(g_device));
     MGPU_HIP(hipStreamCreateWithFlags(&pb->stream, hipStreamNonBlocking));
     pb->arena.stream = pb->stream;
+    pb->cub_tmp.arena = &pb->arena;
     mappraiser_gpu_options_default(&pb->opt);
     if (opt) pb->opt = *opt;
     pb->prof.on = pb->opt.profile != 0;

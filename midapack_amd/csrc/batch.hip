@@ -120,6 +120,12 @@ int solve_items(mgpu_batch_item *items, const std::vector<int> &idx, const std::
     MGPU_RET(P.arena.alloc(&d_inv, (size_t) ninv));
     MGPU_RET(P.arena.alloc(&d_err, 1));
     MGPU_HIP(hipMemsetAsync(d_err, 0, sizeof(int), s));
+    // host arrays go through the pinned staging pool (large ones), device arrays are copied on s
+    auto upload = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (!on_dev) return h2d_staged(P.device, dst, src, bytes, s);
+        MGPU_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+        return MGPU_OK;
+    };
     std::vector<int> sizes, blam, bl2;
     sizes.reserve((size_t) nb_tot);
     blam.reserve((size_t) nb_tot);
@@ -128,11 +134,11 @@ int solve_items(mgpu_batch_item *items, const std::vector<int> &idx, const std::
     for (int r = 0; r < R; r++) {
         const mgpu_batch_item &it = items[idx[r]];
         const int64_t o = off[r];
-        MGPU_HIP(hipMemcpyAsync(d_pix + 3 * o, it.pix, sizeof(int) * 3 * (size_t) it.m, kind, s));
-        MGPU_HIP(hipMemcpyAsync(d_w + 3 * o, it.pixweights, sizeof(double) * 3 * (size_t) it.m, kind, s));
-        MGPU_HIP(hipMemcpyAsync(P.d_sig + o, it.signal, sizeof(double) * (size_t) it.m, kind, s));
-        MGPU_HIP(hipMemcpyAsync(P.d_noise + o, it.noise, sizeof(double) * (size_t) it.m, kind, s));
-        MGPU_HIP(hipMemcpyAsync(d_inv + irow, it.invtt, sizeof(double) * (size_t) it.nb * it.lambda, kind, s));
+        MGPU_RET(upload(d_pix + 3 * o, it.pix, sizeof(int) * 3 * (size_t) it.m));
+        MGPU_RET(upload(d_w + 3 * o, it.pixweights, sizeof(double) * 3 * (size_t) it.m));
+        MGPU_RET(upload(P.d_sig + o, it.signal, sizeof(double) * (size_t) it.m));
+        MGPU_RET(upload(P.d_noise + o, it.noise, sizeof(double) * (size_t) it.m));
+        MGPU_RET(upload(d_inv + irow, it.invtt, sizeof(double) * (size_t) it.nb * it.lambda));
         irow += (int64_t) it.nb * it.lambda;
         k_batch_pix<<<grid_for(3 * (int64_t) it.m, 256), 256, 0, s>>>(d_pix + 3 * o, 3 * (int64_t) it.m,
                                                                       (int) (3 * pix0[r]), 3 * grid[r], d_err);
@@ -213,6 +219,7 @@ int solve_items(mgpu_batch_item *items, const std::vector<int> &idx, const std::
         ir.t_setup_ms = setup_ms;
         ir.true_res_rel = o.true_res_rel;
         ir.tile_variant = P.tp.variant(P.opt.precision == MGPU_MIXED ? 1 : 0);
+        ir.dev_peak_bytes = (long long) P.arena.peak;
         MGPU_HIP(hipStreamSynchronize(s));
         if (done) (*done)(idx[r]);
         return MGPU_OK;
@@ -226,6 +233,7 @@ int solve_items(mgpu_batch_item *items, const std::vector<int> &idx, const std::
         ir.t_solve_ms = rep.t_solve_ms;
         ir.refreshes = rep.refreshes;
         ir.true_res_rel = segs[r].true_res_rel;
+        ir.dev_peak_bytes = (long long) P.arena.peak;
     }
     return MGPU_OK;
 }

@@ -13,6 +13,7 @@
 //                           coefficients per distinct filter, twiddle table
 #pragma once
 
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -75,8 +76,9 @@ void set_error(const char *fmt, ...);
 // Process-wide cache of device blocks: hipFree synchronises the device and
 // hipMalloc of large blocks costs milliseconds, so blocks released by a
 // problem are kept per device (size classes of 2 MiB above 1 MiB) and handed
-// to later allocations of the same class.  On an allocation failure the
-// cache of the device is emptied and the allocation retried.
+// to later allocations of at most 1/8 fewer bytes (*cls returns the block's
+// class).  On an allocation failure the cache of the device is emptied and
+// the allocation retried.
 // mappraiser_gpu_finalize() empties every cache.
 int dev_alloc(void **p, size_t bytes, int device, size_t *cls);
 void dev_free(void *p, size_t cls, int device);
@@ -108,6 +110,7 @@ struct Arena {
     std::vector<Blk> blks;
     int device = 0;
     hipStream_t stream = nullptr;
+    size_t live = 0, peak = 0;  // bytes held (whole blocks) now and at the high-water mark
     template <typename T>
     int alloc(T **p, size_t n) {
         *p = nullptr;
@@ -117,6 +120,8 @@ struct Arena {
         const int rc = dev_alloc(&q, n * sizeof(T), device, &cls);
         if (rc != MGPU_OK) return rc;
         blks.push_back({q, cls});
+        live += cls;
+        peak = std::max(peak, live);
         *p = (T *) q;
         return MGPU_OK;
     }
@@ -125,6 +130,7 @@ struct Arena {
             if (b.p == p) {
                 if (stream) (void) hipStreamSynchronize(stream);
                 dev_free(b.p, b.cls, device);
+                live -= b.cls;
                 b.p = nullptr;
             }
     }
@@ -134,26 +140,19 @@ struct Arena {
     }
 };
 
-// Scratch that grows on demand (hipcub temporaries, host staging).
+// Scratch that grows on demand (hipcub temporaries), held in the problem's arena.
 struct Scratch {
-    void *p = nullptr;
+    Arena *arena = nullptr;
+    unsigned char *p = nullptr;
     size_t n = 0;
     int ensure(size_t bytes) {
         if (bytes <= n) return MGPU_OK;
-        if (p) (void) hipFree(p);
+        if (p) arena->release(p);
         p = nullptr;
         n = 0;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            (void) hipGetLastError();
-            set_error("hipMalloc scratch of %zu bytes failed", bytes);
-            return MGPU_ENOMEM;
-        }
+        MGPU_RET(arena->alloc(&p, bytes));
         n = bytes;
         return MGPU_OK;
-    }
-    ~Scratch() {
-        if (p) (void) hipFree(p);
     }
 };
 

@@ -836,6 +836,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         rep->tile_variant = pb.tp.variant(mixed ? 1 : 0);
         rep->true_res_rel = true_rel[0];
         rep->t_2lvl_ms = two ? pb.tl.build_ms : 0.0;
+        rep->dev_peak_bytes = (long long) pb.arena.peak;
     }
     if (segs) {
         segs->assign((size_t) nseg, SegReport{});

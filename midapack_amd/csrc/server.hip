@@ -14,6 +14,13 @@
 // pool and the queue is scanned again.  Each result is bit-identical to the
 // request's solo solve: batching changes neither the kernels nor the
 // summation order of an item.
+//
+// Footprints are estimated per sample and per grid pixel and corrected by
+// what finished groups actually held (their arena's high-water mark), so the
+// budget tracks the real usage.  A group whose allocations fail while other
+// solves hold memory goes back to the queue head; such requests are later
+// admitted one at a time and only onto an idle device, where a second failure
+// is final.
 #include "mgpu_internal.h"
 
 #include <algorithm>
@@ -36,7 +43,9 @@ struct Request {
     double tol = 1e-8;
     int maxiter = 500;
     int grid = 0;
-    int64_t bytes = 0;
+    int64_t est = 0;    // footprint estimate before the learned correction
+    int64_t bytes = 0;  // bytes charged to the budget while admitted
+    int retries = 0;    // times sent back to the queue after an allocation failure
     Clock::time_point t_submit, t_start, t_done;
     bool done = false;
     int status = MGPU_OK;
@@ -57,6 +66,7 @@ struct Server {
     std::unordered_map<long long, std::unique_ptr<Request>> reqs;
     std::list<Worker> workers;
     int64_t budget = 0, used = 0;
+    double correction = 1.0;  // max over finished groups of (measured peak / estimate)
     int max_slots = 4, max_batch = 16, running = 0;
     int device = 0;  // the device current when the server was created
     long long next_ticket = 1;
@@ -64,7 +74,7 @@ struct Server {
     mgpu_options opt{};
     std::thread dispatcher;
     // statistics
-    long long admitted = 0, groups = 0;
+    long long admitted = 0, groups = 0, requeued = 0;
     int64_t peak_used = 0;
     int peak_running = 0;
 
@@ -74,13 +84,18 @@ struct Server {
 
 // Device bytes of one solve: inputs staged on the device (pixels, weights,
 // signal, noise), pointing tables (q, Q/U weights, sliced ELL with its sort
-// keys during the build), TOD work vectors in both precisions, and the map
-// work vectors and 3x3 blocks per pixel of the grid.
+// keys and scan temporaries during the build), TOD work vectors in both
+// precisions, and the map work vectors and 3x3 blocks per pixel of the grid.
 static int64_t request_bytes(const mgpu_batch_item &it, int grid, bool mixed) {
-    const int64_t per_sample = 12 + 24 + 8 + 8 + 4 + 16 + 24 + 16 + 12 + (mixed ? 8 + 8 + 8 : 0);
+    const int64_t per_sample = 160 + (mixed ? 32 : 0);
     const int64_t per_pixel = 8 * 3 * 9 + 8 * 9 + 8 + 4 + 4 * 4;
     return (int64_t) it.m * per_sample + (int64_t) std::min<int64_t>(grid, it.m) * per_pixel + (64 << 20);
 }
+
+// Limits of one merged group: the batch solve indexes samples and map values
+// with int32 (each item padded to whole chunks).
+constexpr int64_t kGroupSamples = (int64_t) 1 << 30;
+constexpr int64_t kGroupPixels = (int64_t) 1 << 28;
 
 void Server::run_group(std::vector<Request *> grp, Worker *self) {
     const int n = (int) grp.size();
@@ -95,8 +110,15 @@ void Server::run_group(std::vector<Request *> grp, Worker *self) {
         mx[i] = grp[i]->maxiter;
     }
     const Clock::time_point t0 = Clock::now();
-    for (Request *r : grp) r->t_start = t0;
-    // a request is answered as soon as its own segment has stopped
+    int64_t group_bytes = 0;
+    for (Request *r : grp) {
+        r->t_start = t0;
+        group_bytes += r->bytes;
+    }
+    // A request is answered as soon as its own segment has stopped.  Once
+    // answered, its waiter may collect it and free it: from then on this
+    // thread never touches that request again (answered[i] is this thread's own record).
+    std::vector<char> answered((size_t) n, 0);
     const std::function<void(int)> done = [&](int i) {
         const Clock::time_point t = Clock::now();
         std::lock_guard<std::mutex> g(mu);
@@ -105,6 +127,7 @@ void Server::run_group(std::vector<Request *> grp, Worker *self) {
         r->status = MGPU_OK;
         r->t_done = t;
         r->done = true;
+        answered[i] = 1;
         cv.notify_all();
     };
     int rc = MGPU_OK;
@@ -116,19 +139,36 @@ void Server::run_group(std::vector<Request *> grp, Worker *self) {
         if (rc == MGPU_OK) rc = pcg_batch_impl(items.data(), all, grid, tl, mx, false, &opt, &done);
     }
     std::string err = rc == MGPU_OK ? std::string() : std::string(mappraiser_gpu_last_error());
+    if (rc == MGPU_ENOMEM) dev_cache_trim(device);
     const Clock::time_point t1 = Clock::now();
     std::lock_guard<std::mutex> g(mu);
+    // an allocation failure beside other solves, or in a merged group, is retried
+    // later on its own; alone on an idle device it is final
+    const bool retry = rc == MGPU_ENOMEM && (running > 1 || n > 1);
+    std::vector<Request *> back;
     for (int i = 0; i < n; i++) {
-        Request *r = grp[i];
-        if (!r->done) {  // failed solves answer every request of the group still waiting
-            r->item->rep = items[i].rep;
-            r->status = rc;
-            r->err = err;
-            r->t_done = t1;
-            r->done = true;
+        if (answered[i]) continue;
+        Request *r = grp[i];  // not answered yet: its waiter still blocks, the request is alive
+        if (retry && r->retries < 2) {
+            r->retries++;
+            back.push_back(r);
+            continue;
         }
-        used -= r->bytes;
+        r->item->rep = items[i].rep;
+        r->status = rc;
+        r->err = err;
+        r->t_done = t1;
+        r->done = true;
     }
+    pending.insert(pending.begin(), back.begin(), back.end());
+    requeued += (long long) back.size();
+    if (rc == MGPU_OK && n > 0) {
+        int64_t est = 0;
+        for (Request *r : grp) est += r->est;
+        const double peak = (double) items[0].rep.dev_peak_bytes;
+        if (est > 0 && peak > 0) correction = std::min(4.0, std::max(correction, 1.05 * peak / (double) est));
+    }
+    used -= group_bytes;
     running--;
     self->finished = true;
     cv.notify_all();
@@ -146,7 +186,11 @@ void Server::run_dispatcher() {
                 ++it;
             }
         }
-        auto fits = [&](const Request *r) { return running == 0 || used + r->bytes <= budget; };
+        auto cost = [&](const Request *r) { return (int64_t) ((double) r->est * correction); };
+        auto fits = [&](const Request *r) {
+            if (r->retries > 0) return running == 0;  // failed beside others: run alone
+            return running == 0 || used + cost(r) <= budget;
+        };
         if (stop && pending.empty() && running == 0) break;
         if (pending.empty() || running >= max_slots || !fits(pending.front())) {
             cv.wait(lk);
@@ -156,13 +200,20 @@ void Server::run_dispatcher() {
         std::vector<Request *> grp;
         Request *head = pending.front();
         pending.pop_front();
+        head->bytes = cost(head);
         grp.push_back(head);
         used += head->bytes;
-        for (auto it = pending.begin(); it != pending.end() && (int) grp.size() < max_batch;) {
+        int64_t samples = head->item->m + kChunk, pixels = head->grid;
+        for (auto it = pending.begin(); head->retries == 0 && it != pending.end() && (int) grp.size() < max_batch;) {
             Request *r = *it;
-            if (used + r->bytes <= budget) {
+            const int64_t b = cost(r);
+            if (r->retries == 0 && used + b <= budget && samples + r->item->m + kChunk <= kGroupSamples &&
+                pixels + r->grid <= kGroupPixels) {
+                r->bytes = b;
                 grp.push_back(r);
-                used += r->bytes;
+                used += b;
+                samples += r->item->m + kChunk;
+                pixels += r->grid;
                 it = pending.erase(it);
             } else {
                 ++it;
@@ -225,7 +276,7 @@ int mappraiser_gpu_server_submit(mgpu_server *h, mgpu_batch_item *item, int grid
     r->tol = tol;
     r->maxiter = maxiter;
     r->grid = grid_npix;
-    r->bytes = request_bytes(*item, grid_npix, s.opt.precision == MGPU_MIXED);
+    r->est = request_bytes(*item, grid_npix, s.opt.precision == MGPU_MIXED);
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(s.mu);
     MGPU_ARG(!s.stop, "server is shutting down");
@@ -255,7 +306,7 @@ int mappraiser_gpu_server_wait(mgpu_server *h, long long ticket, double *latency
 }
 
 int mappraiser_gpu_server_stats(mgpu_server *h, long long *admitted, long long *groups, double *peak_gb,
-                                int *peak_running) {
+                                int *peak_running, long long *requeued) {
     MGPU_ARG(h, "NULL handle");
     Server &s = *reinterpret_cast<Server *>(h);
     std::lock_guard<std::mutex> g(s.mu);
@@ -263,6 +314,7 @@ int mappraiser_gpu_server_stats(mgpu_server *h, long long *admitted, long long *
     if (groups) *groups = s.groups;
     if (peak_gb) *peak_gb = (double) s.peak_used / 1e9;
     if (peak_running) *peak_running = s.peak_running;
+    if (requeued) *requeued = s.requeued;
     return MGPU_OK;
 }
 

@@ -64,7 +64,7 @@ class Report(ct.Structure):
                 ("res", ct.c_double), ("n_map", ct.c_int), ("lcount", ct.c_int), ("trash_pix", ct.c_int),
                 ("n_cut", ct.c_int), ("t_setup_ms", ct.c_double), ("t_solve_ms", ct.c_double),
                 ("true_res_rel", ct.c_double), ("refreshes", ct.c_int), ("tile_variant", ct.c_int),
-                ("t_2lvl_ms", ct.c_double)]
+                ("t_2lvl_ms", ct.c_double), ("dev_peak_bytes", ct.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -169,7 +169,8 @@ def load():
                                                ct.POINTER(ct.c_longlong)]
     L.mappraiser_gpu_server_wait.argtypes = [_vp, ct.c_longlong, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]
     L.mappraiser_gpu_server_stats.argtypes = [_vp, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong),
-                                              ct.POINTER(ct.c_double), ct.POINTER(ct.c_int)]
+                                              ct.POINTER(ct.c_double), ct.POINTER(ct.c_int),
+                                              ct.POINTER(ct.c_longlong)]
     L.mappraiser_gpu_server_destroy.argtypes = [_vp]
     L.mappraiser_gpu_noise_invtt.argtypes = [_vp, ct.c_int, _i32p, ct.c_int, ct.c_double, ct.c_int, ct.c_int, _vp,
                                              _vp]
@@ -657,10 +658,11 @@ class Server:
         return _item_result(b, outs), lat.value, q.value
 
     def stats(self) -> dict:
-        a, g, pk, pr = ct.c_longlong(0), ct.c_longlong(0), ct.c_double(0), ct.c_int(0)
-        _check(load().mappraiser_gpu_server_stats(self._h, ct.byref(a), ct.byref(g), ct.byref(pk), ct.byref(pr)),
-               "server_stats")
-        return {"admitted": a.value, "groups": g.value, "peak_hbm_gb": pk.value, "peak_running": pr.value}
+        a, g, pk, pr, rq = ct.c_longlong(0), ct.c_longlong(0), ct.c_double(0), ct.c_int(0), ct.c_longlong(0)
+        _check(load().mappraiser_gpu_server_stats(self._h, ct.byref(a), ct.byref(g), ct.byref(pk), ct.byref(pr),
+                                                  ct.byref(rq)), "server_stats")
+        return {"admitted": a.value, "groups": g.value, "peak_hbm_gb": pk.value, "peak_running": pr.value,
+                "requeued": rq.value}
 
     def close(self):
         if self._h:
