@@ -152,6 +152,18 @@ int mappraiser_gpu_trmatvec(mgpu_problem *pb, const double *y, double *x);
 /* tod <- N^-1 tod in place (apply_weights, pcg_true.c:578-600 -> stbmmProd,
  * toeplitz_wizard.c:80-111). */
 int mappraiser_gpu_apply_weights(mgpu_problem *pb, double *tod);
+/* d_out <- N^-1 d_in on device vectors of m doubles (FP64), the product of
+ * mappraiser_gpu_apply_weights without any allocation: no reference
+ * counterpart; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The product's scratch (per-chunk
+ * partial sums, block means, output-mean slices) lives in the library's
+ * static device array of 2^20 doubles (8 MiB per GPU, held until the process
+ * ends): problems needing more (nchunks + 10 nb doubles, nchunks = ceil(m / 1024))
+ * are rejected with MGPU_EINVAL.  Launched on `stream` (a hipStream_t; NULL:
+ * the handle's stream) and not synchronised.  Calls that overlap on one GPU
+ * share that array and race.  A handle set to MGPU_TILE_AUTOTUNE that has not
+ * yet chosen its tile variant uses the fixed default. */
+int mappraiser_gpu_apply_weights_noalloc(mgpu_problem *pb, const double *d_in, double *d_out, void *stream);
 
 /* Block-Jacobi preconditioner with the degenerate-pixel cut
  * (precondblockjacobilike, precond.c:456-702; build_precond precond=0,

@@ -175,6 +175,8 @@ int mappraiser_gpu_init(int device) {
     MGPU_ARG(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
     MGPU_HIP(hipSetDevice(device));
     g_device = device;
+    // address of the allocation-free products' scratch array on this device
+    MGPU_RET(noalloc_scratch_init(device));
     return MGPU_OK;
 }
 
@@ -296,6 +298,13 @@ int mappraiser_gpu_apply_weights(mgpu_problem *h, double *tod) {
     MGPU_HIP(hipMemcpyAsync(tod, pb.d_tod2, sizeof(double) * pb.pt.m, hipMemcpyDeviceToHost, pb.stream));
     MGPU_HIP(hipStreamSynchronize(pb.stream));
     return MGPU_OK;
+}
+
+int mappraiser_gpu_apply_weights_noalloc(mgpu_problem *h, const double *d_in, double *d_out, void *stream) {
+    MGPU_ARG(h && d_in && d_out, "NULL argument");
+    Problem &pb = *PB(h);
+    MGPU_HIP(hipSetDevice(pb.device));
+    return toeplitz_apply_noalloc(pb, d_in, d_out, static_cast<hipStream_t>(stream));
 }
 
 int mappraiser_gpu_precond_build(mgpu_problem *h, double *signal, double *cond, int *lhits, int *n_map) {

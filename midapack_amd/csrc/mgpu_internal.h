@@ -520,6 +520,11 @@ size_t toeplitz_scratch_doubles(const ToeplitzPlan &T);
 // N^-1 (FP64, partial sums computed) with the scratch laid out in d_scratch
 // (toeplitz_scratch_doubles(T) doubles) instead of the plan's buffers
 int toeplitz_apply_scratch(Problem &pb, const double *d_in, double *d_out, double *d_scratch);
+// N^-1 with the scratch in the library's static device array (no allocation),
+// launched on `stream` (nullptr: the handle's stream); noalloc_scratch_init
+// looks up the array's address on the current device (once per device)
+int noalloc_scratch_init(int device);
+int toeplitz_apply_noalloc(Problem &pb, const double *d_in, double *d_out, hipStream_t stream);
 int tile_variant_count(bool fp32, int log2n);
 // real tile length 2^k the plan of a lambda / longest block would use (0: no tiles)
 int toeplitz_tile_log2(int lambda, int64_t nmax, int forced);

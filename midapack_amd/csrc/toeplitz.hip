@@ -1807,6 +1807,61 @@ __global__ void k_fill_zero(double *__restrict__ p, int64_t n) {
     if (i < n) p[i] = 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// N^-1 without any allocation (mappraiser_gpu_apply_weights_noalloc)
+// ---------------------------------------------------------------------------
+// The product's scratch (per-chunk partial sums, block means, output-mean
+// slices: toeplitz_scratch_doubles(T) doubles) lives in this file-scope device
+// array, one copy per GPU, held until the process ends.  Host code never
+// passes the array's name to a kernel: it uses the address that
+// hipGetSymbolAddress returns for the current device, kept per device.  Every
+// call on a GPU shares that GPU's copy; calls that overlap on one GPU race.
+constexpr size_t kNoallocScratchDoubles = (size_t) 1 << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+constexpr int kNoallocDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kNoallocDevices] = {};
+
+int noalloc_scratch_init(int device) {
+    MGPU_ARG(device >= 0 && device < kNoallocDevices, "device %d out of range of the scratch table", device);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MGPU_OK;
+    int cur = -1;
+    MGPU_HIP(hipGetDevice(&cur));
+    MGPU_ARG(cur == device, "scratch set-up for device %d while device %d is current", device, cur);
+    void *p = nullptr;
+    MGPU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MGPU_ARG(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return MGPU_OK;
+}
+
+int toeplitz_apply_noalloc(Problem &pb, const double *d_in, double *d_out, hipStream_t stream) {
+    ToeplitzPlan &T = pb.tp;
+    const size_t need = toeplitz_scratch_doubles(T);
+    MGPU_ARG(need <= kNoallocScratchDoubles,
+             "apply_weights_noalloc: this problem needs %zu scratch doubles (%lld samples, %d blocks), the "
+             "library's array holds %zu",
+             need, (long long) T.m, T.nb, kNoallocScratchDoubles);
+    MGPU_RET(noalloc_scratch_init(pb.device));
+    double *sc = g_scratch_ptr[pb.device];
+    // the tile variant comes from the handle's choice or the fixed default (no
+    // autotuning here: it would time launches with events)
+    const int keep_variant_opt = pb.opt.tile_variant;
+    if (pb.opt.tile_variant == MGPU_TILE_AUTOTUNE)
+        for (const TileClass &C : T.cls)
+            if (C.variant[0] == 0) pb.opt.tile_variant = 0;
+    const bool keep_prof = pb.prof.on;  // no profiling events either
+    pb.prof.on = false;
+    hipStream_t keep_stream = pb.stream;
+    if (stream) pb.stream = stream;
+    const int rc = toeplitz_apply_scratch(pb, d_in, d_out, sc);
+    pb.stream = keep_stream;
+    pb.prof.on = keep_prof;
+    pb.opt.tile_variant = keep_variant_opt;
+    return rc;
+}
+
 int toeplitz_apply_scratch(Problem &pb, const double *d_in, double *d_out, double *d_scratch) {
     const ToeplitzPlan &T = pb.tp;
     ToeplitzScratch sc;

@@ -36,7 +36,7 @@ EXPORTS = [
     "mappraiser_gpu_options_default", "mappraiser_gpu_problem_create",
     "mappraiser_gpu_problem_create_device", "mappraiser_gpu_problem_destroy",
     "mappraiser_gpu_local_shape", "mappraiser_gpu_matvec", "mappraiser_gpu_trmatvec",
-    "mappraiser_gpu_apply_weights", "mappraiser_gpu_precond_build", "mappraiser_gpu_precond_apply",
+    "mappraiser_gpu_apply_weights", "mappraiser_gpu_apply_weights_noalloc", "mappraiser_gpu_precond_build", "mappraiser_gpu_precond_apply",
     "mappraiser_gpu_pcg", "mappraiser_gpu_solve_device", "mappraiser_gpu_kernel_times",
     "mappraiser_gpu_tile_variants", "mappraiser_gpu_pcg_batch", "mappraiser_gpu_server_create",
     "mappraiser_gpu_server_submit", "mappraiser_gpu_server_wait", "mappraiser_gpu_server_stats",
@@ -152,6 +152,7 @@ def load():
     L.mappraiser_gpu_matvec.argtypes = [_vp, _f64p, _f64p]
     L.mappraiser_gpu_trmatvec.argtypes = [_vp, _f64p, _f64p]
     L.mappraiser_gpu_apply_weights.argtypes = [_vp, _f64p]
+    L.mappraiser_gpu_apply_weights_noalloc.argtypes = [_vp, _vp, _vp, _vp]
     L.mappraiser_gpu_precond_build.argtypes = [_vp, _vp, _vp, _vp, ct.POINTER(ct.c_int)]
     L.mappraiser_gpu_precond_apply.argtypes = [_vp, _f64p, _f64p]
     L.mappraiser_gpu_pcg.argtypes = [_vp, _f64p, _f64p, ct.c_double, ct.c_int, ct.c_int, _f64p, _vp,
@@ -451,6 +452,18 @@ class Problem:
         v = _c(tod, np.float64).copy()
         _check(load().mappraiser_gpu_apply_weights(self._h, v), "apply_weights")
         return v
+
+    def apply_weights_noalloc(self, d_in, d_out, stream=None):
+        """d_out <- N^-1 d_in on device vectors (integer device addresses, or
+        objects with data_ptr(); m float64 each) without any allocation; queued
+        on `stream` (a raw hipStream_t, None: the handle's stream), not
+        synchronised."""
+        def ptr(a):
+            return a.data_ptr() if hasattr(a, "data_ptr") else int(a)
+
+        _check(load().mappraiser_gpu_apply_weights_noalloc(self._h, _vp(ptr(d_in)), _vp(ptr(d_out)),
+                                                           _vp(stream) if stream else None),
+               "apply_weights_noalloc")
 
     def precond_build(self, signal=None):
         """Returns (signal after the cut, cond, lhits)."""
