@@ -195,7 +195,8 @@ int mappraiser_gpu_solve_device(mgpu_problem *pb, double *d_signal, const double
  * milliseconds and launch counts per kernel class (MGPU_KT_*), measured with
  * HIP events on the handle's stream; reset != 0 clears them afterwards.  With
  * pb == NULL: the totals over every drop-in solve (mappraiser_gpu_pcg_gls,
- * MLmap, MTmap) run with opt.profile = 1 in this process.  No reference
+ * MLmap, MTmap) and batch (mappraiser_gpu_pcg_batch) run with opt.profile = 1
+ * in this process.  No reference
  * counterpart (the reference prints MPI_Wtime splits,
  * pcg_true.c:27,98-99,178-179). */
 int mappraiser_gpu_kernel_times(mgpu_problem *pb, double *ms, long long *counts, int reset);
@@ -277,8 +278,8 @@ int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int grid_npix, 
  * on (its report then carries no solve time and, in MGPU_MIXED, no FP64 true
  * residual).  Results are bit-identical to the requests' solo solves.
  * hbm_budget_gb <= 0 uses 80% of the free device memory.  Footprints are
- * estimated per sample and per grid pixel and scaled by the largest ratio of
- * measured to estimated bytes seen in finished groups.  A group whose device
+ * estimated per sample and per grid pixel and, once groups have finished,
+ * scaled by 1.05 x the largest ratio of measured to estimated bytes seen.  A group whose device
  * allocations fail while other solves run, or that merged several requests,
  * returns to the queue head; its requests then run one at a time on an idle
  * device (a failure there is reported).  A merged group stays within 2^30

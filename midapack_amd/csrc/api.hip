@@ -22,6 +22,19 @@ static thread_local char g_err[1024] = "";
 static std::mutex g_kt_mu;
 static double g_kt_ms[MGPU_NKT] = {0};
 static long long g_kt_cnt[MGPU_NKT] = {0};
+
+// adds a finished solve's kernel-class times (opt.profile) to the process totals
+int kernel_times_accumulate(Problem &pb) {
+    if (!pb.prof.on) return MGPU_OK;
+    MGPU_HIP(hipStreamSynchronize(pb.stream));
+    pb.prof.collect();
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    for (int i = 0; i < MGPU_NKT; i++) {
+        g_kt_ms[i] += pb.prof.ms[i];
+        g_kt_cnt[i] += pb.prof.cnt[i];
+    }
+    return MGPU_OK;
+}
 static int g_device = 0;
 
 void set_error(const char *fmt, ...) {
@@ -579,15 +592,7 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
     }
     r.lcount = lcount;
     r.trash_pix = trash;
-    if (pb.prof.on) {
-        MGPU_HIP(hipStreamSynchronize(s));
-        pb.prof.collect();
-        std::lock_guard<std::mutex> g(g_kt_mu);
-        for (int i = 0; i < MGPU_NKT; i++) {
-            g_kt_ms[i] += pb.prof.ms[i];
-            g_kt_cnt[i] += pb.prof.cnt[i];
-        }
-    }
+    MGPU_RET(kernel_times_accumulate(pb));
     if (outpath && ref) {
         char fn[4096];
         std::snprintf(fn, sizeof(fn), "%s/pcg_residuals_%s.dat", outpath, ref);

@@ -235,6 +235,7 @@ int solve_items(mgpu_batch_item *items, const std::vector<int> &idx, const std::
         ir.true_res_rel = segs[r].true_res_rel;
         ir.dev_peak_bytes = (long long) P.arena.peak;
     }
+    MGPU_RET(kernel_times_accumulate(P));
     return MGPU_OK;
 }
 

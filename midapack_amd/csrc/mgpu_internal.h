@@ -590,6 +590,9 @@ int pcg_batch_impl(mgpu_batch_item *items, const std::vector<int> &all, const st
                    const std::vector<double> &tol_of, const std::vector<int> &maxiter_of, bool on_dev,
                    const mgpu_options *opt, const std::function<void(int)> *done);
 int check_item(const mgpu_batch_item &it, int i);
+// adds a finished solve's kernel-class times (opt.profile) to the process totals
+// that mappraiser_gpu_kernel_times(NULL, ...) reports
+int kernel_times_accumulate(Problem &pb);
 // invtt rows [nb*lambda] (device) from the noise timestream (device), see mappraiser_gpu_noise_invtt
 int noise_invtt(Problem &pb, const double *d_noise, int nb, const int *block_sizes, int lambda, double fsamp,
                 int window_kind, double *d_invtt, double *h_params);

@@ -15,9 +15,10 @@
 // request's solo solve: batching changes neither the kernels nor the
 // summation order of an item.
 //
-// Footprints are estimated per sample and per grid pixel and corrected by
-// what finished groups actually held (their arena's high-water mark), so the
-// budget tracks the real usage.  A group whose allocations fail while other
+// Footprints are estimated per sample and per grid pixel (a bound from the
+// allocations of a solve) and, once groups have finished, scaled by the
+// largest ratio of what a group actually held (its arena's high-water mark)
+// to its estimate, plus 5%, so the budget tracks the real usage.  A group whose allocations fail while other
 // solves hold memory goes back to the queue head; such requests are later
 // admitted one at a time and only onto an idle device, where a second failure
 // is final.
@@ -66,7 +67,7 @@ struct Server {
     std::unordered_map<long long, std::unique_ptr<Request>> reqs;
     std::list<Worker> workers;
     int64_t budget = 0, used = 0;
-    double correction = 1.0;  // max over finished groups of (measured peak / estimate)
+    double correction = 0.0;  // max over finished groups of 1.05 x (measured peak / estimate); 0: none yet
     int max_slots = 4, max_batch = 16, running = 0;
     int device = 0;  // the device current when the server was created
     long long next_ticket = 1;
@@ -186,7 +187,7 @@ void Server::run_dispatcher() {
                 ++it;
             }
         }
-        auto cost = [&](const Request *r) { return (int64_t) ((double) r->est * correction); };
+        auto cost = [&](const Request *r) { return (int64_t) ((double) r->est * (correction > 0 ? correction : 1.0)); };
         auto fits = [&](const Request *r) {
             if (r->retries > 0) return running == 0;  // failed beside others: run alone
             return running == 0 || used + cost(r) <= budget;

@@ -516,7 +516,7 @@ class Problem:
 
 def dropin_kernel_times(reset=False):
     """{class: (total ms, launches)} over the drop-in solves (pcg_gls, MLmap, MTmap)
-    run with options(profile=True) in this process."""
+    and batches run with options(profile=True) in this process."""
     ms = np.zeros(len(KERNEL_CLASSES), np.float64)
     cnt = np.zeros(len(KERNEL_CLASSES), np.int64)
     _check(load().mappraiser_gpu_kernel_times(None, ms.ctypes.data, cnt.ctypes.data, int(reset)), "kernel_times")
