@@ -194,6 +194,11 @@ struct TileClass {
     TileDesc *d_tiles32 = nullptr;
     float *d_coef32L = nullptr;      // nfilters * (N32/2) (a_k, b_k)
     float *d_tw32L = nullptr;
+    // host copies of both tile lists in sample order, and the tiles of the
+    // segments still running during a segmented PCG (-1: every tile)
+    std::vector<TileDesc> h_tiles, h_tiles32;
+    TileDesc *d_tiles_run = nullptr, *d_tiles32_run = nullptr;
+    int ntiles_run = -1, ntiles32_run = -1;
 };
 
 struct ToeplitzPlan {
@@ -262,6 +267,9 @@ struct Pointing {
     float2 *d_cw12f = nullptr;
     int *d_pixg = nullptr; // [npix0] global pixel number p (valid pixels, increasing)
     int *d_chunk_perm = nullptr; // [ceil(m/kChunk)] chunk processed by P's workgroup b
+    std::vector<int> h_chunk_order; // the chunks in P's order before they are dealt to the XCDs
+    int *d_perm_run = nullptr;   // chunks of the segments still running (segmented PCG)
+    int nch_run = -1;            // their count (-1: every chunk, d_chunk_perm)
 };
 
 // ---------------------------------------------------------------------------
@@ -368,7 +376,28 @@ struct Segments {
     // that P, N^-1 and P^T skip the work of stopped segments
     int *d_chunk_seg = nullptr;
     int *d_block_seg = nullptr;
+    std::vector<int> h_chunk_seg, h_block_seg;  // host copies
 };
+
+// Work lists of a segmented PCG restricted to its running segments: P's chunks
+// (dealt to the XCDs again) and every tile class's tiles.  While the PCG loop
+// runs (Problem::live_state set) the P and tile launches use them, so stopped
+// segments cost no workgroups; run_lists_clear restores the full lists.
+struct Problem;
+int run_lists_update(Problem &pb, const std::vector<int> &state);
+void run_lists_clear(Problem &pb);
+// XCD x runs workgroups b = x, x + 8, ...: position b of the result holds element
+// off[b % 8] + b / 8 of `order`, so each XCD gets a contiguous run of it
+template <typename T>
+std::vector<T> deal_to_xcds(const std::vector<T> &order) {
+    constexpr int kX = 8;
+    const int n = (int) order.size();
+    int off[kX + 1] = {0};
+    for (int x = 0; x < kX; x++) off[x + 1] = off[x] + (x < n ? (n - 1 - x) / kX + 1 : 0);
+    std::vector<T> o(order.size());
+    for (int b = 0; b < n; b++) o[b] = order[off[b % kX] + b / kX];
+    return o;
+}
 
 // ---------------------------------------------------------------------------
 // Two-level preconditioner (build_precond precond = 1, 2): deflation space Z,

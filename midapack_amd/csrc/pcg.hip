@@ -470,8 +470,61 @@ int segments_ensure(Problem &pb) {
     MGPU_HIP(hipMemcpyAsync(G.d_chunk_seg, cseg.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
     MGPU_HIP(hipMemcpyAsync(G.d_block_seg, bseg.data(), sizeof(int) * bseg.size(), hipMemcpyHostToDevice, s));
     MGPU_HIP(hipStreamSynchronize(s));
+    G.h_chunk_seg = std::move(cseg);
+    G.h_block_seg = std::move(bseg);
     G.npix = npix;
     return MGPU_OK;
+}
+
+int run_lists_update(Problem &pb, const std::vector<int> &state) {
+    const Segments &G = pb.sg;
+    if (G.h_chunk_seg.empty()) return MGPU_OK;
+    hipStream_t s = pb.stream;
+    auto running = [&](int sg) { return sg < 0 || state[sg] == 0; };
+    // the lists are built on the host and copied while the stream is idle
+    // (between blocks of iterations); the host vectors live until the sync below
+    std::vector<std::vector<int>> keep_perm;
+    std::vector<std::vector<TileDesc>> keep_tiles;
+    Pointing &P = pb.pt;
+    if (!P.h_chunk_order.empty()) {
+        std::vector<int> ord;
+        ord.reserve(P.h_chunk_order.size());
+        for (int c : P.h_chunk_order)
+            if (running(G.h_chunk_seg[c])) ord.push_back(c);
+        if (!P.d_perm_run) MGPU_RET(pb.arena.alloc(&P.d_perm_run, P.h_chunk_order.size()));
+        keep_perm.push_back(deal_to_xcds(ord));
+        const std::vector<int> &perm = keep_perm.back();
+        if (!perm.empty())
+            MGPU_HIP(hipMemcpyAsync(P.d_perm_run, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice, s));
+        P.nch_run = (int) perm.size();
+    }
+    auto tiles = [&](const std::vector<TileDesc> &all, TileDesc *&d, int &n) -> int {
+        if (all.empty()) return MGPU_OK;
+        std::vector<TileDesc> run;
+        run.reserve(all.size());
+        for (const TileDesc &t : all)
+            if (running(G.h_block_seg[t.block])) run.push_back(t);
+        if (!d) MGPU_RET(pb.arena.alloc(&d, all.size()));
+        keep_tiles.push_back(deal_to_xcds(run));
+        const std::vector<TileDesc> &o = keep_tiles.back();
+        if (!o.empty()) MGPU_HIP(hipMemcpyAsync(d, o.data(), sizeof(TileDesc) * o.size(), hipMemcpyHostToDevice, s));
+        n = (int) o.size();
+        return MGPU_OK;
+    };
+    for (TileClass &C : pb.tp.cls) {
+        MGPU_RET(tiles(C.h_tiles, C.d_tiles_run, C.ntiles_run));
+        MGPU_RET(tiles(C.h_tiles32, C.d_tiles32_run, C.ntiles32_run));
+    }
+    MGPU_HIP(hipStreamSynchronize(s));
+    return MGPU_OK;
+}
+
+void run_lists_clear(Problem &pb) {
+    pb.pt.nch_run = -1;
+    for (TileClass &C : pb.tp.cls) {
+        C.ntiles_run = -1;
+        C.ntiles32_run = -1;
+    }
 }
 
 int ensure_work(Problem &pb) {
@@ -637,9 +690,13 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     // segment is computed again after the loop, for the true residual)
     struct LiveState {
         Problem &pb;
-        ~LiveState() { pb.live_state = nullptr; }
+        ~LiveState() {
+            pb.live_state = nullptr;
+            run_lists_clear(pb);
+        }
     } live{pb};
     pb.live_state = state;
+    run_lists_clear(pb);
 
     const int poll = pb.opt.poll_every > 0 ? pb.opt.poll_every : 8;
     const int refresh = mixed ? (pb.opt.refresh_every > 0 ? pb.opt.refresh_every : 10) : 0;
@@ -747,6 +804,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     bool done = maxiter <= 1;  // maxiter == 1: no loop iteration (for k = 1; k < K)
     bool pending = false;      // mixed precision: some segment awaits its FP64 refresh
     int ndone_seen = 0;        // stopped segments at the last report
+    int ndone_lists = 0;       // stopped segments when the work lists were last rebuilt
     while (!done) {
         if (pending) {
             // an eager block whose first iteration refreshes the pending segments;
@@ -781,9 +839,27 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             ndone_seen = hf[F_NDONE];
             MGPU_RET(report_stopped());
         }
+        if (nseg > 1 && hf[F_NDONE] > ndone_lists && !done) {
+            // segments stopped: later blocks launch only the running segments'
+            // chunks and tiles (a captured block is captured again with them)
+            ndone_lists = hf[F_NDONE];
+            std::vector<int> hstate((size_t) nseg);
+            MGPU_HIP(hipMemcpyAsync(hstate.data(), state, sizeof(int) * nseg, hipMemcpyDeviceToHost, s));
+            MGPU_HIP(hipStreamSynchronize(s));
+            MGPU_RET(run_lists_update(pb, hstate));
+            if (gb.exec) {
+                (void) hipGraphExecDestroy(gb.exec);
+                gb.exec = nullptr;
+            }
+            if (gb.graph) {
+                (void) hipGraphDestroy(gb.graph);
+                gb.graph = nullptr;
+            }
+        }
     }
     pb.prof.tag = 0;
     pb.live_state = nullptr;
+    run_lists_clear(pb);
     phase.reset();
     MGPU_HIP(hipEventRecord(pb.ev1, s));
     // results

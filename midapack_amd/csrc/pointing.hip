@@ -269,8 +269,6 @@ static void release_fp32(Problem &pb) {
 // rows (other detectors' passes over them) run on one XCD close together in
 // time and find those rows in that XCD's L2 instead of the MALL.
 // ---------------------------------------------------------------------------
-constexpr int kXcds = 8;
-
 __global__ __launch_bounds__(256) void k_chunk_key(const int *__restrict__ q, int64_t m, int *__restrict__ key) {
     __shared__ int red[4];
     const int64_t c0 = (int64_t) blockIdx.x * kChunk;
@@ -325,10 +323,9 @@ static int build_chunk_order(Problem &pb) {
         for (int g : gord)
             for (int c = g * kChunkGroup; c < std::min(nch, (g + 1) * kChunkGroup); c++) order[k++] = c;
     }
-    // XCD x runs workgroups b = x, x + 8, ...: give it a contiguous run of the sorted chunks
-    int off[kXcds + 1] = {0};
-    for (int x = 0; x < kXcds; x++) off[x + 1] = off[x] + (x < nch ? (nch - 1 - x) / kXcds + 1 : 0);
-    for (int b = 0; b < nch; b++) perm[b] = order[off[b % kXcds] + b / kXcds];
+    // each XCD gets a contiguous run of the sorted chunks
+    perm = deal_to_xcds(order);
+    P.h_chunk_order = order;
     if (!P.d_chunk_perm) MGPU_RET(pb.arena.alloc(&P.d_chunk_perm, (size_t) nch));
     MGPU_HIP(hipMemcpyAsync(P.d_chunk_perm, perm.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
     MGPU_HIP(hipStreamSynchronize(s));
@@ -651,7 +648,7 @@ template <typename TX, typename TW, typename TW2>
 static int launch_matvec_t(Problem &pb, const TX *d_x, TX *d_y, bool partials, const TW *w0, const TW2 *w12) {
     const Pointing &P = pb.pt;
     const ToeplitzPlan &T = pb.tp;
-    const unsigned g = grid_for(P.m, kChunk);
+    unsigned g = grid_for(P.m, kChunk);
     partials = partials && T.nb > 0 && T.d_partials && T.d_chunk_block && T.lambda > 1 &&
                T.offset_mode != MGPU_OFFSET_NONE;
     const int *q = P.d_q;
@@ -659,6 +656,11 @@ static int launch_matvec_t(Problem &pb, const TX *d_x, TX *d_y, bool partials, c
     const int *cb = T.d_chunk_block;
     const int *pm = P.d_chunk_perm;
     MGPU_ARG(pm, "pointing has no chunk order");
+    if (pb.live_state && P.nch_run >= 0) {  // segmented PCG: the running segments' chunks only
+        pm = P.d_perm_run;
+        g = (unsigned) P.nch_run;
+        if (g == 0) return MGPU_OK;
+    }
     const int *cs = pb.sg.d_chunk_seg, *ls = cs ? pb.live_state : nullptr;
     if (P.unit_w0) {
         if (partials)

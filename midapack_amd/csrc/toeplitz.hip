@@ -1124,15 +1124,7 @@ static int max_log2n(bool fp32) { return fp32 ? 16 : 15; }
 // dealt a contiguous run of tiles (in sample order).  Neighbouring tiles,
 // which read half of each other's input, then run on one XCD at about the
 // same time and share those reads (and their filter's spectrum) in its L2.
-static std::vector<TileDesc> xcd_order(const std::vector<TileDesc> &t) {
-    constexpr int kXcds = 8;
-    const int n = (int) t.size();
-    int off[kXcds + 1] = {0};
-    for (int x = 0; x < kXcds; x++) off[x + 1] = off[x] + (x < n ? (n - 1 - x) / kXcds + 1 : 0);
-    std::vector<TileDesc> o(t.size());
-    for (int b = 0; b < n; b++) o[b] = t[off[b % kXcds] + b / kXcds];
-    return o;
-}
+static std::vector<TileDesc> xcd_order(const std::vector<TileDesc> &t) { return deal_to_xcds(t); }
 static int min_log2n() { return 10; }
 
 // tile length choice: minimise N (log2 N + c) / L over the sizes that fit LDS
@@ -1395,6 +1387,7 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
         }
         C.ntiles = (int) tiles.size();
         MGPU_RET(pb.arena.alloc(&C.d_tiles, tiles.size()));
+        C.h_tiles = tiles;
         tiles = xcd_order(tiles);
         MGPU_HIP(hipMemcpyAsync(C.d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice, s));
         MGPU_RET(pb.arena.alloc(&C.d_coef64, (size_t) C.nfilters * NC * 2));
@@ -1425,6 +1418,7 @@ int toeplitz_plan(Problem &pb, const int *block_sizes, int nb, int lambda, const
             }
             C.ntiles32 = (int) tf.size();
             MGPU_RET(pb.arena.alloc(&C.d_tiles32, tf.size()));
+            C.h_tiles32 = tf;
             tf = xcd_order(tf);
             MGPU_HIP(hipMemcpyAsync(C.d_tiles32, tf.data(), sizeof(TileDesc) * tf.size(), hipMemcpyHostToDevice, s));
             MGPU_RET(pb.arena.alloc(&C.d_coef32L, (size_t) C.nfilters * NCf * 2));
@@ -1523,18 +1517,25 @@ struct TileUse {
     const TileDesc *tiles;
     const void *coef, *tw;
 };
+// the tiles of a launch: during a segmented PCG (live), those of the running segments
 template <typename R>
-static TileUse tile_use(const TileClass &C) {
-    if (sizeof(R) == 4 && C.log2n32 > 0 && C.fp32_long)
+static TileUse tile_use(const TileClass &C, bool live) {
+    if (sizeof(R) == 4 && C.log2n32 > 0 && C.fp32_long) {
+        if (live && C.ntiles32_run >= 0) return {C.log2n32, C.L32, C.ntiles32_run, C.d_tiles32_run, C.d_coef32L, C.d_tw32L};
         return {C.log2n32, C.L32, C.ntiles32, C.d_tiles32, C.d_coef32L, C.d_tw32L};
-    if (sizeof(R) == 4) return {C.log2n, C.L, C.ntiles, C.d_tiles, C.d_coef32, C.d_tw32};
-    return {C.log2n, C.L, C.ntiles, C.d_tiles, C.d_coef64, C.d_tw64};
+    }
+    const bool run = live && C.ntiles_run >= 0;
+    const int n = run ? C.ntiles_run : C.ntiles;
+    const TileDesc *t = run ? C.d_tiles_run : C.d_tiles;
+    if (sizeof(R) == 4) return {C.log2n, C.L, n, t, C.d_coef32, C.d_tw32};
+    return {C.log2n, C.L, n, t, C.d_coef64, C.d_tw64};
 }
 
 template <typename R, int LOGNC, int LOGV, int LOGR, bool TWO, int MINW>
 static int launch_tiles_cfg(Problem &pb, const TileClass &C, const R *d_in, R *d_out, const void *coef,
                             const void *tw, const ToeplitzScratch &sc) {
-    const TileUse u = tile_use<R>(C);
+    const TileUse u = tile_use<R>(C, pb.live_state != nullptr);
+    if (u.ntiles == 0) return MGPU_OK;
     constexpr int NC = 1 << LOGNC, NT = NC / (1 << LOGV);
     constexpr size_t lds = sizeof(R) * (size_t) lds_elems(NC) * (TWO ? 2 : 1);
     static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
@@ -1556,6 +1557,8 @@ static int launch_tiles_wl(Problem &pb, const TileClass &C, const R *d_in, R *d_
     const void *coef = sizeof(R) == 4 ? (const void *) C.d_coefW32 : (const void *) C.d_coefW64;
     const void *tw = sizeof(R) == 4 ? (const void *) C.d_tw32 : (const void *) C.d_tw64;
     MGPU_ARG(coef && C.log2n == 15 && !(sizeof(R) == 4 && C.fp32_long), "wave-local tiles need N = 32768");
+    const TileUse u = tile_use<R>(C, pb.live_state != nullptr);
+    if (u.ntiles == 0) return MGPU_OK;
     constexpr size_t lds = sizeof(R) * (size_t) kWlLdsElems;
     static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
     auto kern = k_tiles_wl<R, MINW>;
@@ -1564,9 +1567,9 @@ static int launch_tiles_wl(Problem &pb, const TileClass &C, const R *d_in, R *d_
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
         attr_devices |= 1u << pb.device;
     }
-    TileArgs a{d_in, d_out, C.d_tiles, coef, tw, sc.mean, sc.edge_part, C.lambda, C.L, pb.sg.d_block_seg,
+    TileArgs a{d_in, d_out, u.tiles, coef, tw, sc.mean, sc.edge_part, C.lambda, C.L, pb.sg.d_block_seg,
                pb.sg.d_block_seg ? pb.live_state : nullptr};
-    hipLaunchKernelGGL(kern, dim3(C.ntiles), dim3(kWlNT), lds, pb.stream, a);
+    hipLaunchKernelGGL(kern, dim3(u.ntiles), dim3(kWlNT), lds, pb.stream, a);
     MGPU_LAUNCHED();
     return MGPU_OK;
 }
@@ -1575,7 +1578,7 @@ template <typename R>
 static int launch_tiles(Problem &pb, const TileClass &C, const R *d_in, R *d_out, int variant,
                         const ToeplitzScratch &sc) {
     constexpr bool F32 = sizeof(R) == 4;
-    const TileUse u = tile_use<R>(C);
+    const TileUse u = tile_use<R>(C, pb.live_state != nullptr);
     const void *coef = u.coef, *tw = u.tw;
 #define MGPU_TL(LOGNC, LOGV, LOGR, TWO, MINW) \
     return launch_tiles_cfg<R, LOGNC, LOGV, LOGR, TWO, MINW>(pb, C, d_in, d_out, coef, tw, sc)
@@ -1647,7 +1650,7 @@ static int pick_variant(Problem &pb, TileClass &T, const R *d_in, R *d_out, cons
     if (T.variant[slot]) return MGPU_OK;
     // FP32 products use the FP64 tile length unless autotuning picks the long set
     if (slot == 1) T.fp32_long = false;
-    const int log2n = tile_use<R>(T).log2n;
+    const int log2n = tile_use<R>(T, false).log2n;
     const int nv = tile_variant_count(slot == 1, log2n);
     MGPU_ARG(nv > 0, "unsupported tile length 2^%d", log2n);
     // a forced variant applies to the precision of the PCG iterations (FP32 under
