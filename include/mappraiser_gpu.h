@@ -42,7 +42,10 @@ typedef struct mgpu_problem mgpu_problem;
 
 typedef struct mgpu_options {
     int precision;      /* MGPU_FP64 or MGPU_MIXED */
-    int refresh_every;  /* MGPU_MIXED: recompute the FP64 residual every k iterations (default 10) */
+    int refresh_every;  /* MGPU_MIXED: recompute the FP64 residual every k iterations (default 10);
+                           MGPU_REFRESH_ADAPTIVE: when a segment's FP32-updated residual norm has
+                           fallen 10x below its last FP64 value or 64 iterations after it, at
+                           the start of the next poll block */
     int tile_log2;      /* real FFT tile length 2^k of the Toeplitz product; 0 = autotuned choice */
     int poll_every;     /* PCG: iterations launched between host convergence polls (default 8) */
     int profile;        /* 1: time every kernel class with HIP events on the handle's stream */
@@ -77,6 +80,7 @@ typedef struct mgpu_options {
  *   MGPU_OFFSET_NONE       N^-1 = T (no offset removal).
  * lambda == 1 (white noise) never removes offsets, as in apply_weights (pcg_true.c:583-593). */
 #define MGPU_TILE_AUTOTUNE (-1)  /* mgpu_options.tile_variant: opt into timing-based variant choice */
+#define MGPU_REFRESH_ADAPTIVE (-1)  /* mgpu_options.refresh_every: residual refreshes on demand */
 
 #define MGPU_OFFSET_REFERENCE 0
 #define MGPU_OFFSET_SYMMETRIC 1

@@ -372,6 +372,7 @@ struct Segments {
     int *d_wg_seg = nullptr;  // nwg: segment of each workgroup
     int *d_wg_q0 = nullptr;   // nwg: first local pixel of each workgroup
     int *d_state = nullptr;   // nseg: PCG state (0 running, 1 converged, 2 stopped)
+    int *d_rstate = nullptr;  // nseg: nonzero for the segments an FP64 refresh of pending segments skips
     // segment of every P chunk and Toeplitz block (-1: none, e.g. padding), so
     // that P, N^-1 and P^T skip the work of stopped segments
     int *d_chunk_seg = nullptr;

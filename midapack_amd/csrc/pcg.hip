@@ -106,6 +106,8 @@ __global__ __launch_bounds__(kRB) void k_init_scalars(const double *__restrict__
         Sr[S_RES0] = res;
         Sr[S_TOL2] = Sr[S_TOL] * Sr[S_TOL] * res;
         Sr[S_K] = 0.0;
+        Sr[S_RREF] = res;
+        Sr[S_KREF] = 0.0;
         if (hist) hist[(int64_t) r * maxiter] = sqrt(res) / sqrt(res);
         if (np == 0) {  // empty segment (every pixel cut): nothing to solve
             state[r] = 2;
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(kMB) void k_update_xg(double *__restrict__ x, const
 __global__ __launch_bounds__(kRB) void k_scalars(const double *__restrict__ part, int nwg,
                                                  const int *__restrict__ wg0, double *__restrict__ S,
                                                  double *__restrict__ hist, int maxiter, int *__restrict__ state,
-                                                 int nseg, int *__restrict__ flag, int kind) {
+                                                 int nseg, int *__restrict__ flag, int kind, int adaptive) {
     if (stopped(flag)) return;
     const int r = blockIdx.x;
     if (state[r]) return;
@@ -232,12 +234,17 @@ __global__ __launch_bounds__(kRB) void k_scalars(const double *__restrict__ part
         const bool refreshed = kind == IT_REFRESH_ALL || (kind == IT_REFRESH_PENDING && pend);
         const bool verify = kind != IT_FP64 && !refreshed;
         bool pend_now = pend && !refreshed;
+        if (refreshed) {
+            Sr[S_RREF] = res;
+            Sr[S_KREF] = k;
+        }
         int st = 0;
         if (res <= Sr[S_TOL2] && !verify)
             st = 1;
         else if ((Sr[S_STOPK] > 0.0 && k >= Sr[S_STOPK]) || k + 1.0 >= Sr[S_MAXK])
             st = 2;
         if (res <= Sr[S_TOL2] && verify) pend_now = true;
+        if (adaptive && verify && (res <= kAdaptDrop * Sr[S_RREF] || k - Sr[S_KREF] >= kAdaptGap)) pend_now = true;
         if (st) pend_now = false;
         if (pend_now != pend) {
             Sr[S_PEND] = pend_now ? 1.0 : 0.0;
@@ -270,6 +277,13 @@ __global__ __launch_bounds__(kMB) void k_hupdate(double *__restrict__ h, const d
 // segment r skips the refresh of an IT_REFRESH_PENDING iteration unless it is pending
 __device__ __forceinline__ bool skip_refresh(const double *__restrict__ S, int kind, int r) {
     return kind == IT_REFRESH_PENDING && S[(int64_t) r * S_NSCAL + S_PEND] == 0.0;
+}
+
+// segments an IT_REFRESH_PENDING iteration's FP64 operator skips: stopped or not pending
+__global__ void k_refresh_mask(const int *__restrict__ state, const double *__restrict__ S, int nseg,
+                               int *__restrict__ rstate) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nseg) rstate[r] = state[r] != 0 || S[(int64_t) r * S_NSCAL + S_PEND] == 0.0;
 }
 
 // FP64 refresh of the residual: g = b - Ax (mixed precision); state == nullptr
@@ -430,6 +444,8 @@ int segments_ensure(Problem &pb) {
     if (G.d_state && G.nseg != nseg) {
         pb.arena.release(G.d_state);
         G.d_state = nullptr;
+        if (G.d_rstate) pb.arena.release(G.d_rstate);
+        G.d_rstate = nullptr;
     }
     G.nseg = nseg;
     const size_t nw = std::max<size_t>(wseg.size(), 1);
@@ -438,6 +454,7 @@ int segments_ensure(Problem &pb) {
     MGPU_RET(pb.arena.alloc(&G.d_wg_seg, nw));
     MGPU_RET(pb.arena.alloc(&G.d_wg_q0, nw));
     if (!G.d_state) MGPU_RET(pb.arena.alloc(&G.d_state, (size_t) nseg));
+    if (!G.d_rstate) MGPU_RET(pb.arena.alloc(&G.d_rstate, (size_t) nseg));
     MGPU_HIP(hipMemcpyAsync(G.d_q, G.h_q.data(), sizeof(int) * (nseg + 1), hipMemcpyHostToDevice, s));
     MGPU_HIP(hipMemcpyAsync(G.d_wg0, G.h_wg0.data(), sizeof(int) * (nseg + 1), hipMemcpyHostToDevice, s));
     if (!wseg.empty()) {
@@ -699,7 +716,9 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     run_lists_clear(pb);
 
     const int poll = pb.opt.poll_every > 0 ? pb.opt.poll_every : 8;
-    const int refresh = mixed ? (pb.opt.refresh_every > 0 ? pb.opt.refresh_every : 10) : 0;
+    // refresh period of mixed precision; adaptive: no period, segments ask (S_PEND)
+    const bool adaptive = mixed && pb.opt.refresh_every == MGPU_REFRESH_ADAPTIVE;
+    const int refresh = !mixed || adaptive ? 0 : (pb.opt.refresh_every > 0 ? pb.opt.refresh_every : 10);
     // Iterations are queued in blocks between host polls.  A graph block must
     // repeat exactly: an even length (g/gp swap parity) and, in mixed
     // precision, a multiple of the refresh period; eager blocks have the same length.
@@ -719,7 +738,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
         ++launched;
         pb.prof.tag = launched;
         const bool all = refresh && launched % refresh == 0;
-        const bool do_refresh = all || (refresh && refresh_pending);
+        const bool do_refresh = all || (mixed && refresh_pending);
         const int kind = !mixed ? IT_FP64 : all ? IT_REFRESH_ALL : do_refresh ? IT_REFRESH_PENDING : IT_MIXED;
         std::swap(g, gp);  // pcg_true.c:140-142
         // q = A h with fused <h, q>
@@ -744,7 +763,17 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             pb.prof.begin(MGPU_KT_REFRESH, s);
             const bool keep = pb.prof.on;
             pb.prof.on = false;  // the FP64 operator inside is accounted as a refresh
-            MGPU_RET(apply_A64(pb, x, Ax, nullptr));
+            if (kind == IT_REFRESH_PENDING && nseg > 1) {
+                // only the pending segments need A x: the others' chunks and tiles return at once
+                k_refresh_mask<<<grid_for(nseg, 64), 64, 0, s>>>(state, S, nseg, G.d_rstate);
+                MGPU_LAUNCHED();
+                pb.live_state = G.d_rstate;
+                const int rc = apply_A64(pb, x, Ax, nullptr);
+                pb.live_state = state;
+                MGPU_RET(rc);
+            } else {
+                MGPU_RET(apply_A64(pb, x, Ax, nullptr));
+            }
             k_refresh<<<nwg, kMB, 0, s>>>(bvec, Ax, g, wseg, wq0, sq, state, flag, flag + F_NREF, S, kind);
             MGPU_LAUNCHED();
             if (two) {
@@ -759,7 +788,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             pb.prof.end(s);
         }
         pb.prof.begin(MGPU_KT_MAP, s);
-        k_scalars<<<nseg, kRB, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag, kind);
+        k_scalars<<<nseg, kRB, 0, s>>>(part, nwg, wg0, S, d_hist, maxiter, state, nseg, flag, kind, adaptive ? 1 : 0);
         MGPU_LAUNCHED();
         k_hupdate<<<nwg, kMB, 0, s>>>(h, Cg, wseg, wq0, sq, S, state, flag);
         MGPU_LAUNCHED();

@@ -25,9 +25,22 @@ enum {
     S_TRUE_R2,  // ||b - A x||^2 of the returned x (mixed precision)
     S_TRUE_B2,  // ||b||^2
     S_PEND,     // mixed precision: 1 while an FP32-updated residual of this segment
-                // met the stop test and awaits its FP64 refresh
-    S_NSCAL = 16
+                // met the stop test (or, with adaptive refreshes, drifted far enough
+                // from its last FP64 value) and awaits its FP64 refresh
+    S_RREF,     // adaptive refreshes: ||g||^2 of the last FP64 residual
+    S_KREF,     // adaptive refreshes: iteration of the last FP64 residual
+    S_NSCAL = 18
 };
+
+// Adaptive refreshes (MGPU_REFRESH_ADAPTIVE): a segment asks for an FP64
+// residual once its FP32-updated ||g||^2 has fallen below kAdaptDrop times the
+// last FP64 one (the norm by 10x), or kAdaptGap iterations after it.  The
+// recurrence's drift grows with the largest residual since the last
+// replacement, so replacing it at every decade keeps the drift a small
+// fraction of the current residual (reliable updates, Sleijpen and van der
+// Vorst 1996).
+constexpr double kAdaptDrop = 1e-2;
+constexpr double kAdaptGap = 64.0;
 
 // device flags: [0] every segment stopped, [1] segment 0 converged, [2] largest
 // iteration count reached, [3] number of stopped segments, [4] never set,

@@ -23,6 +23,7 @@ MGPU_OFFSET_REFERENCE = 0
 MGPU_OFFSET_SYMMETRIC = 1
 MGPU_OFFSET_NONE = 2
 MGPU_TILE_AUTOTUNE = -1
+MGPU_REFRESH_ADAPTIVE = -1
 MGPU_APOD_CHEBWIN = 0
 MGPU_APOD_GAUSSIAN = 1
 
