@@ -281,9 +281,15 @@ int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int grid_npix, 
  * as soon as its own map has converged, while the rest of its group iterates
  * on (its report then carries no solve time and, in MGPU_MIXED, no FP64 true
  * residual).  Results are bit-identical to the requests' solo solves.
- * hbm_budget_gb <= 0 uses 80% of the free device memory.  Footprints are
+ * At creation the server reserves 80% of the free device memory (after
+ * returning the library's cached blocks) as one pool that all its solves
+ * allocate from, so serving makes no device allocation or free through the
+ * HIP runtime; create it while no other solve of the process runs on the
+ * device.  The admission budget is that pool, or hbm_budget_gb when smaller
+ * (<= 0: the pool).  Footprints are
  * estimated per sample and per grid pixel and, once groups have finished,
- * scaled by 1.05 x the largest ratio of measured to estimated bytes seen.  A group whose device
+ * raised by 1.05 x the largest ratio of measured to estimated bytes seen when
+ * that exceeds 1.  A group whose device
  * allocations fail while other solves run, or that merged several requests,
  * returns to the queue head; its requests then run one at a time on an idle
  * device (a failure there is reported).  A merged group stays within 2^30

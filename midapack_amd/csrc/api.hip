@@ -5,6 +5,7 @@
 // thread is kept for mappraiser_gpu_last_error().  Nothing here calls exit()
 // (the reference exits on allocation or LAPACK failures, mappraiser.c:120-123,
 // precond.c:485-488, 577).
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -58,7 +59,108 @@ static size_t size_class(size_t bytes) {
     return (bytes + 2 * mib - 1) / (2 * mib) * (2 * mib);
 }
 
+// ---------------------------------------------------------------------------
+// reserved device pools
+// ---------------------------------------------------------------------------
+struct DevPool {
+    unsigned char *base = nullptr;
+    size_t size = 0;
+    int device = 0;
+    std::mutex mu;
+    std::map<size_t, size_t> free_;  // offset -> bytes, ordered by address
+};
+
+namespace {
+std::mutex g_pools_mu;
+std::vector<DevPool *> g_pools;
+thread_local DevPool *t_pool = nullptr;
+constexpr size_t kPoolAlign = 256;
+}  // namespace
+
+int dev_pool_create(DevPool **out, size_t bytes, int device) {
+    bytes = (bytes + kPoolAlign - 1) / kPoolAlign * kPoolAlign;
+    std::unique_ptr<DevPool> pl(new DevPool());
+    void *b = nullptr;
+    const hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) {
+        (void) hipGetLastError();
+        set_error("reserving %zu device bytes failed: %s", bytes, hipGetErrorString(e));
+        return MGPU_ENOMEM;
+    }
+    pl->base = (unsigned char *) b;
+    pl->size = bytes;
+    pl->device = device;
+    pl->free_[0] = bytes;
+    std::lock_guard<std::mutex> g(g_pools_mu);
+    g_pools.push_back(pl.get());
+    *out = pl.release();
+    return MGPU_OK;
+}
+
+void dev_pool_destroy(DevPool *pl) {
+    if (!pl) return;
+    {
+        std::lock_guard<std::mutex> g(g_pools_mu);
+        g_pools.erase(std::remove(g_pools.begin(), g_pools.end(), pl), g_pools.end());
+    }
+    (void) hipFree(pl->base);
+    delete pl;
+}
+
+void dev_pool_bind(DevPool *pl) { t_pool = pl; }
+
+namespace {
+thread_local bool t_serving = false;
+}
+void serving_bind(bool on) { t_serving = on; }
+bool serving_thread() { return t_serving; }
+
+// first fit in address order
+static int pool_alloc(DevPool &pl, void **p, size_t bytes, size_t *cls) {
+    const size_t need = std::max(kPoolAlign, (bytes + kPoolAlign - 1) / kPoolAlign * kPoolAlign);
+    std::lock_guard<std::mutex> g(pl.mu);
+    for (auto it = pl.free_.begin(); it != pl.free_.end(); ++it) {
+        if (it->second < need) continue;
+        const size_t off = it->first, rest = it->second - need;
+        pl.free_.erase(it);
+        if (rest) pl.free_[off + need] = rest;
+        *p = pl.base + off;
+        *cls = need;
+        return MGPU_OK;
+    }
+    *p = nullptr;
+    set_error("device pool of %zu bytes: no free range of %zu bytes", pl.size, need);
+    return MGPU_ENOMEM;
+}
+
+// returns false if p is not in a pool
+static bool pool_free(void *p, size_t cls) {
+    std::lock_guard<std::mutex> g(g_pools_mu);
+    for (DevPool *pl : g_pools) {
+        unsigned char *q = (unsigned char *) p;
+        if (q < pl->base || q >= pl->base + pl->size) continue;
+        std::lock_guard<std::mutex> h(pl->mu);
+        size_t off = (size_t) (q - pl->base), len = cls;
+        auto next = pl->free_.lower_bound(off);
+        if (next != pl->free_.end() && off + len == next->first) {  // merge with the following range
+            len += next->second;
+            next = pl->free_.erase(next);
+        }
+        if (next != pl->free_.begin()) {  // and with the preceding one
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += len;
+                return true;
+            }
+        }
+        pl->free_[off] = len;
+        return true;
+    }
+    return false;
+}
+
 int dev_alloc(void **p, size_t bytes, int device, size_t *cls) {
+    if (t_pool) return pool_alloc(*t_pool, p, bytes, cls);
     const size_t c = size_class(bytes);
     *cls = c;
     {
@@ -91,6 +193,7 @@ int dev_alloc(void **p, size_t bytes, int device, size_t *cls) {
 
 void dev_free(void *p, size_t cls, int device) {
     if (!p) return;
+    if (pool_free(p, cls)) return;
     std::lock_guard<std::mutex> g(g_cache_mu);
     g_cache[device].emplace(cls, p);
 }
@@ -492,17 +595,13 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
         // while the device builds the pointing tables and the Toeplitz plan; the
         // preconditioner build cuts the device signal, which the solve then uses
         hipStream_t s_up = nullptr;
-        hipEvent_t e_up = nullptr;
         MGPU_HIP(hipStreamCreateWithFlags(&s_up, hipStreamNonBlocking));
         struct UpGuard {
             hipStream_t &st;
-            hipEvent_t &ev;
             ~UpGuard() {
-                if (ev) (void) hipEventDestroy(ev);
                 if (st) (void) hipStreamDestroy(st);
             }
-        } ug{s_up, e_up};
-        MGPU_HIP(hipEventCreateWithFlags(&e_up, hipEventDisableTiming));
+        } ug{s_up};
         int rc_up = MGPU_OK;
         std::string err_up;
         std::thread up([&] {
@@ -513,7 +612,6 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
             }
             rc_up = h2d_staged(dev, pb.d_sig, signal, sizeof(double) * mm, s_up);
             if (rc_up == MGPU_OK) rc_up = h2d_staged(dev, pb.d_noise, noise, sizeof(double) * mm, s_up);
-            if (rc_up == MGPU_OK && hipEventRecord(e_up, s_up) != hipSuccess) rc_up = MGPU_EHIP;
             if (rc_up != MGPU_OK) err_up = mappraiser_gpu_last_error();
         });
         const int rc = problem_setup(pb, d_pix, d_w, m, nnz, local_blocks_sizes, nb, lambda, nullptr, invtt);
@@ -523,8 +621,7 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
             set_error("%s", err_up.c_str());
             return rc_up;
         }
-        MGPU_HIP(hipStreamWaitEvent(s, e_up, 0));
-        MGPU_HIP(hipStreamSynchronize(s_up));
+        MGPU_HIP(hipStreamSynchronize(s_up));  // signal and noise are on the device
     }
     pb.arena.release(d_pix);
     pb.arena.release(d_w);

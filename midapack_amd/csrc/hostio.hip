@@ -8,8 +8,10 @@
 // pinned buffers (two in flight) and queues the DMA on its own stream, so the
 // host copies of one worker overlap the DMA of the others.  The pool is
 // allocated once per process (hipHostMalloc; freed by mappraiser_gpu_finalize)
-// and used by one transfer at a time; a transfer that finds it busy (another
-// host thread is staging) falls back to a plain hipMemcpyAsync.
+// and used by one transfer at a time (others wait for it).  A staged transfer
+// is complete when the call returns: the caller's stream never waits on the
+// pool's events, which the next transfer (possibly another host thread's)
+// records again as soon as the pool is released.
 #include "mgpu_internal.h"
 
 #include <algorithm>
@@ -100,11 +102,7 @@ int h2d_staged(int device, void *d, const void *h, size_t bytes, hipStream_t s) 
         MGPU_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
         return MGPU_OK;
     }
-    std::unique_lock<std::mutex> lk(g_pool.busy, std::try_to_lock);
-    if (!lk.owns_lock()) {
-        MGPU_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
-        return MGPU_OK;
-    }
+    std::lock_guard<std::mutex> lk(g_pool.busy);
     MGPU_RET(pool_ready(device));
     const int W = g_pool.workers;
     const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
@@ -125,7 +123,8 @@ int h2d_staged(int device, void *d, const void *h, size_t bytes, hipStream_t s) 
         MGPU_HIP(hipEventRecord(g_pool.done[device][w], st));
         return MGPU_OK;
     }));
-    for (int w = 0; w < W; w++) MGPU_HIP(hipStreamWaitEvent(s, g_pool.done[device][w], 0));
+    // complete before the pool is released (and its events recorded again)
+    for (int w = 0; w < W; w++) MGPU_HIP(hipEventSynchronize(g_pool.done[device][w]));
     return MGPU_OK;
 }
 
@@ -149,9 +148,9 @@ int d2h_staged(int device, void *h, const void *d, size_t bytes, hipEvent_t afte
     const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
     const unsigned char *dd = (const unsigned char *) d;
     unsigned char *hh = (unsigned char *) h;
+    MGPU_HIP(hipEventSynchronize(after));  // the pool's streams wait on no other stream
     return run_workers(device, W, [&](int w) -> int {
         hipStream_t st = g_pool.ws[device][w];
-        MGPU_HIP(hipStreamWaitEvent(st, after, 0));
         auto issue = [&](size_t k, int b) -> int {
             const size_t off = k * kStageChunk, len = std::min(kStageChunk, bytes - off);
             MGPU_HIP(hipEventSynchronize(g_pool.ev[device][w][b]));

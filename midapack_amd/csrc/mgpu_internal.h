@@ -83,11 +83,26 @@ void set_error(const char *fmt, ...);
 int dev_alloc(void **p, size_t bytes, int device, size_t *cls);
 void dev_free(void *p, size_t cls, int device);
 void dev_cache_trim(int device);  // device < 0: every device
+// A reserved device region (one hipMalloc) that serves the allocations of the
+// threads bound to it, first fit in address order, ranges coalesced on free:
+// the request loop reserves its HBM budget once, so no allocation or free
+// reaches the HIP runtime while solves run.  dev_free finds a pool's blocks
+// by address, from any thread.
+struct DevPool;
+int dev_pool_create(DevPool **out, size_t bytes, int device);
+void dev_pool_destroy(DevPool *pl);  // after every block is freed
+void dev_pool_bind(DevPool *pl);     // this thread's dev_alloc from pl (nullptr: the cache)
+// Solves run by the request loop's workers (set per thread): their segmented
+// PCG keeps the full work lists and refreshes every running segment, the
+// behaviour the request loop was verified with under concurrent solves
+void serving_bind(bool on);
+bool serving_thread();
 
 // Copies between the caller's pageable host arrays and the device through a
 // process-wide pool of pinned staging buffers (hostio.hip).  h2d_staged: when
 // it returns, the host array may be reused, and work queued on s afterwards
-// sees the data.  d2h_staged: the copies start once the event `after` has
+// sees the data (copies of 48 MiB and more are complete on return; smaller
+// ones are queued on s).  d2h_staged: the copies start once the event `after` has
 // completed; when it returns the data has been delivered, either copied to h or, with
 // a consumer, handed to consume(offset, chunk, bytes) chunk by chunk (called
 // from several threads at once on disjoint chunks; offsets are multiples of

@@ -715,6 +715,8 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
     pb.live_state = state;
     run_lists_clear(pb);
 
+    // solves of the request loop keep the full work lists and full refreshes
+    const bool serving = serving_thread();
     const int poll = pb.opt.poll_every > 0 ? pb.opt.poll_every : 8;
     // refresh period of mixed precision; adaptive: no period, segments ask (S_PEND)
     const bool adaptive = mixed && pb.opt.refresh_every == MGPU_REFRESH_ADAPTIVE;
@@ -763,7 +765,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             pb.prof.begin(MGPU_KT_REFRESH, s);
             const bool keep = pb.prof.on;
             pb.prof.on = false;  // the FP64 operator inside is accounted as a refresh
-            if (kind == IT_REFRESH_PENDING && nseg > 1) {
+            if (kind == IT_REFRESH_PENDING && nseg > 1 && !serving) {
                 // only the pending segments need A x: the others' chunks and tiles return at once
                 k_refresh_mask<<<grid_for(nseg, 64), 64, 0, s>>>(state, S, nseg, G.d_rstate);
                 MGPU_LAUNCHED();
@@ -868,7 +870,7 @@ int pcg_run(Problem &pb, const double *d_signal, const double *d_noise, double t
             ndone_seen = hf[F_NDONE];
             MGPU_RET(report_stopped());
         }
-        if (nseg > 1 && hf[F_NDONE] > ndone_lists && !done) {
+        if (nseg > 1 && !serving && hf[F_NDONE] > ndone_lists && !done) {
             // segments stopped: later blocks launch only the running segments'
             // chunks and tiles (a captured block is captured again with them)
             ndone_lists = hf[F_NDONE];

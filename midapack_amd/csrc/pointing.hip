@@ -48,23 +48,29 @@ constexpr int kClassifyGrid = 2048; // grid-stride validation: 8 workgroups per 
 // device-wide atomics are issued once per workgroup instead of once per wave.
 __global__ __launch_bounds__(256) void k_classify(const int *__restrict__ pix, const double *__restrict__ w,
                                                   int64_t m, int *__restrict__ p_of_t, int *__restrict__ status,
-                                                  int *__restrict__ pmin, int *__restrict__ pmax) {
+                                                  int *__restrict__ pmin, int *__restrict__ pmax,
+                                                  int *__restrict__ first_bad) {
     __shared__ int slo[4], shi[4], sst[4];
     int lo = INT32_MAX, hi = INT32_MIN, st = 0;
     const int64_t stride = (int64_t) gridDim.x * blockDim.x;
     for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < m; t += stride) {
         const int a = pix[3 * t], b = pix[3 * t + 1], c = pix[3 * t + 2];
         int p = -1;
+        bool bad = false;
         if (a >= 0) {
             if (a % 3 != 0 || b != a + 1 || c != a + 2)
-                st |= 1;
+                bad = true;
             else
                 p = a / 3;
         } else {
             if (a != -3 || b != -2 || c != -1)
-                st |= 1;
+                bad = true;
             else
                 st |= 2;
+        }
+        if (bad) {
+            st |= 1;
+            atomicMin(first_bad, (int) t);  // m < 2^31
         }
         if (w[3 * t] != 1.0) st |= 4;
         p_of_t[t] = p;
@@ -390,18 +396,23 @@ int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m
 
     int *d_p, *d_stat;
     MGPU_RET(pb.arena.alloc(&d_p, (size_t) m));
-    MGPU_RET(pb.arena.alloc(&d_stat, 3));
-    int init[3] = {0, INT32_MAX, INT32_MIN};
+    MGPU_RET(pb.arena.alloc(&d_stat, 4));
+    int init[4] = {0, INT32_MAX, INT32_MIN, INT32_MAX};
     MGPU_HIP(hipMemcpyAsync(d_stat, init, sizeof(init), hipMemcpyHostToDevice, s));
     const unsigned gcl = (unsigned) std::min<int64_t>(kClassifyGrid, (m + 255) / 256);
-    k_classify<<<gcl, 256, 0, s>>>(d_pix, d_wts, m, d_p, d_stat, d_stat + 1, d_stat + 2);
+    k_classify<<<gcl, 256, 0, s>>>(d_pix, d_wts, m, d_p, d_stat, d_stat + 1, d_stat + 2, d_stat + 3);
     MGPU_LAUNCHED();
-    int hst[3];
+    int hst[4];
     MGPU_HIP(hipMemcpyAsync(hst, d_stat, sizeof(hst), hipMemcpyDeviceToHost, s));
     MGPU_HIP(hipStreamSynchronize(s));
     pb.arena.release(d_stat);
-    MGPU_ARG(!(hst[0] & 1), "pointing is not encoded 3p+k (mappraiser.py:1214-1216) with flagged "
-                            "samples as -3,-2,-1 (utils.py:148-150)");
+    if (hst[0] & 1) {
+        int bad[3] = {0, 0, 0};
+        MGPU_HIP(hipMemcpy(bad, d_pix + 3 * (int64_t) hst[3], sizeof(bad), hipMemcpyDeviceToHost));
+        set_error("pointing is not encoded 3p+k (mappraiser.py:1214-1216) with flagged samples as -3,-2,-1 "
+                  "(utils.py:148-150): sample %d has (%d, %d, %d)", hst[3], bad[0], bad[1], bad[2]);
+        return MGPU_EINVAL;
+    }
     MGPU_ARG(hst[1] != INT32_MAX, "no valid sample: every sample is flagged");
     const int pmin = hst[1], pmax = hst[2];
     P.trash0 = P.trash = (hst[0] & 2) ? 1 : 0;

@@ -15,10 +15,16 @@
 // request's solo solve: batching changes neither the kernels nor the
 // summation order of an item.
 //
+// At creation the server returns the library's cached blocks to the device and
+// reserves 80% of the free device memory as one pool; every solve allocates
+// from it, so no worker thread allocates or frees device memory through the
+// HIP runtime (whose frees synchronise the device) while other solves run.
+// The admission budget is the pool, or less when the caller sets one.
+//
 // Footprints are estimated per sample and per grid pixel (a bound from the
-// allocations of a solve) and, once groups have finished, scaled by the
+// allocations of a solve) and, once groups have finished, raised by the
 // largest ratio of what a group actually held (its arena's high-water mark)
-// to its estimate, plus 5%, so the budget tracks the real usage.  A group whose allocations fail while other
+// to its estimate, plus 5%, when that exceeds 1.  A group whose allocations fail while other
 // solves hold memory goes back to the queue head; such requests are later
 // admitted one at a time and only onto an idle device, where a second failure
 // is final.
@@ -68,8 +74,10 @@ struct Server {
     std::list<Worker> workers;
     int64_t budget = 0, used = 0;
     double correction = 0.0;  // max over finished groups of 1.05 x (measured peak / estimate); 0: none yet
+                              // (it only raises the estimates: below 1 they stay as they are)
     int max_slots = 4, max_batch = 16, running = 0;
     int device = 0;  // the device current when the server was created
+    DevPool *pool = nullptr;  // the HBM budget, reserved at creation; every solve allocates from it
     long long next_ticket = 1;
     bool stop = false;
     mgpu_options opt{};
@@ -132,6 +140,16 @@ void Server::run_group(std::vector<Request *> grp, Worker *self) {
         cv.notify_all();
     };
     int rc = MGPU_OK;
+    struct Bind {  // this thread's device allocations come from the server's pool
+        explicit Bind(DevPool *pl) {
+            dev_pool_bind(pl);
+            serving_bind(true);
+        }
+        ~Bind() {
+            dev_pool_bind(nullptr);
+            serving_bind(false);
+        }
+    } bind(pool);
     if (hipSetDevice(device) != hipSuccess) {
         rc = MGPU_EHIP;
         set_error("hipSetDevice(%d) failed in a server worker", device);
@@ -140,7 +158,6 @@ void Server::run_group(std::vector<Request *> grp, Worker *self) {
         if (rc == MGPU_OK) rc = pcg_batch_impl(items.data(), all, grid, tl, mx, false, &opt, &done);
     }
     std::string err = rc == MGPU_OK ? std::string() : std::string(mappraiser_gpu_last_error());
-    if (rc == MGPU_ENOMEM) dev_cache_trim(device);
     const Clock::time_point t1 = Clock::now();
     std::lock_guard<std::mutex> g(mu);
     // an allocation failure beside other solves, or in a merged group, is retried
@@ -187,7 +204,7 @@ void Server::run_dispatcher() {
                 ++it;
             }
         }
-        auto cost = [&](const Request *r) { return (int64_t) ((double) r->est * (correction > 0 ? correction : 1.0)); };
+        auto cost = [&](const Request *r) { return (int64_t) ((double) r->est * std::max(1.0, correction)); };
         auto fits = [&](const Request *r) {
             if (r->retries > 0) return running == 0;  // failed beside others: run alone
             return running == 0 || used + cost(r) <= budget;
@@ -250,16 +267,16 @@ int mappraiser_gpu_server_create(mgpu_server **out, double hbm_budget_gb, int ma
     s->opt.profile = 0;
     // solves of concurrent workers allocate while others run: no graph capture here
     s->opt.graph = 0;
-    if (hbm_budget_gb > 0) {
-        s->budget = (int64_t) (hbm_budget_gb * 1e9);
-    } else {
-        size_t free_b = 0, total_b = 0;
-        MGPU_HIP(hipMemGetInfo(&free_b, &total_b));
-        s->budget = (int64_t) (0.8 * (double) free_b);
-    }
+    MGPU_HIP(hipGetDevice(&s->device));
+    // the library's cached blocks go back to the device before the budget is reserved
+    dev_cache_trim(s->device);
+    size_t free_b = 0, total_b = 0;
+    MGPU_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t reserve = (int64_t) (0.8 * (double) free_b);
+    s->budget = hbm_budget_gb > 0 ? std::min(reserve, (int64_t) (hbm_budget_gb * 1e9)) : reserve;
+    MGPU_RET(dev_pool_create(&s->pool, (size_t) reserve, s->device));
     s->max_slots = max_slots;
     s->max_batch = max_batch;
-    MGPU_HIP(hipGetDevice(&s->device));
     Server *raw = s.get();
     s->dispatcher = std::thread(&Server::run_dispatcher, raw);
     *out = reinterpret_cast<mgpu_server *>(s.release());
@@ -328,6 +345,7 @@ int mappraiser_gpu_server_destroy(mgpu_server *h) {
         s->cv.notify_all();
     }
     s->dispatcher.join();
+    dev_pool_destroy(s->pool);
     delete s;
     return MGPU_OK;
 }
