@@ -261,6 +261,78 @@ int problem_setup(Problem &pb, const int *d_pix, const double *d_w, int64_t m, i
     return elapsed_ms(pb, &pb.t_create_ms);
 }
 
+int problem_setup_packed(Problem &pb, const int *d_p, double2 *d_w12, int64_t m, int pmin, int pmax,
+                         bool any_flagged, const int *block_sizes, int nb, int lambda, const double *h_invtt) {
+    MGPU_HIP(hipEventRecord(pb.ev0, pb.stream));
+    MGPU_RET(pointing_build_packed(pb, d_p, d_w12, m, pmin, pmax, any_flagged));
+    MGPU_RET(toeplitz_plan(pb, block_sizes, nb, lambda, nullptr, h_invtt));
+    MGPU_HIP(hipEventRecord(pb.ev1, pb.stream));
+    return elapsed_ms(pb, &pb.t_create_ms);
+}
+
+// What the host learned while packing the caller's pointing for upload.
+struct PackStats {
+    int pmin = INT32_MAX, pmax = INT32_MIN;
+    bool bad = false;      // some sample not encoded 3p+k / -3,-2,-1
+    bool flagged = false;  // some sample flagged
+    bool w0 = false;       // some I weight is not 1
+};
+
+// The pointing uploaded packed: per sample its global pixel p (-1 flagged,
+// 4 bytes instead of the 12 of pix[3t..3t+2]) and its (Q, U) weights (16
+// bytes instead of 24), built in the staging buffers while the encoding and
+// the I weights are checked.  Nothing is usable when ps->bad or ps->w0.
+static int upload_packed(int dev, const int *pix, const double *w, size_t m, int *d_p, double2 *d_w12,
+                         PackStats *ps) {
+    std::mutex mu;
+    MGPU_RET(h2d_staged_fill(dev, d_p, sizeof(int) * m, [&](size_t off, void *chunk, size_t len) {
+        const size_t t0 = off / sizeof(int), n = len / sizeof(int);
+        int *o = static_cast<int *>(chunk);
+        int lo = INT32_MAX, hi = INT32_MIN;
+        bool bad = false, fl = false;
+        for (size_t i = 0; i < n; i++) {
+            const int *q = pix + 3 * (t0 + i);
+            const int a = q[0];
+            int p = -1;
+            if (a >= 0) {
+                if (a % 3 == 0 && q[1] == a + 1 && q[2] == a + 2) {
+                    p = a / 3;
+                    lo = std::min(lo, p);
+                    hi = std::max(hi, p);
+                } else {
+                    bad = true;
+                }
+            } else if (a == -3 && q[1] == -2 && q[2] == -1) {
+                fl = true;
+            } else {
+                bad = true;
+            }
+            o[i] = p;
+        }
+        std::lock_guard<std::mutex> g(mu);
+        ps->pmin = std::min(ps->pmin, lo);
+        ps->pmax = std::max(ps->pmax, hi);
+        ps->bad = ps->bad || bad;
+        ps->flagged = ps->flagged || fl;
+    }));
+    if (ps->bad) return MGPU_OK;
+    return h2d_staged_fill(dev, d_w12, sizeof(double2) * m, [&](size_t off, void *chunk, size_t len) {
+        const size_t t0 = off / sizeof(double2), n = len / sizeof(double2);
+        double *o = static_cast<double *>(chunk);
+        bool w0 = false;
+        for (size_t i = 0; i < n; i++) {
+            const double *q = w + 3 * (t0 + i);
+            w0 = w0 || q[0] != 1.0;
+            o[2 * i] = q[1];
+            o[2 * i + 1] = q[2];
+        }
+        if (w0) {
+            std::lock_guard<std::mutex> g(mu);
+            ps->w0 = true;
+        }
+    });
+}
+
 int precond_build_timed(Problem &pb, double *d_signal) {
     TimerRange tr("Preconditioner computation time");  // pcg_true.c:48-56
     MGPU_HIP(hipEventRecord(pb.ev0, pb.stream));
@@ -581,16 +653,31 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
     hipStream_t s = pb.stream;
     const int dev = pb.device;
     const size_t mm = (size_t) m;
-    int *d_pix;
-    double *d_w;
-    MGPU_RET(pb.arena.alloc(&d_pix, mm * nnz));
-    MGPU_RET(pb.arena.alloc(&d_w, mm * nnz));
+    int *d_pix = nullptr, *d_p = nullptr;
+    double *d_w = nullptr;
+    double2 *d_w12 = nullptr;
     MGPU_RET(pb.arena.alloc(&pb.d_sig, mm));
     MGPU_RET(pb.arena.alloc(&pb.d_noise, mm));
+    PackStats ps;
+    bool packed = false;
     {
         TimerRange tr("Initializing pointing matrix");  // mappraiser.c:92-103 (with the Toeplitz set-up)
-        MGPU_RET(h2d_staged(dev, d_pix, pix, sizeof(int) * mm * nnz, s));
-        MGPU_RET(h2d_staged(dev, d_w, pixweights, sizeof(double) * mm * nnz, s));
+        // packed when the pointing is encoded 3p+k and every I weight is 1 (the
+        // configurations of MLmap); otherwise the arrays go up as they are
+        MGPU_RET(pb.arena.alloc(&d_p, mm));
+        MGPU_RET(pb.arena.alloc(&d_w12, mm));
+        MGPU_RET(upload_packed(dev, pix, pixweights, mm, d_p, d_w12, &ps));
+        packed = !ps.bad && !ps.w0;
+        if (!packed) {
+            pb.arena.release(d_p);
+            pb.arena.release(d_w12);
+            d_p = nullptr;
+            d_w12 = nullptr;
+            MGPU_RET(pb.arena.alloc(&d_pix, mm * nnz));
+            MGPU_RET(pb.arena.alloc(&d_w, mm * nnz));
+            MGPU_RET(h2d_staged(dev, d_pix, pix, sizeof(int) * mm * nnz, s));
+            MGPU_RET(h2d_staged(dev, d_w, pixweights, sizeof(double) * mm * nnz, s));
+        }
         // signal and noise go up on a stream of their own (another host thread)
         // while the device builds the pointing tables and the Toeplitz plan; the
         // preconditioner build cuts the device signal, which the solve then uses
@@ -614,7 +701,9 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
             if (rc_up == MGPU_OK) rc_up = h2d_staged(dev, pb.d_noise, noise, sizeof(double) * mm, s_up);
             if (rc_up != MGPU_OK) err_up = mappraiser_gpu_last_error();
         });
-        const int rc = problem_setup(pb, d_pix, d_w, m, nnz, local_blocks_sizes, nb, lambda, nullptr, invtt);
+        const int rc = packed ? problem_setup_packed(pb, d_p, d_w12, m, ps.pmin, ps.pmax, ps.flagged,
+                                                     local_blocks_sizes, nb, lambda, invtt)
+                              : problem_setup(pb, d_pix, d_w, m, nnz, local_blocks_sizes, nb, lambda, nullptr, invtt);
         up.join();
         MGPU_RET(rc);
         if (rc_up != MGPU_OK) {
@@ -623,8 +712,9 @@ int pcg_gls_impl(const char *outpath, const char *ref, int m, int nnz, int *pix,
         }
         MGPU_HIP(hipStreamSynchronize(s_up));  // signal and noise are on the device
     }
-    pb.arena.release(d_pix);
-    pb.arena.release(d_w);
+    // the packed Q/U weights now belong to the pointing tables
+    for (void *p : {(void *) d_pix, (void *) d_w, (void *) d_p})
+        if (p) pb.arena.release(p);
     TimerRange total("Total PCG time");  // mappraiser.c:208-231
     if (tm) MGPU_RET(templates_build(pb, tm));
     MGPU_RET(precond_build_timed(pb, pb.d_sig));

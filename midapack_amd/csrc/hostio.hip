@@ -128,6 +128,30 @@ int h2d_staged(int device, void *d, const void *h, size_t bytes, hipStream_t s) 
     return MGPU_OK;
 }
 
+int h2d_staged_fill(int device, void *d, size_t bytes, const StageProducer &fill) {
+    std::lock_guard<std::mutex> lk(g_pool.busy);
+    MGPU_RET(pool_ready(device));
+    const int W = g_pool.workers;
+    const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
+    unsigned char *dd = (unsigned char *) d;
+    MGPU_RET(run_workers(device, W, [&](int w) -> int {
+        hipStream_t st = g_pool.ws[device][w];
+        int j = 0;
+        for (size_t k = (size_t) w; k < nch; k += (size_t) W, j++) {
+            const int b = j & 1;
+            const size_t off = k * kStageChunk, len = std::min(kStageChunk, bytes - off);
+            MGPU_HIP(hipEventSynchronize(g_pool.ev[device][w][b]));
+            fill(off, buffer(w, b), len);
+            MGPU_HIP(hipMemcpyAsync(dd + off, buffer(w, b), len, hipMemcpyHostToDevice, st));
+            MGPU_HIP(hipEventRecord(g_pool.ev[device][w][b], st));
+        }
+        MGPU_HIP(hipEventRecord(g_pool.done[device][w], st));
+        return MGPU_OK;
+    }));
+    for (int w = 0; w < W; w++) MGPU_HIP(hipEventSynchronize(g_pool.done[device][w]));
+    return MGPU_OK;
+}
+
 int d2h_staged(int device, void *h, const void *d, size_t bytes, hipEvent_t after, const StageConsumer *consume) {
     std::unique_lock<std::mutex> lk(g_pool.busy, std::defer_lock);
     if (bytes >= kStageMin) lk.try_lock();

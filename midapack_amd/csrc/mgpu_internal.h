@@ -108,6 +108,12 @@ bool serving_thread();
 // from several threads at once on disjoint chunks; offsets are multiples of
 // 12 MiB).
 using StageConsumer = std::function<void(size_t off, const void *chunk, size_t bytes)>;
+// h2d_staged_fill: `bytes` of device data produced on the host chunk by chunk:
+// fill(off, chunk, len) writes bytes [off, off + len) of the data into the
+// pinned chunk (called from several threads at once on disjoint chunks;
+// offsets are multiples of 12 MiB).  Complete when the call returns.
+using StageProducer = std::function<void(size_t off, void *chunk, size_t bytes)>;
+int h2d_staged_fill(int device, void *d, size_t bytes, const StageProducer &fill);
 int h2d_staged(int device, void *d, const void *h, size_t bytes, hipStream_t s);
 int d2h_staged(int device, void *h, const void *d, size_t bytes, hipEvent_t after,
                const StageConsumer *consume = nullptr);
@@ -526,6 +532,13 @@ struct Problem {
 // entry points between translation units
 // ---------------------------------------------------------------------------
 int pointing_build(Problem &pb, const int *d_pix, const double *d_w, int64_t m, int nnz);
+// the same from a packed upload: d_p = each sample's global pixel (-1 flagged),
+// d_w12 = its (Q, U) weights with the I weight 1 (the arena block becomes P's
+// w12); pmin/pmax over the valid samples, any_flagged if some sample is flagged
+int pointing_build_packed(Problem &pb, const int *d_p, double2 *d_w12, int64_t m, int pmin, int pmax,
+                          bool any_flagged);
+int pointing_build_from_p(Problem &pb, const int *d_p, int pmin, int pmax, bool any_flagged, const double *d_wts,
+                          double2 *d_w12);
 int pointing_local_indices(Problem &pb, int *d_out);   // MatLocalShape indices [m*3]
 int pointing_lindices(Problem &pb, int *d_out);        // lindices [lcount]
 int pointing_make_fp32(Problem &pb);

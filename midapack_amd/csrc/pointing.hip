@@ -113,6 +113,8 @@ __global__ void k_mark(const int *__restrict__ p_of_t, int64_t m, int pmin, unsi
     }
 }
 
+// w: the caller's weights (3 per sample), or nullptr when the Q/U weights were
+// uploaded packed (they are then P's w12 already and w12 is not written)
 __global__ void k_assign(const int *__restrict__ p_of_t, int64_t m, int pmin, const unsigned *__restrict__ rank,
                          const double *__restrict__ w, int *__restrict__ q, double *__restrict__ w0,
                          double2 *__restrict__ w12, unsigned *__restrict__ key, int *__restrict__ tval,
@@ -123,7 +125,7 @@ __global__ void k_assign(const int *__restrict__ p_of_t, int64_t m, int pmin, co
         int qq = p >= 0 ? (int) rank[p - pmin] : -1;
         q[t] = qq;
         if (w0) w0[t] = w[3 * t];
-        w12[t] = make_double2(w[3 * t + 1], w[3 * t + 2]);
+        if (w) w12[t] = make_double2(w[3 * t + 1], w[3 * t + 2]);
         key[t] = qq >= 0 ? (unsigned) qq : trash_key;
         tval[t] = (int) t;
     }
@@ -414,9 +416,32 @@ int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m
         return MGPU_EINVAL;
     }
     MGPU_ARG(hst[1] != INT32_MAX, "no valid sample: every sample is flagged");
-    const int pmin = hst[1], pmax = hst[2];
-    P.trash0 = P.trash = (hst[0] & 2) ? 1 : 0;
     P.unit_w0 = !(hst[0] & 4);
+    MGPU_RET(pointing_build_from_p(pb, d_p, hst[1], hst[2], (hst[0] & 2) != 0, d_wts, nullptr));
+    pb.arena.release(d_p);
+    return MGPU_OK;
+}
+
+int pointing_build_packed(Problem &pb, const int *d_p, double2 *d_w12, int64_t m, int pmin, int pmax,
+                          bool any_flagged) {
+    Pointing &P = pb.pt;
+    MGPU_ARG(m > 0 && m < (int64_t) INT32_MAX, "m=%lld out of range", (long long) m);
+    MGPU_ARG(pmin <= pmax, "no valid sample: every sample is flagged");
+    P.m = m;
+    P.nnz = 3;
+    P.unit_w0 = true;
+    return pointing_build_from_p(pb, d_p, pmin, pmax, any_flagged, nullptr, d_w12);
+}
+
+// the pointing tables from each sample's global pixel p (-1: flagged): local
+// pixel ids, Q/U weights (from the caller's weights d_wts, or adopted from the
+// packed upload d_w12), the sample order by pixel and the sliced ELL of P^T
+int pointing_build_from_p(Problem &pb, const int *d_p, int pmin, int pmax, bool any_flagged, const double *d_wts,
+                          double2 *d_w12) {
+    Pointing &P = pb.pt;
+    const int64_t m = P.m;
+    hipStream_t s = pb.stream;
+    P.trash0 = P.trash = any_flagged ? 1 : 0;
     const int64_t span = (int64_t) pmax - pmin + 1;
 
     unsigned *d_flags, *d_rank;
@@ -436,7 +461,10 @@ int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m
 
     MGPU_RET(pb.arena.alloc(&P.d_q, (size_t) m));
     if (!P.unit_w0) MGPU_RET(pb.arena.alloc(&P.d_w0, (size_t) m));
-    MGPU_RET(pb.arena.alloc(&P.d_w12, (size_t) m));
+    if (d_w12)
+        P.d_w12 = d_w12;  // the packed upload is P's Q/U weights (sample order)
+    else
+        MGPU_RET(pb.arena.alloc(&P.d_w12, (size_t) m));
     MGPU_RET(pb.arena.alloc(&P.d_pixg, (size_t) P.npix0));
     unsigned *d_key, *d_skey;
     int *d_t;
@@ -472,8 +500,7 @@ int pointing_build(Problem &pb, const int *d_pix, const double *d_wts, int64_t m
     MGPU_LAUNCHED();
     MGPU_RET(build_ell(pb));
     MGPU_HIP(hipStreamSynchronize(s));
-    for (void *p : {(void *) d_cnt, (void *) d_key, (void *) d_skey, (void *) d_t, (void *) d_flags, (void *) d_rank,
-                    (void *) d_p})
+    for (void *p : {(void *) d_cnt, (void *) d_key, (void *) d_skey, (void *) d_t, (void *) d_flags, (void *) d_rank})
         pb.arena.release(p);
     return MGPU_OK;
 }

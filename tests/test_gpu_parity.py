@@ -250,6 +250,29 @@ def test_pcg_full_solve_and_determinism(gpu):
     assert _rel(g.x, r.x) <= 1e-7
 
 
+def test_pcg_tight_solve_matches_oracle(gpu):
+    """The map gap at tol 1e-8 is the stopping tolerance's: solved to 1e-12,
+    GPU and oracle maps agree to 1e-9, and each 1e-8 map is as close to the
+    tight solution as the oracle's own 1e-8 map."""
+    wl = W.make_workload("c1-shape", 2, 65536, 64, 64, 1024, 3)
+    print("case: 2 x 65536, lambda 1024, tol 1e-12 and 1e-8", flush=True)
+    maps = {}
+    for tol in (1e-12, 1e-8):
+        with O.ninv_mode(O.NINV_REAL_SPECTRUM):
+            r = O.gls_solve(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=tol,
+                            maxiter=2000)
+        g = gpu.pcg_gls(wl.pix, wl.weights, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=tol,
+                        maxiter=2000)
+        assert g.report["converged"] and r.converged
+        maps[tol] = (g.x, r.x, g.report["iterations"], r.iterations)
+    gt, ot, gi, oi = maps[1e-12]
+    g8, o8, _, _ = maps[1e-8]
+    print(f"tol 1e-12: {gi} / {oi} iterations, GPU vs oracle {_rel(gt, ot):.2e}; tol 1e-8 vs tight: GPU "
+          f"{_rel(g8, ot):.2e}, oracle {_rel(o8, ot):.2e}", flush=True)
+    assert _rel(gt, ot) <= 1e-9
+    assert _rel(g8, ot) <= 2 * _rel(o8, ot) + 1e-10
+
+
 def test_dropin_side_effects_match_oracle(gpu, fixture_rank1, tmp_path):
     f = fixture_rank1
     with O.ninv_mode(O.NINV_REAL_SPECTRUM):
@@ -376,6 +399,26 @@ def test_invalid_inputs_are_rejected(gpu):
     with gpu.Problem(wl.pix, wl.weights, wl.block_sizes, wl.lam, wl.invtt) as pb:
         with pytest.raises(gpu.GPUError, match="before precond_build"):
             pb.precond_apply(np.zeros(pb.map_size()))
+
+
+@pytest.mark.parametrize("w0", ["unit", "nonunit"])
+def test_dropin_packed_upload_equals_raw_upload(gpu, w0):
+    """The drop-in solve uploads the pointing packed (pixel p, Q/U weights)
+    when every I weight is 1, and as given otherwise: either way its map is
+    bit-identical to the handle path, which always uploads the raw arrays."""
+    wl, pix = _cut_workload()
+    for t in (17, 4096, 65537, 131000):  # flagged samples
+        pix[3 * t:3 * t + 3] = [-3, -2, -1]
+    w = wl.weights.copy()
+    if w0 == "nonunit":
+        w[0::3] = 0.5 + np.random.default_rng(8).random(w.size // 3)
+    print(f"case: drop-in vs handle path, I weights {w0}, 4 flagged samples, forced cut", flush=True)
+    g = gpu.pcg_gls(pix, w, wl.signal, wl.noise, wl.block_sizes, wl.lam, wl.invtt, tol=1e-8, maxiter=300)
+    with gpu.Problem(pix, w, wl.block_sizes, wl.lam, wl.invtt) as pb:
+        sig, cond, hits = pb.precond_build(wl.signal)
+        x, hist, rep = pb.pcg(sig, wl.noise, tol=1e-8, maxiter=300)
+    assert g.report["iterations"] == rep.iterations and g.report["n_cut"] > 0
+    assert np.array_equal(g.x, x) and np.array_equal(g.cond, cond) and np.array_equal(g.hits, hits)
 
 
 def test_dropin_inplace_equals_copies(gpu, fixture_rank1):
