@@ -68,7 +68,17 @@ void set_error(const char *fmt, ...);
 
 // Launch check: kernel launches report configuration errors through
 // hipGetLastError.
-#define MGPU_LAUNCHED() MGPU_HIP(hipGetLastError())
+// kernel launches queued by this thread (MGPU_LAUNCHED counts them): lets the
+// profiler share one event between two ranges with no launch between them
+inline unsigned long long &launch_count() {
+    static thread_local unsigned long long n = 0;
+    return n;
+}
+#define MGPU_LAUNCHED()        \
+    do {                       \
+        ++launch_count();      \
+        MGPU_HIP(hipGetLastError()); \
+    } while (0)
 
 // ---------------------------------------------------------------------------
 // device buffers
@@ -321,6 +331,12 @@ struct Profiler {
     int open_cat = -1;
     int tag = 0;  // PCG iteration the next launches belong to (0 = outside the loop)
     hipEvent_t open_a = nullptr;
+    // end event of the last range: the next range starts from it when nothing
+    // was launched in between on the same stream (half the events in the PCG
+    // loop, whose ranges follow each other)
+    hipEvent_t last_end = nullptr;
+    hipStream_t last_stream = nullptr;
+    unsigned long long last_launch = 0;
 
     hipEvent_t get() {
         if (!free_ev.empty()) {
@@ -335,8 +351,12 @@ struct Profiler {
     void begin(int cat, hipStream_t s) {
         if (!on) return;
         open_cat = cat;
-        open_a = get();
-        (void) hipEventRecord(open_a, s);
+        if (last_end && last_stream == s && last_launch == launch_count()) {
+            open_a = last_end;
+        } else {
+            open_a = get();
+            (void) hipEventRecord(open_a, s);
+        }
     }
     void end(hipStream_t s) {
         if (!on || open_cat < 0) return;
@@ -344,25 +364,30 @@ struct Profiler {
         (void) hipEventRecord(b, s);
         pending.push_back(Rec{open_cat, tag, open_a, b});
         open_cat = -1;
+        last_end = b;
+        last_stream = s;
+        last_launch = launch_count();
     }
     // call after the stream has been synchronised; launches of iterations past
     // `last_iter` were queued after convergence and returned at once: not counted
     void collect(int last_iter = 0x7fffffff) {
-        for (auto &r : pending) {
+        for (size_t i = 0; i < pending.size(); i++) {
+            const Rec &r = pending[i];
             float t = 0.f;
             if (r.tag <= last_iter && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
                 ms[r.cat] += t;
                 cnt[r.cat] += 1;
             }
-            free_ev.push_back(r.a);
+            if (i == 0 || pending[i - 1].b != r.a) free_ev.push_back(r.a);  // a shared start is the previous end
             free_ev.push_back(r.b);
         }
         pending.clear();
+        last_end = nullptr;  // returned to the free list
     }
     ~Profiler() {
-        for (auto &r : pending) {
-            (void) hipEventDestroy(r.a);
-            (void) hipEventDestroy(r.b);
+        for (size_t i = 0; i < pending.size(); i++) {
+            if (i == 0 || pending[i - 1].b != pending[i].a) (void) hipEventDestroy(pending[i].a);
+            (void) hipEventDestroy(pending[i].b);
         }
         for (auto e : free_ev) (void) hipEventDestroy(e);
     }
