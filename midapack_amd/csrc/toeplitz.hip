@@ -40,6 +40,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <atomic>
 #include <unordered_map>
 
 namespace mgpu {
@@ -1543,10 +1544,10 @@ static int launch_tiles_cfg(Problem &pb, const TileClass &C, const R *d_in, R *d
     constexpr size_t lds = sizeof(R) * (size_t) lds_elems(NC) * (TWO ? 2 : 1);
     static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
     auto kern = k_tiles<R, LOGNC, LOGV, LOGR, TWO, MINW>;
-    static unsigned attr_devices = 0;  // devices on which the LDS limit was raised
-    if (pb.device < 32 && !(attr_devices & (1u << pb.device))) {
+    static std::atomic<unsigned> attr_devices{0};  // devices on which the LDS limit was raised
+    if (pb.device < 32 && !(attr_devices.load(std::memory_order_acquire) & (1u << pb.device))) {
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-        attr_devices |= 1u << pb.device;
+        attr_devices.fetch_or(1u << pb.device, std::memory_order_release);
     }
     TileArgs a{d_in, d_out, u.tiles, coef, tw, sc.mean, sc.edge_part, C.lambda, u.L, pb.sg.d_block_seg,
                pb.sg.d_block_seg ? pb.live_state : nullptr};
@@ -1565,10 +1566,10 @@ static int launch_tiles_wl(Problem &pb, const TileClass &C, const R *d_in, R *d_
     constexpr size_t lds = sizeof(R) * (size_t) kWlLdsElems;
     static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
     auto kern = k_tiles_wl<R, MINW>;
-    static unsigned attr_devices = 0;  // devices on which the LDS limit was raised
-    if (pb.device < 32 && !(attr_devices & (1u << pb.device))) {
+    static std::atomic<unsigned> attr_devices{0};  // devices on which the LDS limit was raised
+    if (pb.device < 32 && !(attr_devices.load(std::memory_order_acquire) & (1u << pb.device))) {
         MGPU_HIP(hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-        attr_devices |= 1u << pb.device;
+        attr_devices.fetch_or(1u << pb.device, std::memory_order_release);
     }
     TileArgs a{d_in, d_out, u.tiles, coef, tw, sc.mean, sc.edge_part, C.lambda, C.L, pb.sg.d_block_seg,
                pb.sg.d_block_seg ? pb.live_state : nullptr};
