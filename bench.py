@@ -89,8 +89,6 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=50, help="requests per serving rate (0: skip)")
     ap.add_argument("--serve-rates", default="1,2,5,10,20,50", help="Poisson arrival rates (req/s)")
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded requests cycled by the load")
-    ap.add_argument("--serve-slots", type=int, default=1,
-                    help="concurrent batched solves of the request loop (arrivals still merge into groups)")
     ap.add_argument("--no-noise-model", action="store_true", help="skip the invtt-from-noise section")
     ap.add_argument("--no-templates", action="store_true", help="skip the template map-making (MTmap) section")
     ap.add_argument("--only-templates", action="store_true",
@@ -521,13 +519,13 @@ def serve_section(G, W, args):
         log(f"serve: FP64 reference map {len(ref)}: {ref[-1].report['iterations']} iterations")
     omx = G.options(precision=G.MGPU_MIXED, offset_mode=sym)
     out = {"requests_per_rate": args.serve_requests, "pool": len(reqs), "precision": "mixed",
-           "slots": args.serve_slots, "max_batch": 8, "rates": []}
+           "slots": 1, "max_batch": 8, "rates": []}
     rng = np.random.default_rng(55)
     for rate in [float(v) for v in args.serve_rates.split(",")]:
         n = args.serve_requests
         arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
         tickets = [None] * n
-        with G.Server(hbm_budget_gb=0, max_slots=args.serve_slots, max_batch=8, opts=omx) as srv:
+        with G.Server(hbm_budget_gb=0, max_slots=1, max_batch=8, opts=omx) as srv:
             t0 = time.perf_counter()
 
             def feed():

@@ -275,9 +275,10 @@ int mappraiser_gpu_pcg_batch(int nitems, mgpu_batch_item *items, int grid_npix, 
  * later queued request that fits, up to max_batch, whatever their sizes,
  * bands, grids, tolerances and iteration caps; each admitted group runs as
  * one batched solve (as mappraiser_gpu_pcg_batch, each request with its own
- * grid, tol and maxiter) on its own stream and host thread, so solves
- * admitted at different times overlap on the device (opt->graph is ignored:
- * no graph is captured while other threads allocate).  A request is answered
+ * grid, tol and maxiter) on a stream and host thread of its own (opt->graph
+ * is ignored).  One group runs on the device at a time: max_slots must be 1
+ * (other values are refused with MGPU_EINVAL; with several concurrent groups
+ * the loop stopped answering under sustained load, cause not found).  A request is answered
  * as soon as its own map has converged, while the rest of its group iterates
  * on (its report then carries no solve time and, in MGPU_MIXED, no FP64 true
  * residual).  Results are bit-identical to the requests' solo solves.

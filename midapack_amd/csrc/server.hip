@@ -7,11 +7,13 @@
 // budget that running solves leave; with it, it admits every later queued
 // request that still fits, up to max_batch, whatever their sizes, bands,
 // grids, tolerances and iteration caps.  An admitted group runs as one
-// segmented batch solve (batch.hip) on its own stream and host thread, so
-// solves admitted at different times overlap on the device; each request is
-// answered as soon as its own segment has stopped (the rest of its group
-// iterates on), and when the group finishes its memory and slot return to the
-// pool and the queue is scanned again.  Each result is bit-identical to the
+// segmented batch solve (batch.hip) on a host thread and stream of its own;
+// each request is answered as soon as its own segment has stopped (the rest of
+// its group iterates on), and when the group finishes its memory and slot
+// return to the pool and the queue is scanned again.  One group runs on the
+// device at a time (max_slots must be 1): with several concurrent groups the
+// loop stopped answering under sustained load (cause not found), so that mode
+// is refused.  Each result is bit-identical to the
 // request's solo solve: batching changes neither the kernels nor the
 // summation order of an item.
 //
@@ -75,7 +77,7 @@ struct Server {
     int64_t budget = 0, used = 0;
     double correction = 0.0;  // max over finished groups of 1.05 x (measured peak / estimate); 0: none yet
                               // (it only raises the estimates: below 1 they stay as they are)
-    int max_slots = 4, max_batch = 16, running = 0;
+    int max_slots = 1, max_batch = 16, running = 0;
     int device = 0;  // the device current when the server was created
     DevPool *pool = nullptr;  // the HBM budget, reserved at creation; every solve allocates from it
     long long next_ticket = 1;
@@ -119,10 +121,11 @@ void Server::run_group(std::vector<Request *> grp, Worker *self) {
         mx[i] = grp[i]->maxiter;
     }
     const Clock::time_point t0 = Clock::now();
-    int64_t group_bytes = 0;
+    int64_t group_bytes = 0, group_est = 0;  // read now: answered requests may be freed by their waiters
     for (Request *r : grp) {
         r->t_start = t0;
         group_bytes += r->bytes;
+        group_est += r->est;
     }
     // A request is answered as soon as its own segment has stopped.  Once
     // answered, its waiter may collect it and free it: from then on this
@@ -181,10 +184,9 @@ void Server::run_group(std::vector<Request *> grp, Worker *self) {
     pending.insert(pending.begin(), back.begin(), back.end());
     requeued += (long long) back.size();
     if (rc == MGPU_OK && n > 0) {
-        int64_t est = 0;
-        for (Request *r : grp) est += r->est;
         const double peak = (double) items[0].rep.dev_peak_bytes;
-        if (est > 0 && peak > 0) correction = std::min(4.0, std::max(correction, 1.05 * peak / (double) est));
+        if (group_est > 0 && peak > 0)
+            correction = std::min(4.0, std::max(correction, 1.05 * peak / (double) group_est));
     }
     used -= group_bytes;
     running--;
@@ -259,7 +261,11 @@ extern "C" {
 int mappraiser_gpu_server_create(mgpu_server **out, double hbm_budget_gb, int max_slots, int max_batch,
                                  const mgpu_options *opt) {
     MGPU_ARG(out, "NULL argument");
-    MGPU_ARG(max_slots >= 1 && max_batch >= 1, "max_slots=%d / max_batch=%d must be >= 1", max_slots, max_batch);
+    MGPU_ARG(max_batch >= 1, "max_batch=%d must be >= 1", max_batch);
+    MGPU_ARG(max_slots == 1,
+             "max_slots=%d: concurrent groups are disabled (with several groups on the device the request loop "
+             "stopped answering under sustained load, cause not found); use max_slots=1",
+             max_slots);
     MGPU_ARG(!opt || opt->precond == 0, "precond=%d: the request loop solves with block Jacobi", opt->precond);
     std::unique_ptr<Server> s(new Server());
     mappraiser_gpu_options_default(&s->opt);

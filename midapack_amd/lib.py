@@ -647,7 +647,7 @@ class Server:
     """Request loop of mappraiser_gpu_server_*: submit() returns a ticket,
     wait(ticket) returns (BatchResult, latency_ms, queue_ms)."""
 
-    def __init__(self, hbm_budget_gb=0.0, max_slots=4, max_batch=16, opts: Options | None = None):
+    def __init__(self, hbm_budget_gb=0.0, max_slots=1, max_batch=16, opts: Options | None = None):
         L = load()
         self._h = _vp()
         self._opts = opts or options()

@@ -39,6 +39,17 @@ def test_host_only_entry_points():
     assert isinstance(G.last_error(), str)
 
 
+def test_server_refuses_concurrent_groups():
+    # checked before any device call: one group on the device at a time
+    for slots in (0, 2, 4):
+        try:
+            G.Server(max_slots=slots).close()
+        except G.GPUError as e:
+            assert f"max_slots={slots}" in str(e)
+        else:
+            raise AssertionError(f"max_slots={slots} was accepted")
+
+
 def _params(name):
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     m = re.search(rf"\bvoid {name}\((.*?)\);", text, flags=re.S)

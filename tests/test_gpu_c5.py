@@ -58,7 +58,7 @@ def _serve(gpu, pool, opts, rate, seed, maxiter):
     """Submit the pool under Poisson arrivals; returns [(result, latency_ms)], stats."""
     arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, len(pool)))
     tickets = [None] * len(pool)
-    with gpu.Server(hbm_budget_gb=0, max_slots=4, max_batch=8, opts=opts) as srv:
+    with gpu.Server(hbm_budget_gb=0, max_slots=1, max_batch=8, opts=opts) as srv:
         t0 = time.perf_counter()
 
         def feed():
