@@ -150,76 +150,91 @@ __device__ constexpr int bitrev(int k) {
     return r;
 }
 
+// Radix-2 decimation-in-time butterfly of a stage of length LEN at (a, b):
+// a <- a + w b, b <- a - w b with w = exp(-2 pi i J / LEN).  A general w costs
+// six FMA-unit operations: a + w b as two fused chains, a - w b = 2a - (a + w b).
 template <typename R, int LEN, int J>
-__device__ __forceinline__ void twiddle_dif(R &dr, R &di) {
-    // (dr + i di) * exp(-2 pi i J / LEN)
+__device__ __forceinline__ void bfly_dit(R &ar, R &ai, R &br, R &bi) {
     constexpr int e = (32 / LEN) * J;  // index into the 32-point table
+    R r0, i0, r1, i1;
     if constexpr (J == 0) {
-        return;
-    } else if constexpr (4 * J == LEN) {  // -i
-        R t = dr;
-        dr = di;
-        di = -t;
-    } else if constexpr (8 * J == LEN) {  // (1 - i)/sqrt2
+        r0 = ar + br;
+        i0 = ai + bi;
+        r1 = ar - br;
+        i1 = ai - bi;
+    } else if constexpr (4 * J == LEN) {  // w = -i: w b = bi - i br
+        r0 = ar + bi;
+        i0 = ai - br;
+        r1 = ar - bi;
+        i1 = ai + br;
+    } else if constexpr (8 * J == LEN) {  // w = (1 - i)/sqrt2: w b = c (br + bi) + i c (bi - br)
         const R c = (R) kCos32[4];
-        R a = dr, b = di;
-        dr = c * (a + b);
-        di = c * (b - a);
-    } else if constexpr (8 * J == 3 * LEN) {  // (-1 - i)/sqrt2
+        const R u = br + bi, v = bi - br;
+        r0 = fma(c, u, ar);
+        i0 = fma(c, v, ai);
+        r1 = fma(-c, u, ar);
+        i1 = fma(-c, v, ai);
+    } else if constexpr (8 * J == 3 * LEN) {  // w = -(1 + i)/sqrt2: w b = -c (br - bi) - i c (br + bi)
         const R c = (R) kCos32[4];
-        R a = dr, b = di;
-        dr = c * (b - a);
-        di = -c * (a + b);
+        const R u = br - bi, v = br + bi;
+        r0 = fma(-c, u, ar);
+        i0 = fma(-c, v, ai);
+        r1 = fma(c, u, ar);
+        i1 = fma(c, v, ai);
     } else {
+        // w = cos x - i sin x: w b = (c br + s bi) + i (c bi - s br)
         const R c = (R) kCos32[e], s = (R) kCos32[(e + 24) & 31];  // sin(x) = cos(x - pi/2)
-        R a = dr, b = di;
-        dr = fma(a, c, b * s);
-        di = fma(b, c, -(a * s));
+        r0 = fma(c, br, fma(s, bi, ar));
+        i0 = fma(c, bi, fma(-s, br, ai));
+        r1 = fma((R) 2, ar, -r0);
+        i1 = fma((R) 2, ai, -i0);
     }
+    ar = r0;
+    ai = i0;
+    br = r1;
+    bi = i1;
 }
 
+// every butterfly of the stage of length LEN (groups START, START + LEN, ...)
 template <typename R, int RAD, int LEN, int START, int J>
-struct DifStage {
+struct DitStage {
     static __device__ __forceinline__ void run(R *re, R *im) {
         if constexpr (START < RAD) {
             if constexpr (J < LEN / 2) {
                 constexpr int i0 = START + J, i1 = START + J + LEN / 2;
-                R ar = re[i0], ai = im[i0], br = re[i1], bi = im[i1];
-                re[i0] = ar + br;
-                im[i0] = ai + bi;
-                R dr = ar - br, di = ai - bi;
-                twiddle_dif<R, LEN, J>(dr, di);
-                re[i1] = dr;
-                im[i1] = di;
-                DifStage<R, RAD, LEN, START, J + 1>::run(re, im);
+                bfly_dit<R, LEN, J>(re[i0], im[i0], re[i1], im[i1]);
+                DitStage<R, RAD, LEN, START, J + 1>::run(re, im);
             } else {
-                DifStage<R, RAD, LEN, START + LEN, 0>::run(re, im);
+                DitStage<R, RAD, LEN, START + LEN, 0>::run(re, im);
             }
         }
     }
 };
 
+// stages LEN, 2 LEN, ..., RAD
 template <typename R, int RAD, int LEN>
-struct DifAll {
+struct DitAll {
     static __device__ __forceinline__ void run(R *re, R *im) {
-        if constexpr (LEN >= 2) {
-            DifStage<R, RAD, LEN, 0, 0>::run(re, im);
-            DifAll<R, RAD, LEN / 2>::run(re, im);
+        if constexpr (LEN <= RAD) {
+            DitStage<R, RAD, LEN, 0, 0>::run(re, im);
+            DitAll<R, RAD, 2 * LEN>::run(re, im);
         }
     }
 };
 
-// forward DFT of RAD points at re/im[0..RAD), result in natural order
+// forward DFT of RAD points at re/im[0..RAD), natural order in and out: the
+// inputs are renamed into bit-reversed order (no data movement), then the
+// radix-2 DIT stages run in place (RAD = 32: 388 operations, 16: 148)
 template <typename R, int RAD>
 __device__ __forceinline__ void dft(R *re, R *im) {
     if constexpr (RAD > 1) {
-        DifAll<R, RAD, RAD>::run(re, im);
         R tr[RAD], ti[RAD];
 #pragma unroll
         for (int k = 0; k < RAD; k++) {
             tr[k] = re[bitrev<RAD>(k)];
             ti[k] = im[bitrev<RAD>(k)];
         }
+        DitAll<R, RAD, 2>::run(tr, ti);
 #pragma unroll
         for (int k = 0; k < RAD; k++) {
             re[k] = tr[k];
