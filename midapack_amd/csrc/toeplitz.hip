@@ -715,7 +715,10 @@ __device__ __forceinline__ void wl_wave_sync() {
 template <typename R, int MINW>
 __global__ __launch_bounds__(kWlNT, MINW) void k_tiles_wl(TileArgs a) {
     constexpr int NC = kWlNC, NT = kWlNT;
-    constexpr bool SCHED_FENCE = sizeof(R) == 8;
+    // scheduling fences between phases in both precisions: they keep the loads of
+    // a later phase from being hoisted over the current one (FP32 at 128 VGPRs:
+    // 21 registers spilled instead of 33, 2 % faster)
+    constexpr bool SCHED_FENCE = true;
     extern __shared__ unsigned char smem_raw[];
     R *lds = reinterpret_cast<R *>(smem_raw);
 
@@ -730,7 +733,6 @@ __global__ __launch_bounds__(kWlNT, MINW) void k_tiles_wl(TileArgs a) {
     const Cx<R> *__restrict__ tw = reinterpret_cast<const Cx<R> *>(a.tw);
     const Cx<R> *__restrict__ cw = reinterpret_cast<const Cx<R> *>(a.coef) + (int64_t) d.filter * NC + t;
     const R mean = (R) a.mean[d.block];
-    const R mean_out = (R) mean_out_of(a.edge_part, d.block, d.blk_hi - d.blk_lo);
     const int64_t in0 = d.out_lo - (a.lambda - 1);
     const R *__restrict__ yb = y + d.blk_lo;
     R *__restrict__ zb = z + d.blk_lo;
@@ -867,6 +869,8 @@ __global__ __launch_bounds__(kWlNT, MINW) void k_tiles_wl(TileArgs a) {
     wl_twiddle<R, 32>(re, im, tw[t]);
     if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
     dft<R, 32>(re, im);
+    // loaded here rather than at the start: one register less over the whole tile
+    const R mean_out = (R) mean_out_of(a.edge_part, d.block, d.blk_hi - d.blk_lo);
     const int lo = a.lambda - 1, hi = a.lambda - 1 + a.L;
     if (interior) {
         R *__restrict__ zt = zb + rel0;
