@@ -28,7 +28,8 @@
 // entry B_b[r][c] is summed by one thread over its samples in increasing order
 // (BuildKernel's order, so B is bit-identical to it); B^+ = V diag(1/lambda) V^T
 // from rocSOLVER's batched symmetric eigensolver with dgelss's cut
-// (|lambda| <= 1e-12 max |lambda| dropped).
+// (|lambda| <= 1e-12 max |lambda| dropped), one eigensolve per CES: its
+// detectors' kernels differ only by their weight w_b (B_b = w_b K_c).
 #include "mgpu_internal.h"
 
 #include <rocblas/rocblas.h>
@@ -323,6 +324,19 @@ __global__ __launch_bounds__(kPT *kPT) void k_pinv_form(const double *__restrict
         __syncthreads();
     }
     if (r0 + tr < n && c0 + tc < n) Binv[(int64_t) z * n * n + (int64_t) (r0 + tr) * n + c0 + tc] = acc;
+}
+
+// Binv_b = sc[j] * P_rep[j] for the j-th block of a run (blockIdx.y): the
+// pseudo-inverse of B_b = w_b K from that of its CES's representative block
+__global__ __launch_bounds__(256) void k_pinv_scale(const double *__restrict__ P, const int *__restrict__ rep,
+                                                    const double *__restrict__ sc, int64_t nn,
+                                                    double *__restrict__ Binv) {
+    const int j = blockIdx.y;
+    const double *__restrict__ src = P + (int64_t) rep[j] * nn;
+    double *__restrict__ dst = Binv + (int64_t) j * nn;
+    const double f = sc[j];
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (int64_t) gridDim.x * blockDim.x)
+        dst[i] = f * src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -719,51 +733,90 @@ int templates_build(Problem &pb, const mgpu_templates *t) {
         MGPU_HIP(hipStreamSynchronize(s));
         pb.arena.release(d_tk);
     }
-    // pseudo-inverses: one batched eigensolve per run of consecutive CES with
-    // the same template count (their blocks are contiguous nt x nt matrices),
-    // each run's working copy kept under kEigBatchBytes
+    // Pseudo-inverses.  Every entry of B_b is a sum of w_b times products of the
+    // CES's template values, so B_b = w_b K_c for all detectors b of CES c: one
+    // eigensolve per CES (its first block of non-zero weight, the representative
+    // r), and B_b^+ = (w_r / w_b) B_r^+ with the rank of B_r (dgelss's cut is
+    // relative, so it keeps the same eigenvalues); a block of weight 0 has
+    // B_b = 0, B_b^+ = 0 and rank 0.  One batched eigensolve per run of
+    // consecutive CES with the same template count, each run's working copies
+    // kept under kEigBatchBytes.
     M.h_rank.assign((size_t) nb, 0);
     if (M.nB > 0) {
         constexpr size_t kEigBatchBytes = size_t(4) << 30;
+        std::vector<double> w((size_t) nb);
+        MGPU_HIP(hipMemcpyAsync(w.data(), T.d_T0, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+        MGPU_HIP(hipStreamSynchronize(s));
         RocblasHandle rh;
         MGPU_ROCBLAS(rocblas_create_handle(&rh.h));
         MGPU_ROCBLAS(rocblas_set_stream(rh.h, s));
-        int *d_rank = nullptr, *d_info = nullptr;
-        MGPU_RET(pb.arena.alloc(&d_rank, (size_t) nb));
-        MGPU_RET(pb.arena.alloc(&d_info, (size_t) nb));
+        int *d_rank = nullptr, *d_info = nullptr, *d_rep = nullptr;
+        double *d_sc = nullptr;
+        MGPU_RET(pb.arena.alloc(&d_rank, (size_t) nces));
+        MGPU_RET(pb.arena.alloc(&d_info, (size_t) nces));
+        MGPU_RET(pb.arena.alloc(&d_rep, (size_t) nb));
+        MGPU_RET(pb.arena.alloc(&d_sc, (size_t) nb));
         for (int c = 0; c < nces;) {
             const int b0 = c * ndet;
             const int n = M.h_nt[b0];
-            const size_t per_ces = (size_t) ndet * n * n * sizeof(double);
+            const size_t per_ces = (size_t) 2 * n * n * sizeof(double);  // eigenvectors and pseudo-inverse
             int e = c + 1;
             while (e < nces && M.h_nt[(size_t) e * ndet] == n && (size_t) (e + 1 - c) * per_ces <= kEigBatchBytes) e++;
-            const int batch = (e - c) * ndet;
+            const int nrep = e - c, nblk = nrep * ndet;
+            const int c_first = c;
             c = e;
             if (n == 0) continue;
-            double *V = nullptr, *D = nullptr, *E = nullptr;
-            MGPU_RET(pb.arena.alloc(&V, (size_t) batch * n * n));
-            MGPU_RET(pb.arena.alloc(&D, (size_t) batch * n));
-            MGPU_RET(pb.arena.alloc(&E, (size_t) batch * n));
-            MGPU_HIP(hipMemcpyAsync(V, M.d_B + M.h_boff[b0], sizeof(double) * batch * n * n, hipMemcpyDeviceToDevice,
-                                    s));
+            const int64_t nn = (int64_t) n * n;
+            double *V = nullptr, *D = nullptr, *E = nullptr, *Pinv = nullptr;
+            MGPU_RET(pb.arena.alloc(&V, (size_t) nrep * nn));
+            MGPU_RET(pb.arena.alloc(&D, (size_t) nrep * n));
+            MGPU_RET(pb.arena.alloc(&E, (size_t) nrep * n));
+            MGPU_RET(pb.arena.alloc(&Pinv, (size_t) nrep * nn));
+            std::vector<int> rep_blk((size_t) nrep), rep_of((size_t) nblk);
+            std::vector<double> sc((size_t) nblk);
+            for (int k = 0; k < nrep; k++) {
+                const int cb = (c_first + k) * ndet;
+                int r = cb;
+                for (int j2 = 0; j2 < ndet; j2++)
+                    if (w[(size_t) cb + j2] != 0.0) {
+                        r = cb + j2;
+                        break;
+                    }
+                rep_blk[k] = r;
+                for (int j2 = 0; j2 < ndet; j2++) {
+                    const int b = cb + j2;
+                    rep_of[(size_t) (b - b0)] = k;
+                    sc[(size_t) (b - b0)] = (w[b] != 0.0 && w[r] != 0.0) ? w[r] / w[b] : 0.0;
+                }
+                MGPU_HIP(hipMemcpyAsync(V + k * nn, M.d_B + M.h_boff[r], sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                                        s));
+            }
+            MGPU_HIP(hipMemcpyAsync(d_rep, rep_of.data(), sizeof(int) * nblk, hipMemcpyHostToDevice, s));
+            MGPU_HIP(hipMemcpyAsync(d_sc, sc.data(), sizeof(double) * nblk, hipMemcpyHostToDevice, s));
             MGPU_ROCBLAS(rocsolver_dsyevd_strided_batched(rh.h, rocblas_evect_original, rocblas_fill_upper, n, V, n,
-                                                          (rocblas_stride) n * n, D, n, E, n, d_info, batch));
-            std::vector<int> info((size_t) batch);
-            MGPU_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+                                                          (rocblas_stride) nn, D, n, E, n, d_info, nrep));
+            std::vector<int> info((size_t) nrep), rank((size_t) nrep);
+            MGPU_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * nrep, hipMemcpyDeviceToHost, s));
             MGPU_HIP(hipStreamSynchronize(s));
-            for (int j = 0; j < batch; j++)
-                MGPU_ARG(info[j] == 0, "eigensolve of template kernel %d failed (info %d)", b0 + j, info[j]);
-            k_eig_inv<<<batch, 256, 0, s>>>(D, n, d_rank + b0);
+            for (int k = 0; k < nrep; k++)
+                MGPU_ARG(info[k] == 0, "eigensolve of template kernel %d failed (info %d)", rep_blk[k], info[k]);
+            k_eig_inv<<<nrep, 256, 0, s>>>(D, n, d_rank);
             MGPU_LAUNCHED();
             const unsigned tiles = (unsigned) ((n + kPT - 1) / kPT);
-            k_pinv_form<<<dim3(tiles, tiles, batch), kPT * kPT, 0, s>>>(V, D, n, M.d_Binv + M.h_boff[b0]);
+            k_pinv_form<<<dim3(tiles, tiles, nrep), kPT * kPT, 0, s>>>(V, D, n, Pinv);
             MGPU_LAUNCHED();
+            k_pinv_scale<<<dim3((unsigned) std::min<int64_t>(grid_for(nn, 256), 1024), nblk), 256, 0, s>>>(
+                    Pinv, d_rep, d_sc, nn, M.d_Binv + M.h_boff[b0]);
+            MGPU_LAUNCHED();
+            MGPU_HIP(hipMemcpyAsync(rank.data(), d_rank, sizeof(int) * nrep, hipMemcpyDeviceToHost, s));
             MGPU_HIP(hipStreamSynchronize(s));
+            for (int b = b0; b < b0 + nblk; b++)
+                M.h_rank[(size_t) b] = w[b] != 0.0 ? rank[(size_t) rep_of[(size_t) (b - b0)]] : 0;
             pb.arena.release(V);
             pb.arena.release(D);
             pb.arena.release(E);
+            pb.arena.release(Pinv);
         }
-        MGPU_HIP(hipMemcpyAsync(M.h_rank.data(), d_rank, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
     }
     MGPU_HIP(hipEventRecord(pb.ev1, s));
     MGPU_HIP(hipStreamSynchronize(s));
