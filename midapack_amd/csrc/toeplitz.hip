@@ -386,6 +386,15 @@ struct Passes {
 // ---------------------------------------------------------------------------
 // the tile kernel
 // ---------------------------------------------------------------------------
+// the lane index as a value the compiler cannot tie to the one computed at the
+// kernel's start: the inverse transform derives its exchange coordinates from
+// it again instead of keeping them live (and spilled) across the spectral step
+__device__ __forceinline__ int lane_again() {
+    int t = (int) threadIdx.x;
+    __asm__ volatile("" : "+v"(t));
+    return t;
+}
+
 // slices of the symmetric mode's output-mean sum (see k_block_mean)
 constexpr int kEdgeSlices = 8;
 
@@ -835,65 +844,73 @@ __global__ __launch_bounds__(kWlNT, MINW) void k_tiles_wl(TileArgs a) {
     // ---- inverse B2: 16-point DFTs over k2b ----
     dft<R, 16>(re, im);
     dft<R, 16>(re + 16, im + 16);
-    // ---- inverse E2 ----
+    {
+        // the lane's coordinates again for the inverse transform (same values)
+        const int t = lane_again(), w = t >> 6, l = t & 63, kidx = l >> 4, t1 = l & 15;
+        R *__restrict__ reg = lds + w * kWlRegion;
+        int kp, ap, kq, aq;
+        wl_pair(w, l, kp, ap, kq, aq);
+        const int k1 = wl_k1(w, kidx);
+        // ---- inverse E2 ----
 #pragma unroll
-    for (int p = 0; p < 2; p++) {
-        R *v = p ? im : re;
+        for (int p = 0; p < 2; p++) {
+            R *v = p ? im : re;
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            reg[kp * kWlCol + ap * kWlRow + q] = v[q];
-            reg[kq * kWlCol + aq * kWlRow + q] = v[16 + q];
+            for (int q = 0; q < 16; q++) {
+                reg[kp * kWlCol + ap * kWlRow + q] = v[q];
+                reg[kq * kWlCol + aq * kWlRow + q] = v[16 + q];
+            }
+            wl_wave_sync();
+#pragma unroll
+            for (int q = 0; q < 32; q++) v[q] = reg[kidx * kWlCol + q * kWlRow + t1];
+            wl_wave_sync();
         }
-        wl_wave_sync();
-#pragma unroll
-        for (int q = 0; q < 32; q++) v[q] = reg[kidx * kWlCol + q * kWlRow + t1];
-        wl_wave_sync();
-    }
-    // ---- inverse B1: twiddle, 32-point DFT over k2a ----
-    wl_twiddle<R, 32>(re, im, tw[32 * t1]);
-    if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
-    dft<R, 32>(re, im);
-    // ---- inverse E1 (every wave has finished reading its E2 region) ----
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < 2; p++) {
-        R *v = p ? im : re;
-#pragma unroll
-        for (int q = 0; q < 32; q++) lds[(t1 + 16 * q) * kWlPlaneStride + k1] = v[q];
+        // ---- inverse B1: twiddle, 32-point DFT over k2a ----
+        wl_twiddle<R, 32>(re, im, tw[32 * t1]);
+        if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+        dft<R, 32>(re, im);
+        // ---- inverse E1 (every wave has finished reading its E2 region) ----
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 32; q++) v[q] = lds[t * kWlPlaneStride + q];
-        if (p == 0) __syncthreads();
-    }
-    // ---- inverse A: twiddle, 32-point DFT over k1 -> conj(y) at n = t + 512 r ----
-    wl_twiddle<R, 32>(re, im, tw[t]);
-    if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
-    dft<R, 32>(re, im);
-    // loaded here rather than at the start: one register less over the whole tile
-    const R mean_out = (R) mean_out_of(a.edge_part, d.block, d.blk_hi - d.blk_lo);
-    const int lo = a.lambda - 1, hi = a.lambda - 1 + a.L;
-    if (interior) {
-        R *__restrict__ zt = zb + rel0;
-        const unsigned L = (unsigned) a.L;
+        for (int p = 0; p < 2; p++) {
+            R *v = p ? im : re;
 #pragma unroll
-        for (int r = 0; r < 32; r++) {
-            const int i0 = 2 * (t + r * NT);
-            const int s_lo = 2 * r * NT, s_hi = s_lo + 2 * NT;  // samples of slot r (uniform)
-            if (s_lo >= lo && s_hi <= hi) {
-                zt[i0] = re[r] - mean_out;
-                zt[i0 + 1] = -im[r] - mean_out;
-            } else if (s_hi > lo && s_lo < hi) {
-                if ((unsigned) (i0 - lo) < L) zt[i0] = re[r] - mean_out;
-                if ((unsigned) (i0 + 1 - lo) < L) zt[i0 + 1] = -im[r] - mean_out;
-            }
+            for (int q = 0; q < 32; q++) lds[(t1 + 16 * q) * kWlPlaneStride + k1] = v[q];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 32; q++) v[q] = lds[t * kWlPlaneStride + q];
+            if (p == 0) __syncthreads();
         }
-    } else {
+        // ---- inverse A: twiddle, 32-point DFT over k1 -> conj(y) at n = t + 512 r ----
+        wl_twiddle<R, 32>(re, im, tw[t]);
+        if constexpr (SCHED_FENCE) __builtin_amdgcn_sched_barrier(0);
+        dft<R, 32>(re, im);
+        // loaded here rather than at the start: one register less over the whole tile
+        const R mean_out = (R) mean_out_of(a.edge_part, d.block, d.blk_hi - d.blk_lo);
+        const int lo = a.lambda - 1, hi = a.lambda - 1 + a.L;
+        if (interior) {
+            R *__restrict__ zt = zb + rel0;
+            const unsigned L = (unsigned) a.L;
 #pragma unroll
-        for (int r = 0; r < 32; r++) {
-            const int i0 = 2 * (t + r * NT);
-            const int o0 = rel0 + i0;
-            if (i0 >= lo && i0 < hi && o0 < nblk) zb[(unsigned) o0] = re[r] - mean_out;
-            if (i0 + 1 >= lo && i0 + 1 < hi && o0 + 1 < nblk) zb[(unsigned) (o0 + 1)] = -im[r] - mean_out;
+            for (int r = 0; r < 32; r++) {
+                const int i0 = 2 * (t + r * NT);
+                const int s_lo = 2 * r * NT, s_hi = s_lo + 2 * NT;  // samples of slot r (uniform)
+                if (s_lo >= lo && s_hi <= hi) {
+                    zt[i0] = re[r] - mean_out;
+                    zt[i0 + 1] = -im[r] - mean_out;
+                } else if (s_hi > lo && s_lo < hi) {
+                    if ((unsigned) (i0 - lo) < L) zt[i0] = re[r] - mean_out;
+                    if ((unsigned) (i0 + 1 - lo) < L) zt[i0 + 1] = -im[r] - mean_out;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 32; r++) {
+                const int i0 = 2 * (t + r * NT);
+                const int o0 = rel0 + i0;
+                if (i0 >= lo && i0 < hi && o0 < nblk) zb[(unsigned) o0] = re[r] - mean_out;
+                if (i0 + 1 >= lo && i0 + 1 < hi && o0 + 1 < nblk) zb[(unsigned) (o0 + 1)] = -im[r] - mean_out;
+            }
         }
     }
 }
