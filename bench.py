@@ -18,7 +18,11 @@ Prints one JSON line.  Beside `value` it reports the same solve with the
 inputs already resident in HBM (`device_resident`).  The roofline entry is for
 the dominant kernel (the Toeplitz FFT tiles of N^-1): algorithmic bytes per
 launch over its average launch duration, measured with HIP events on the
-library's stream during the device-resident solves.
+library's stream during the timed drop-in solves.  Further sections: mixed
+precision (both refresh rules), the reference's own offset operator, the
+two-level preconditioners, the CPU oracle on 1 and on all host cores, C1,
+the noise-model builder, template map-making (C6), batched Monte Carlo
+realizations (C4) and the request loop under Poisson arrivals (C5).
 """
 from __future__ import annotations
 
@@ -91,6 +95,8 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded requests cycled by the load")
     ap.add_argument("--no-noise-model", action="store_true", help="skip the invtt-from-noise section")
     ap.add_argument("--no-templates", action="store_true", help="skip the template map-making (MTmap) section")
+    ap.add_argument("--twolvl-zn", type=int, default=8,
+                    help="deflation space size of the two-level preconditioner section (0: skip)")
     ap.add_argument("--only-templates", action="store_true",
                     help="run only the template map-making section (C6) and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -134,13 +140,13 @@ def main():
     offset = {"symmetric": G.MGPU_OFFSET_SYMMETRIC, "reference": G.MGPU_OFFSET_REFERENCE,
               "none": G.MGPU_OFFSET_NONE}[args.offset]
 
-    def solve(precision, profile=True, offset_mode=offset, maxiter=args.maxiter, refresh=10):
+    def solve(precision, profile=True, offset_mode=offset, maxiter=args.maxiter, refresh=10, precond=0, zn=0):
         d_sig.copy_(d_sig0)  # the cut zeroes signal samples in place (precond.c:605,610)
         torch.cuda.synchronize()
         pb = G.Problem.from_device(wl.m, d_pix.data_ptr(), d_w.data_ptr(), wl.block_sizes, wl.lam,
                                    d_inv.data_ptr(),
                                    opts=G.options(precision=precision, profile=profile, offset_mode=offset_mode,
-                                                  refresh_every=refresh))
+                                                  refresh_every=refresh, precond=precond, z_2lvl=zn))
         x, hist, rep = pb.solve_device(d_sig.data_ptr(), d_noise.data_ptr(), tol=args.tol, maxiter=maxiter)
         kt = pb.kernel_times()
         pb.close()
@@ -313,6 +319,28 @@ def main():
                                                  "iteration of its lowest residual"}
         log(f"reference offsets: converged={rr.converged}, min residual {np.nanmin(hr):.2e} at {kmin}, "
             f"{tref:.3f} s to reach it")
+
+    if args.twolvl_zn > 0:
+        # MLmap's precond 1 / 2 (two-level: block Jacobi plus a deflation space of
+        # Zn vectors, rebuilt by every PCG call) on the same workload, inputs in HBM
+        tl = {"zn": args.twolvl_zn, "block_jacobi_resident_s": round(per_map_dev, 4),
+              "block_jacobi_iterations": int(rep_d.iterations)}
+        for pc, name in ((1, "a_priori"), (2, "a_posteriori")):
+            try:
+                solve(G.MGPU_FP64, profile=False, precond=pc, zn=args.twolvl_zn)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                _, _, r2, _ = solve(G.MGPU_FP64, profile=False, precond=pc, zn=args.twolvl_zn)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                tl[name] = {"time_to_map_s": round(dt, 4), "iterations": int(r2.iterations),
+                            "converged": bool(r2.converged), "build_ms": round(r2.t_2lvl_ms, 2),
+                            "solve_ms": round(r2.t_solve_ms, 2)}
+                log(f"two-level precond {pc} (Zn {args.twolvl_zn}): {dt:.3f} s resident, {r2.iterations} iterations "
+                    f"(block Jacobi {rep_d.iterations}), deflation space built in {r2.t_2lvl_ms:.1f} ms")
+            except Exception as e:  # reported, not hidden
+                tl[name] = {"error": str(e)}
+        out["twolvl_c2"] = tl
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args, reps[-1]["iterations"])
