@@ -9,6 +9,7 @@ the same ranks and within 1e-8; the PCG residual history within 1e-9 and the
 map within 1e-6 once the I monopole, which the offset-like templates leave
 undetermined, is removed.
 """
+import dataclasses
 import os
 
 import numpy as np
@@ -70,6 +71,33 @@ def test_products_and_kernels_match_the_oracle(gpu, over):
         # templates are filtered out
         pure = w * tm.tvec(tau)
         assert np.abs(pb.templates_apply(gpu.MGPU_TMPL_FILTER, pure)).max() <= 1e-9 * np.abs(pure).max()
+
+
+def test_pseudo_inverses_with_detector_weights_of_zero(gpu):
+    """The kernels of a CES's detectors differ only by their weight, so one
+    eigensolve per CES serves them all (scaled by the weight ratio).  A detector
+    of weight 0 has B_b = 0: pseudo-inverse 0, rank 0; a CES whose detectors all
+    weigh 0 likewise; the others still match the oracle's per-block SVD."""
+    wl = W.config_templates_small(nces=3, ndet=3)
+    inv = wl.invtt.copy()
+    inv[0] = 0.0          # the first detector of CES 0 (the representative moves to the next)
+    inv[3:6] = 0.0        # every detector of CES 1
+    wl = dataclasses.replace(wl, invtt=inv)
+    print("case: templates, detector weights of zero", flush=True)
+    tm = _otm(wl)
+    with _problem(gpu, wl) as pb:
+        ntot, nt, ranks = pb.templates_info()
+        B, Bi = pb.templates_kernel()
+    oB = tm.kernel()
+    assert np.array_equal(B, oB)
+    oBi, oranks = tm.invkernel(oB)
+    assert np.array_equal(ranks, oranks)
+    assert ranks[0] == 0 and all(r == 0 for r in ranks[3:6]) and all(r > 0 for r in ranks[6:])
+    for k, (g, o) in enumerate(zip(tm.blocks(Bi), tm.blocks(oBi))):
+        if not np.any(o):
+            assert not np.any(g), f"block {k}: nonzero pseudo-inverse of a zero kernel"
+        else:
+            assert np.abs(g - o).max() <= 1e-8 * np.abs(o).max(), f"block {k}"
 
 
 @pytest.mark.parametrize("precond,zn", [(0, 0), (1, 2)])
