@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--c4-real", type=int, default=32, help="batched Monte Carlo realizations (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=50, help="requests per serving rate (0: skip)")
     ap.add_argument("--serve-rates", default="1,2,5,10,20,50", help="Poisson arrival rates (req/s)")
-    ap.add_argument("--serve-pool", type=int, default=8, help="distinct seeded requests cycled by the load")
+    ap.add_argument("--serve-pool", type=int, default=16, help="distinct seeded requests cycled by the load")
     ap.add_argument("--no-noise-model", action="store_true", help="skip the invtt-from-noise section")
     ap.add_argument("--no-templates", action="store_true", help="skip the template map-making (MTmap) section")
     ap.add_argument("--twolvl-zn", type=int, default=8,
